@@ -1,0 +1,199 @@
+"""coupe_amd -- coupe's Recursive Coordinate Bisection on AMD Instinct MI355X.
+
+Host-side mirror of coupe's RCB interface (coupe/src/lib.rs:76-91,
+coupe/src/algorithms/recursive_bisection.rs:779-812):
+
+    >>> import numpy as np, coupe_amd
+    >>> points = np.array([[1., 1.], [-1., 1.], [1., -1.], [-1., -1.]])
+    >>> weights = np.ones(4, dtype=np.int32)
+    >>> part_ids = np.zeros(4, dtype=np.uint64)
+    >>> coupe_amd.Rcb(iter_count=2).partition(part_ids, (points, weights))
+
+``partition`` writes the part ids into ``part_ids`` like coupe's
+``&mut [usize]`` and raises :class:`InputLenMismatch` like coupe's
+``Error::InputLenMismatch``.  Host (numpy) inputs go through the C entry point
+``coupe_rcb``; torch tensors already on the GPU go through
+``coupe_hip_rcb_device`` with no PCIe copy.  The computation always runs in
+the HIP library; nothing here computes ids.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Any, Tuple
+
+import numpy as np
+
+from . import _native
+from ._native import strerror
+
+__all__ = ["Rcb", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror"]
+
+
+class Error(Exception):
+    """coupe::Error (coupe/src/algorithms.rs:45-73)."""
+
+
+class InputLenMismatch(Error):
+    """Input sets don't have matching lengths (coupe::Error::InputLenMismatch)."""
+
+    def __init__(self, expected: int, actual: int):
+        super().__init__(f"input sets don't have the same length (expected {expected} items, got {actual})")
+        self.expected = expected
+        self.actual = actual
+
+
+class CoupeError(Error):
+    """A status code of the C ABI other than the ones coupe::Error models."""
+
+    def __init__(self, code: int):
+        super().__init__(f"{strerror(code)} (coupe_err {code})")
+        self.code = code
+
+
+class Partition:
+    """coupe::Partition: ``partition(part_ids, data) -> metadata``."""
+
+    def partition(self, part_ids, data):  # pragma: no cover - interface
+        raise NotImplementedError
+
+
+_WTYPES = {np.dtype(np.int32): _native.COUPE_INT, np.dtype(np.int64): _native.COUPE_INT64,
+           np.dtype(np.float64): _native.COUPE_DOUBLE}
+
+
+def _is_device_tensor(x) -> bool:
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
+def _weight_type(dtype) -> int:
+    key = np.dtype(str(dtype).replace("torch.", ""))
+    if key not in _WTYPES:
+        raise TypeError(f"weights must be int32, int64 or float64, got {dtype}")
+    return _WTYPES[key]
+
+
+class DeviceContext:
+    """Owns a coupe_hip_ctx (device work buffers reused across calls)."""
+
+    def __init__(self, device: int = 0):
+        self._ctx = _native.lib().coupe_hip_ctx_create(int(device))
+        if not self._ctx:
+            raise RuntimeError(f"no usable HIP device {device}")
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def stats(self) -> dict:
+        st = _native.Stats()
+        _native.lib().coupe_hip_last_stats(self._ctx, C.byref(st))
+        return st.as_dict()
+
+    def set_profiling(self, enable: bool):
+        _native.lib().coupe_hip_set_profiling(self._ctx, 1 if enable else 0)
+
+    def kernel_times(self) -> dict:
+        """Per-kernel-class launches, total event-timed ms and algorithmic
+        bytes of the last call (profiling must be enabled)."""
+        n = len(_native.KERNELS)
+        arr = (_native.KernelTime * n)()
+        _native.lib().coupe_hip_kernel_times(self._ctx, arr, n)
+        return {k: {"launches": arr[i].launches, "total_ms": arr[i].total_ms, "algo_bytes": arr[i].algo_bytes}
+                for i, k in enumerate(_native.KERNELS)}
+
+    def close(self):
+        if self._ctx:
+            _native.lib().coupe_hip_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def _context_for(device: int) -> DeviceContext:
+    if device not in _default_ctx:
+        _default_ctx[device] = DeviceContext(device)
+    return _default_ctx[device]
+
+
+@dataclass
+class Rcb(Partition):
+    """coupe::Rcb { iter_count, tolerance } (recursive_bisection.rs:779-792).
+
+    ``iter_count``: number of bisection levels (at most 2^iter_count parts).
+    ``tolerance``: per-split tolerance on the normalised imbalance.
+    """
+
+    iter_count: int = 0
+    tolerance: float = 0.0
+
+    def partition(self, part_ids, data: Tuple[Any, Any], *, stream=None, context: DeviceContext = None):
+        points, weights = data
+        if _is_device_tensor(points) or _is_device_tensor(part_ids):
+            return self._partition_device(part_ids, points, weights, stream, context)
+        return self._partition_host(part_ids, points, weights)
+
+    # -- host memory: coupe_rcb (include/coupe.h) --------------------------------
+    def _partition_host(self, part_ids, points, weights):
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        w = np.ascontiguousarray(weights)
+        if w.dtype == np.bool_ or w.dtype.kind == "u":
+            w = w.astype(np.int64)
+        n_part = len(part_ids)
+        # rcb(): weights first, then points (recursive_bisection.rs:661-672)
+        if w.shape[0] != n_part:
+            raise InputLenMismatch(n_part, w.shape[0])
+        if pts.shape[0] != n_part:
+            raise InputLenMismatch(n_part, pts.shape[0])
+        if not isinstance(part_ids, np.ndarray) or part_ids.dtype != np.uint64 or not part_ids.flags.c_contiguous:
+            raise TypeError("part_ids must be a contiguous numpy uint64 array (coupe's &mut [usize])")
+        if n_part == 0:
+            return None
+        dim = pts.shape[1] if pts.ndim == 2 else 0
+        wtype = _weight_type(w.dtype)
+        L = _native.lib()
+        dp = L.coupe_data_array(n_part, _native.COUPE_DOUBLE, pts.ctypes.data)
+        dw = L.coupe_data_array(n_part, wtype, w.ctypes.data)
+        try:
+            err = L.coupe_rcb(part_ids.ctypes.data, dim, dp, dw, int(self.iter_count), float(self.tolerance))
+        finally:
+            L.coupe_data_free(dp)
+            L.coupe_data_free(dw)
+        if err != _native.OK:
+            raise CoupeError(err)
+        return None
+
+    # -- device memory: coupe_hip_rcb_device (include/coupe_hip.h) --------------
+    def _partition_device(self, part_ids, points, weights, stream, context):
+        import torch
+
+        n_part = part_ids.shape[0]
+        if weights.shape[0] != n_part:
+            raise InputLenMismatch(n_part, weights.shape[0])
+        if points.shape[0] != n_part:
+            raise InputLenMismatch(n_part, points.shape[0])
+        if part_ids.dtype not in (torch.int64, torch.uint64) or not part_ids.is_contiguous():
+            raise TypeError("part_ids must be a contiguous 64-bit integer tensor")
+        if points.dtype != torch.float64 or not points.is_contiguous():
+            raise TypeError("points must be a contiguous float64 tensor of shape (N, D)")
+        if not weights.is_contiguous():
+            raise TypeError("weights must be contiguous")
+        dev = points.device.index or 0
+        ctx = context or _context_for(dev)
+        st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
+        dim = points.shape[1] if points.dim() == 2 else 0
+        wtype = _weight_type(weights.dtype)
+        err = _native.lib().coupe_hip_rcb_device(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), n_part, dim,
+                                                  points.data_ptr(), wtype, weights.data_ptr(),
+                                                  int(self.iter_count), float(self.tolerance))
+        if err != _native.OK:
+            raise CoupeError(err)
+        return None

@@ -1,0 +1,120 @@
+"""ctypes bindings of libcoupe_hip.so (include/coupe.h, include/coupe_hip.h).
+
+The library is loaded from this package directory only; a missing library is
+an error (there is no fallback implementation).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcoupe_hip.so")
+WORKLOADS_PATH = os.path.join(HERE, "libcoupe_workloads.so")
+
+# enum coupe_err (include/coupe.h)
+OK, ALLOC, CRASH, BAD_DIMENSION, BAD_TYPE, BIPART_ONLY, LEN_MISMATCH, NOT_FOUND, NEG_VALUES = range(9)
+# enum coupe_type
+COUPE_INT, COUPE_INT64, COUPE_DOUBLE = 0, 1, 2
+
+# Every symbol the headers declare, in declaration order.
+EXPORTS = [
+    "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
+    "coupe_rcb",
+    "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_last_stats",
+    "coupe_hip_set_profiling", "coupe_hip_kernel_times",
+]
+
+KERNELS = ["prologue", "hist", "decide", "partition", "ids", "other"]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("launches", C.c_uint32), ("total_ms", C.c_double), ("algo_bytes", C.c_double)]
+
+ITH_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t)
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("points", C.c_uint64),
+        ("levels", C.c_uint32),
+        ("passes", C.c_uint32),
+        ("max_passes", C.c_uint32),
+        ("weight_mode", C.c_uint32),
+        ("weight_bytes", C.c_uint32),
+        ("bins", C.c_uint32),
+        ("unit_points", C.c_uint64),
+        ("device_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """The loaded product library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m coupe_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.coupe_strerror.argtypes = [C.c_int]
+        L.coupe_strerror.restype = C.c_char_p
+        L.coupe_data_free.argtypes = [C.c_void_p]
+        L.coupe_data_free.restype = None
+        L.coupe_data_array.argtypes = [C.c_size_t, C.c_int, C.c_void_p]
+        L.coupe_data_array.restype = C.c_void_p
+        L.coupe_data_constant.argtypes = [C.c_size_t, C.c_int, C.c_void_p]
+        L.coupe_data_constant.restype = C.c_void_p
+        L.coupe_data_fn.argtypes = [C.c_void_p, C.c_size_t, C.c_int, ITH_FN]
+        L.coupe_data_fn.restype = C.c_void_p
+        L.coupe_rcb.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double]
+        L.coupe_rcb.restype = C.c_int
+        L.coupe_hip_ctx_create.argtypes = [C.c_int]
+        L.coupe_hip_ctx_create.restype = C.c_void_p
+        L.coupe_hip_ctx_destroy.argtypes = [C.c_void_p]
+        L.coupe_hip_ctx_destroy.restype = None
+        L.coupe_hip_rcb_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
+                                           C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_double]
+        L.coupe_hip_rcb_device.restype = C.c_int
+        L.coupe_hip_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+        L.coupe_hip_last_stats.restype = C.c_int
+        L.coupe_hip_set_profiling.argtypes = [C.c_void_p, C.c_int]
+        L.coupe_hip_set_profiling.restype = C.c_int
+        L.coupe_hip_kernel_times.argtypes = [C.c_void_p, C.POINTER(KernelTime), C.c_int]
+        L.coupe_hip_kernel_times.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def strerror(code: int) -> str:
+    return lib().coupe_strerror(int(code)).decode()
+
+
+_wl = None
+
+
+def workloads_lib():
+    global _wl
+    if _wl is None:
+        if not os.path.exists(WORKLOADS_PATH):
+            raise RuntimeError(f"{WORKLOADS_PATH} is missing: build it with `python -m coupe_amd.build`")
+        L = C.CDLL(WORKLOADS_PATH)
+        P = C.c_void_p
+        L.wl_uniform.argtypes = [P, P, C.c_uint64, C.c_int, C.c_uint64, C.c_int]
+        L.wl_gauss_mix.argtypes = [P, P, C.c_uint64, C.c_uint64, C.c_double]
+        L.wl_jgrid.argtypes = [P, P, C.c_uint64, C.c_uint64, C.c_int]
+        L.wl_box.argtypes = [P, P, C.c_uint64, C.c_int, P, P, C.c_uint64, C.c_int]
+        L.wl_batch_header.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      P, P, P, P, P]
+        L.wl_instance_seed.argtypes = [C.c_uint64, C.c_uint64]
+        L.wl_instance_seed.restype = C.c_uint64
+        L.wl_splitmix64.argtypes = [C.c_uint64, C.c_uint64]
+        L.wl_splitmix64.restype = C.c_uint64
+        L.wl_uniform01.argtypes = [C.c_uint64, C.c_uint64]
+        L.wl_uniform01.restype = C.c_double
+        _wl = L
+    return _wl

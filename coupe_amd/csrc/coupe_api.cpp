@@ -1,0 +1,187 @@
+// coupe_api.cpp -- the C ABI of include/coupe.h and include/coupe_hip.h.
+//
+// coupe_rcb keeps coupe's host-memory contract (coupe-ffi/src/lib.rs:272-304):
+// the data sets are read on the host, staged to HBM, partitioned on the
+// current HIP device, and the ids copied back into the caller's array.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "rcb_engine.h"
+
+using coupe_hip::Context;
+
+// Data sets (coupe-ffi/src/data.rs:11-174): borrowed array, repeated value,
+// or per-index callback.
+struct coupe_data {
+    enum Kind { Array, Constant, Fn } kind;
+    uintptr_t len;
+    enum coupe_type type;
+    const void *ptr;   // array or value
+    const void *context;
+    const void *(*i_th)(const void *, uintptr_t);
+};
+
+extern "C" {
+
+const char *coupe_strerror(enum coupe_err err) {
+    // Same strings as coupe-ffi/src/lib.rs:69-110.
+    switch (err) {
+        case COUPE_ERR_OK: return "success";
+        case COUPE_ERR_ALLOC: return "allocation failed";
+        case COUPE_ERR_CRASH: return "coupe encountered a bug and crashed";
+        case COUPE_ERR_BAD_DIMENSION: return "this algorithm does not support the given mesh dimension";
+        case COUPE_ERR_BAD_TYPE: return "this algorithm does not support the given type";
+        case COUPE_ERR_BIPART_ONLY: return "this algorithm does not support k-way partitioning";
+        case COUPE_ERR_LEN_MISMATCH: return "input iters (e.g. weights and points) don't have the same length";
+        case COUPE_ERR_NOT_FOUND: return "no partition has been found for the given constraints";
+        case COUPE_ERR_NEG_VALUES: return "this algorithm does not support negative values";
+    }
+    return "unknown error";
+}
+
+void coupe_data_free(coupe_data *data) { delete data; }
+
+coupe_data *coupe_data_array(uintptr_t len, enum coupe_type type, const void *data) {
+    return new (std::nothrow) coupe_data{coupe_data::Array, len, type, data, nullptr, nullptr};
+}
+
+coupe_data *coupe_data_constant(uintptr_t len, enum coupe_type type, const void *value) {
+    return new (std::nothrow) coupe_data{coupe_data::Constant, len, type, value, nullptr, nullptr};
+}
+
+coupe_data *coupe_data_fn(const void *context, uintptr_t len, enum coupe_type type,
+                          const void *(*i_th)(const void *, uintptr_t)) {
+    return new (std::nothrow) coupe_data{coupe_data::Fn, len, type, nullptr, context, i_th};
+}
+
+}  // extern "C"
+
+namespace {
+
+size_t type_size(enum coupe_type t) { return t == COUPE_INT ? sizeof(int) : 8; }
+
+// Host view of `n` elements of `elem` bytes: the caller's array when there is
+// one, else a materialised copy in `store`.
+const void *materialise(const coupe_data *d, size_t elem, std::vector<unsigned char> &store) {
+    if (d->kind == coupe_data::Array) return d->ptr;
+    store.resize(d->len * elem);
+    unsigned char *out = store.data();
+    if (d->kind == coupe_data::Constant) {
+        for (uintptr_t i = 0; i < d->len; i++) memcpy(out + i * elem, d->ptr, elem);
+    } else {
+        for (uintptr_t i = 0; i < d->len; i++) memcpy(out + i * elem, d->i_th(d->context, i), elem);
+    }
+    return out;
+}
+
+std::mutex g_default_lock;
+Context *g_default[64];
+
+Context *default_context() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> g(g_default_lock);
+    if (!g_default[dev]) {
+        Context *c = new (std::nothrow) Context();
+        if (!c) return nullptr;
+        c->device = dev;
+        g_default[dev] = c;
+    }
+    return g_default[dev];
+}
+
+}  // namespace
+
+extern "C" enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension, const coupe_data *points,
+                                    const coupe_data *weights, uintptr_t iter_count, double tolerance) {
+    const uintptr_t n = points->len;
+    if (n != weights->len) return COUPE_ERR_LEN_MISMATCH;   // lib.rs:285-288
+    if (dimension != 2 && dimension != 3) return COUPE_ERR_BAD_DIMENSION;   // lib.rs:297-301
+    if (n == 0) return COUPE_ERR_OK;
+    if (weights->type != COUPE_INT && weights->type != COUPE_INT64 && weights->type != COUPE_DOUBLE)
+        return COUPE_ERR_BAD_TYPE;
+    Context *cx = default_context();
+    if (!cx) return COUPE_ERR_CRASH;
+    std::lock_guard<std::mutex> g(cx->lock);
+    try {
+        std::vector<unsigned char> pstore, wstore;
+        const void *hp = materialise(points, dimension * sizeof(double), pstore);
+        const size_t wsz = type_size(weights->type);
+        const void *hw = materialise(weights, wsz, wstore);
+        const size_t pbytes = n * dimension * sizeof(double), wbytes = n * wsz, obytes = n * sizeof(uint64_t);
+        cx->in_points.ensure(pbytes);
+        cx->in_weights.ensure(wbytes);
+        cx->out_ids.ensure(obytes);
+        hipStream_t st = nullptr;
+        hipError_t e;
+        if ((e = hipMemcpy(cx->in_points.ptr, hp, pbytes, hipMemcpyHostToDevice)) != hipSuccess)
+            return coupe_hip::map_hip_error(e);
+        if ((e = hipMemcpy(cx->in_weights.ptr, hw, wbytes, hipMemcpyHostToDevice)) != hipSuccess)
+            return coupe_hip::map_hip_error(e);
+        enum coupe_err r = coupe_hip::rcb_device(*cx, st, (uint64_t *)cx->out_ids.ptr, n, dimension,
+                                                 (const double *)cx->in_points.ptr, weights->type,
+                                                 cx->in_weights.ptr, iter_count, tolerance);
+        if (r != COUPE_ERR_OK) return r;
+        if ((e = hipMemcpy(partition, cx->out_ids.ptr, obytes, hipMemcpyDeviceToHost)) != hipSuccess)
+            return coupe_hip::map_hip_error(e);
+    } catch (const std::bad_alloc &) {
+        return COUPE_ERR_ALLOC;
+    } catch (...) {
+        return COUPE_ERR_ALLOC;   // DevBuf::ensure failures
+    }
+    return COUPE_ERR_OK;
+}
+
+extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return nullptr;
+    Context *c = new (std::nothrow) Context();
+    if (!c) return nullptr;
+    c->device = device;
+    return reinterpret_cast<coupe_hip_ctx *>(c);
+}
+
+extern "C" void coupe_hip_ctx_destroy(coupe_hip_ctx *ctx) {
+    if (!ctx) return;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    (void)hipSetDevice(c->device);
+    c->release_all();
+    delete c;
+}
+
+extern "C" enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition, uintptr_t n,
+                                               uintptr_t dimension, const double *d_points,
+                                               enum coupe_type weight_type, const void *d_weights,
+                                               uintptr_t iter_count, double tolerance) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    return coupe_hip::rcb_device(*c, (hipStream_t)stream, d_partition, n, dimension, d_points, weight_type,
+                                 d_weights, iter_count, tolerance);
+}
+
+extern "C" enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out) {
+    if (!ctx || !out) return COUPE_ERR_CRASH;
+    *out = reinterpret_cast<const Context *>(ctx)->stats;
+    return COUPE_ERR_OK;
+}
+
+extern "C" enum coupe_err coupe_hip_set_profiling(coupe_hip_ctx *ctx, int enable) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    c->prof.enabled = enable != 0;
+    return COUPE_ERR_OK;
+}
+
+extern "C" enum coupe_err coupe_hip_kernel_times(const coupe_hip_ctx *ctx, struct coupe_hip_kernel_time *out,
+                                                 int count) {
+    if (!ctx || !out) return COUPE_ERR_CRASH;
+    const Context *c = reinterpret_cast<const Context *>(ctx);
+    for (int k = 0; k < count && k < COUPE_HIP_K_COUNT; k++) out[k] = c->prof.out[k];
+    return COUPE_ERR_OK;
+}

@@ -1,0 +1,89 @@
+// rcb_common.h -- shared types of the MI355X RCB engine (device and host).
+//
+// Vocabulary follows coupe: a *part* (segment) is a node of the bisection
+// tree, i.e. a contiguous range of the working arrays; a *level* is one depth
+// of the tree; a *pass* is one histogram sweep that evaluates several split
+// candidates of every part still searching; a *unit* is the range of one
+// part that one workgroup sweeps.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace coupe_hip {
+
+constexpr int kMaxDim = 3;
+constexpr int kBlock = 256;          // threads per workgroup of the sweeps
+constexpr int kItems = 8;            // points per thread per partition tile
+constexpr int kTile = kBlock * kItems;
+constexpr int kMaxBins = 256;        // histogram bins per part per pass
+
+// Part status during a level.
+enum : uint32_t {
+    kSearching = 0,   // split position not found yet
+    kSplitAtT = 1,    // split found: left = {c < t}
+    kAllLeft = 2,     // no point at or right of t twice: every point goes left
+};
+
+// One part of the current level.  `sum` and `wl` hold the bits of the
+// accumulator type (uint64 two's complement or double).
+struct Part {
+    uint64_t begin;        // first position in the working arrays
+    uint64_t count;        // points in the part
+    uint64_t heap_id;      // iter_id of coupe's recursion (root 0, children 2i+1 / 2i+2)
+    uint64_t sum;          // weight of the part, accumulator bits
+    float lo[kMaxDim];     // inherited box, f32 (coupe casts its f64 box at use)
+    float hi[kMaxDim];
+    // split search state (par_rcb_split's min, max, prev_count_left)
+    float blo, bhi;
+    uint64_t prev;
+    uint32_t axis;
+    uint32_t status;
+    float split;           // split_pos
+    int32_t kstar;         // in-order index of the stopping candidate in the last pass
+    uint64_t wl;           // weight_left, accumulator bits
+    uint64_t cnt_left;     // points going left
+    uint32_t unit_begin;   // first unit of this part in the unit table
+    uint32_t unit_count;
+};
+
+// A unit: the range [begin, begin + count) of part `part`.
+struct Unit {
+    uint64_t begin;
+    uint32_t count;
+    uint32_t part;
+};
+
+// Results of the prologue reductions.
+struct Prologue {
+    uint64_t bmin[kMaxDim];     // ordered keys of the f64 bounding box
+    uint64_t bmax[kMaxDim];
+    uint64_t sum_u64;           // wrapping integer sum of the weights
+    double sum_f64;             // f64 sum (real weights)
+    double sum_abs;             // sum of |w| (exactness guard)
+    uint64_t max_abs_bits;      // bits of max |w| (non-negative doubles order as integers)
+    uint32_t non_integral;      // any weight with a fractional part or |w| > 2^53
+    uint32_t not_i32;           // any weight outside int32
+};
+
+// Storage / accumulation modes chosen after the prologue.
+enum WeightStore : int { kStoreI32 = 0, kStoreI64 = 1, kStoreF64 = 2 };
+
+__host__ __device__ inline uint32_t f32_key(float x) {
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float key_f32(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+__host__ __device__ inline uint64_t f64_key(double x) {
+    uint64_t u = __builtin_bit_cast(uint64_t, x);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double key_f64(uint64_t k) {
+    uint64_t u = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __builtin_bit_cast(double, u);
+}
+
+}  // namespace coupe_hip

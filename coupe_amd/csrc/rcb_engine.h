@@ -1,0 +1,70 @@
+// rcb_engine.h -- context and entry points of the RCB engine (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/coupe.h"
+#include "../../include/coupe_hip.h"
+
+namespace coupe_hip {
+
+// A grow-only device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t need);   // throws HipFailure on allocation failure
+    void release();
+};
+
+// Opt-in per-launch timing with HIP events on the launch stream.
+struct Profiler {
+    bool enabled = false;
+    struct Rec {
+        int kind;
+        int ev;          // index of the start event; stop is ev + 1
+        double bytes;    // algorithmic bytes of this launch
+    };
+    std::vector<hipEvent_t> events;
+    std::vector<Rec> recs;
+    coupe_hip_kernel_time out[COUPE_HIP_K_COUNT]{};
+    void reset();
+    int begin(hipStream_t st, int kind, double bytes);
+    void end(hipStream_t st, int rec);
+    void collect();      // after the stream is synchronised
+    void release();
+};
+
+struct Context {
+    int device = 0;
+    std::mutex lock;                 // one call at a time per context
+    DevBuf cx[2][3];                 // SoA f32 coordinates, two sides
+    DevBuf w[2];                     // stored weights, two sides
+    DevBuf ix[2];                    // original indices, two sides
+    DevBuf parts[2], units[2];       // part / unit tables, current and next level
+    DevBuf bcnt, bw, bmin;           // per-part histogram bins
+    DevBuf ucnt;                     // per-unit bin counts (partition offsets)
+    DevBuf uleft, nchild, nunits;    // scan inputs / outputs
+    DevBuf bsum;                     // scan block sums
+    DevBuf prologue, counters;
+    DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
+    void *host_scratch = nullptr;    // pinned, for small readbacks
+    bool bins_clean = false;
+    double swept_bytes = 0.0;        // HIST algorithmic bytes of the last call
+    coupe_hip_stats stats{};
+    Profiler prof;
+    uint64_t device_bytes() const;
+    void release_all();
+};
+
+enum coupe_err map_hip_error(hipError_t e);
+
+// RCB on device-resident inputs (see coupe_hip_rcb_device).
+enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
+                          const double *d_points, int wkind, const void *d_weights, size_t iter_count,
+                          double tol);
+
+}  // namespace coupe_hip

@@ -1,0 +1,451 @@
+// rcb_engine.hip -- host side of the MI355X RCB engine: device buffers of a
+// context, weight-mode selection, and the level loop that drives the kernels
+// of rcb_kernels.h.  Entry point: coupe_hip_rcb_device (include/coupe_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <type_traits>
+
+#include "rcb_engine.h"
+#include "rcb_kernels.h"
+
+namespace coupe_hip {
+
+// ---------------------------------------------------------------------------
+// Errors: any HIP failure unwinds to the entry point, which maps it onto
+// coupe's codes (out of memory -> COUPE_ERR_ALLOC, anything else -> CRASH).
+// ---------------------------------------------------------------------------
+struct HipFailure {
+    hipError_t err;
+};
+
+static void check(hipError_t e) {
+    if (e != hipSuccess) throw HipFailure{e};
+}
+
+enum coupe_err map_hip_error(hipError_t e) {
+    return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? COUPE_ERR_ALLOC : COUPE_ERR_CRASH;
+}
+
+void DevBuf::ensure(size_t need) {
+    if (need <= bytes) return;
+    if (ptr) {
+        check(hipFree(ptr));
+        ptr = nullptr;
+        bytes = 0;
+    }
+    size_t sz = need < 256 ? 256 : need;
+    check(hipMalloc(&ptr, sz));
+    bytes = sz;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+void Profiler::reset() {
+    recs.clear();
+    for (auto &o : out) o = coupe_hip_kernel_time{};
+}
+
+int Profiler::begin(hipStream_t st, int kind, double bytes) {
+    if (!enabled) return -1;
+    int ev = (int)recs.size() * 2;
+    while ((int)events.size() < ev + 2) {
+        hipEvent_t e;
+        check(hipEventCreate(&e));
+        events.push_back(e);
+    }
+    check(hipEventRecord(events[ev], st));
+    recs.push_back(Rec{kind, ev, bytes});
+    return (int)recs.size() - 1;
+}
+
+void Profiler::end(hipStream_t st, int rec) {
+    if (rec < 0) return;
+    check(hipEventRecord(events[recs[rec].ev + 1], st));
+}
+
+void Profiler::collect() {
+    for (auto &o : out) o = coupe_hip_kernel_time{};
+    if (!enabled) return;
+    for (const Rec &r : recs) {
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, events[r.ev], events[r.ev + 1]));
+        out[r.kind].launches++;
+        out[r.kind].total_ms += ms;
+        out[r.kind].algo_bytes += r.bytes;
+    }
+}
+
+void Profiler::release() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    events.clear();
+    recs.clear();
+}
+
+// Runs `launch` between two profiling events when profiling is on.
+template <typename F>
+static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launch) {
+    int r = cx.prof.begin(st, kind, bytes);
+    launch();
+    check(hipGetLastError());
+    cx.prof.end(st, r);
+}
+
+// Small device counters shared by the kernels of one call.
+struct Counters {
+    uint32_t n_parts[2];
+    uint32_t n_units[2];
+    uint32_t searching;
+    uint32_t pad;
+    uint64_t parts_total;
+    uint64_t units_total;
+    uint64_t min_id;
+    unsigned long long swept;   // points swept by histogram passes
+};
+
+static inline unsigned grid_for(uint64_t items, unsigned per_block) {
+    uint64_t g = (items + per_block - 1) / per_block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Exclusive scan of `n_dev` (device) uint64 values, in place, upper bound n_ub.
+static void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub,
+                     uint64_t *total) {
+    uint64_t nblocks = (n_ub + kScanTile - 1) / kScanTile;
+    if (nblocks < 1) nblocks = 1;
+    cx.bsum.ensure(nblocks * sizeof(uint64_t));
+    uint64_t *bsum = (uint64_t *)cx.bsum.ptr;
+    timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev);
+        k_scan_top<<<1, kBlock, 0, st>>>(bsum, n_dev, total);
+        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev);
+    });
+}
+
+// Points per unit: about 2048 units at level 0 (8 workgroups per CU on 256
+// CUs), in whole partition tiles.
+static uint64_t choose_unit(uint64_t n) {
+    uint64_t u = (n + 2047) / 2048;
+    u = (u + kTile - 1) / kTile * kTile;
+    if (u < (uint64_t)kTile) u = kTile;
+    if (u > (1ull << 20)) u = 1ull << 20;
+    return u;
+}
+
+// Histogram bins per part, bounded by the bin buffers.
+static int choose_bins(uint64_t parts_ub, uint64_t units_ub) {
+    const uint64_t cap = 1ull << 23;   // entries of each bin array
+    int nb = kMaxBins;
+    while (nb > 2 && (parts_ub * nb > cap || units_ub * nb > cap)) nb >>= 1;
+    return nb;
+}
+
+template <int D, typename WS, typename Acc>
+static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_count, double tol, int wrap32,
+                       uint64_t *d_out, WS *w_first) {
+    const uint64_t unit = choose_unit(n);
+    Counters *ctr = (Counters *)cx.counters.ptr;
+    uint32_t *h_search = (uint32_t *)cx.host_scratch;
+
+    // Table capacities: parts at depth l <= min(2^l, n); units <= parts + n / unit.
+    uint64_t parts_cap = 1;
+    for (size_t l = 0; l < iter_count && parts_cap < n; l++) parts_cap *= 2;
+    if (parts_cap > n) parts_cap = n;
+    const uint64_t units_cap = parts_cap + (n + unit - 1) / unit;
+    for (int b = 0; b < 2; b++) {
+        cx.parts[b].ensure(parts_cap * sizeof(Part));
+        cx.units[b].ensure(units_cap * sizeof(Unit));
+    }
+    cx.uleft.ensure(units_cap * sizeof(uint64_t));
+    cx.nchild.ensure(parts_cap * sizeof(uint64_t));
+    cx.nunits.ensure(parts_cap * sizeof(uint64_t));
+    // Bin buffers: the largest parts x bins and units x bins over all levels
+    // (each level picks its bin count from its own table bounds).
+    uint64_t bins = 1, ucnt_entries = 1;
+    {
+        uint64_t pub = 1;
+        for (size_t level = 0; level < iter_count; level++) {
+            const uint64_t uub = std::min<uint64_t>(units_cap, pub + (n + unit - 1) / unit);
+            const int nbl = choose_bins(pub, uub);
+            bins = std::max<uint64_t>(bins, pub * nbl);
+            ucnt_entries = std::max<uint64_t>(ucnt_entries, uub * nbl);
+            if (pub >= parts_cap) break;   // later levels repeat this bound
+            pub = std::min<uint64_t>(2 * pub, parts_cap);
+        }
+    }
+    const bool fresh_bins = cx.bcnt.bytes < bins * 4 || cx.bw.bytes < bins * 8 || cx.bmin.bytes < bins * 4;
+    cx.bcnt.ensure(bins * 4);
+    cx.bw.ensure(bins * 8);
+    cx.bmin.ensure(bins * 4);
+    if (fresh_bins || !cx.bins_clean) {
+        // k_decide leaves bins zeroed after reading them; only new or dirty
+        // buffers need a clear.
+        check(hipMemsetAsync(cx.bcnt.ptr, 0, cx.bcnt.bytes, st));
+        check(hipMemsetAsync(cx.bw.ptr, 0, cx.bw.bytes, st));
+        check(hipMemsetAsync(cx.bmin.ptr, 0xff, cx.bmin.bytes, st));
+        cx.bins_clean = true;
+    }
+    cx.ucnt.ensure(ucnt_entries * 4);
+
+    // Working arrays: slot 0 holds the prologue output, slot 1 the other side.
+    float *cs[2][3];
+    WS *ws[2];
+    uint32_t *ixs[2];
+    for (int b = 0; b < 2; b++) {
+        for (int k = 0; k < 3; k++) cs[b][k] = k < D ? (float *)cx.cx[b][k].ptr : nullptr;
+        ixs[b] = (uint32_t *)cx.ix[b].ptr;
+    }
+    ws[0] = w_first;
+    ws[1] = (WS *)cx.w[1].ptr;
+
+    const int real_mode = std::is_same<Acc, double>::value;
+    const double payload = 4.0 * D + sizeof(WS) + 4.0;   // bytes of one point in the working arrays
+    check(hipMemsetAsync(&ctr->swept, 0, sizeof(ctr->swept), st));
+    timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+        k_init_root<Acc><<<grid_for(units_cap, kBlock), kBlock, 0, st>>>(
+            (Part *)cx.parts[0].ptr, &ctr->n_parts[0], (Unit *)cx.units[0].ptr, &ctr->n_units[0],
+            (const Prologue *)cx.prologue.ptr, n, D, unit, real_mode);
+    });
+
+    int cur = 0;
+    uint64_t parts_ub = 1;
+    cx.stats.levels = 0;
+    cx.stats.passes = 0;
+    cx.stats.max_passes = 0;
+    cx.stats.unit_points = unit;
+    for (size_t level = 0; level < iter_count; level++) {
+        const int nxt = cur ^ 1;
+        Part *parts = (Part *)cx.parts[cur].ptr;
+        Unit *units = (Unit *)cx.units[cur].ptr;
+        uint32_t *np = &ctr->n_parts[cur];
+        uint32_t *nu = &ctr->n_units[cur];
+        const uint64_t units_ub = std::min<uint64_t>(units_cap, parts_ub + (n + unit - 1) / unit);
+        const int nb = choose_bins(parts_ub, units_ub);
+        cx.stats.bins = nb;
+
+        // Split search: passes until no part is searching.
+        uint32_t passes = 0;
+        for (;;) {
+            check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
+            // bytes of HIST launches are set from the swept-points counter at the end
+            timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
+                k_hist<WS, Acc><<<(unsigned)units_ub, kBlock, 0, st>>>(
+                    parts, units, nu, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur], nb, (uint32_t *)cx.bcnt.ptr,
+                    (Acc *)cx.bw.ptr, (uint32_t *)cx.bmin.ptr, (uint32_t *)cx.ucnt.ptr);
+            });
+            timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                    parts, np, (uint32_t *)cx.bcnt.ptr, (Acc *)cx.bw.ptr, (uint32_t *)cx.bmin.ptr, nb, tol,
+                    wrap32, &ctr->searching, &ctr->swept);
+            });
+            passes++;
+            check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            check(hipStreamSynchronize(st));
+            if (*h_search == 0) break;
+            // f32 bisection ends within a few hundred candidates; b >= 1 per pass
+            if (passes >= 4096) throw HipFailure{hipErrorUnknown};
+        }
+        cx.stats.passes += passes;
+        cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+        cx.stats.levels++;
+
+        if (level + 1 < iter_count) {
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_unit_left<<<grid_for(units_ub, kBlock), kBlock, 0, st>>>(
+                    parts, units, nu, (const uint32_t *)cx.ucnt.ptr, nb, (uint64_t *)cx.uleft.ptr);
+            });
+            scan_u64(cx, st, (uint64_t *)cx.uleft.ptr, nu, units_ub, nullptr);
+            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                k_partition<D, WS><<<(unsigned)units_ub, kBlock, 0, st>>>(
+                    parts, units, nu, (const uint64_t *)cx.uleft.ptr, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur],
+                    ixs[cur], cs[nxt][0], cs[nxt][1], cs[nxt][2], ws[nxt], ixs[nxt], n);
+            });
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_children_count<Acc><<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                    parts, np, unit, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.nunits.ptr);
+            });
+            scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
+            scan_u64(cx, st, (uint64_t *)cx.nunits.ptr, np, parts_ub, &ctr->units_total);
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                    parts, np, D, unit, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.nunits.ptr,
+                    (Part *)cx.parts[nxt].ptr, (Unit *)cx.units[nxt].ptr);
+                k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->units_total, &ctr->n_parts[nxt],
+                                              &ctr->n_units[nxt]);
+            });
+            cur = nxt;
+            parts_ub = std::min<uint64_t>(2 * parts_ub, parts_cap);
+        } else {
+            check(hipMemsetAsync(&ctr->min_id, 0xff, sizeof(uint64_t), st));
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_leaf_min<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &ctr->min_id);
+            });
+            // reads the axis coordinate and the index, writes the 8-byte id
+            timed(cx, st, COUPE_HIP_K_IDS, 16.0 * (double)n, [&] {
+                k_write_ids<<<(unsigned)units_ub, kBlock, 0, st>>>(parts, units, nu, cs[cur][0], cs[cur][1],
+                                                                 cs[cur][2], ixs[cur], &ctr->min_id, d_out, n);
+            });
+        }
+    }
+    // HIST bytes: the swept points times (axis coordinate + stored weight).
+    unsigned long long *h_swept = (unsigned long long *)((char *)cx.host_scratch + 64);
+    check(hipMemcpyAsync(h_swept, &ctr->swept, sizeof(*h_swept), hipMemcpyDeviceToHost, st));
+    check(hipStreamSynchronize(st));
+    cx.swept_bytes = (double)*h_swept * (4.0 + sizeof(WS));
+}
+
+template <int D, typename WIN>
+static void prologue(Context &cx, hipStream_t st, const double *pts, const WIN *win, uint64_t n, Prologue *h) {
+    Prologue init;
+    memset(&init, 0, sizeof init);
+    for (int k = 0; k < kMaxDim; k++) {
+        init.bmin[k] = ~0ull;
+        init.bmax[k] = 0;
+    }
+    check(hipMemcpyAsync(cx.prologue.ptr, &init, sizeof init, hipMemcpyHostToDevice, st));
+    unsigned grid = (unsigned)std::min<uint64_t>(grid_for(n, kBlock), 4096);
+    // reads AoS f64 points and the input weights; writes SoA f32, index, int32 weight
+    const double bytes = (double)n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0 + 4.0);
+    timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
+        k_prologue<D, WIN><<<grid, kBlock, 0, st>>>(
+            pts, win, n, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
+            (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr), (uint32_t *)cx.ix[0].ptr, (int32_t *)cx.w[0].ptr,
+            (Prologue *)cx.prologue.ptr);
+    });
+    check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
+    check(hipStreamSynchronize(st));
+}
+
+template <int D, typename WIN>
+static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, const double *pts, const WIN *win,
+                      int wkind, size_t iter_count, double tol) {
+    // Working arrays for slot 0 (prologue output) and slot 1.
+    for (int b = 0; b < 2; b++) {
+        for (int k = 0; k < D; k++) cx.cx[b][k].ensure(n * sizeof(float));
+        cx.ix[b].ensure(n * sizeof(uint32_t));
+    }
+    cx.w[0].ensure(n * sizeof(int32_t));
+    cx.prologue.ensure(sizeof(Prologue));
+    cx.counters.ensure(sizeof(Counters));
+    Prologue h;
+    prologue<D, WIN>(cx, st, pts, win, n, &h);
+
+    if (iter_count == 0) {   // every point in part 0 (:589-601, offset 0)
+        timed(cx, st, COUPE_HIP_K_IDS, 8.0 * (double)n, [&] {
+            k_fill_u64<<<std::min<unsigned>(grid_for(n, kBlock), 4096), kBlock, 0, st>>>(d_out, n, 0);
+        });
+        check(hipStreamSynchronize(st));
+        cx.stats.levels = 0;
+        return;
+    }
+
+    // Weight mode.  Integer kinds and integer-valued doubles whose partial
+    // sums stay exact in f64 (|sum| <= 2^53) use exact int64 sums: the same
+    // bits as coupe's sums in any order.  Other doubles use f64 sums.
+    const bool wrap32 = wkind == COUPE_INT;
+    bool integral = wkind != COUPE_DOUBLE ||
+                    (!h.non_integral && h.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
+    bool fits32 = !h.not_i32;
+    unsigned grid = std::min<unsigned>(grid_for(n, kBlock), 4096);
+    if (integral && fits32) {
+        cx.stats.weight_mode = 0;
+        cx.stats.weight_bytes = 4;
+        cx.w[1].ensure(n * sizeof(int32_t));
+        run_levels<D, int32_t, uint64_t>(cx, st, n, iter_count, tol, wrap32, d_out, (int32_t *)cx.w[0].ptr);
+    } else if (integral) {
+        cx.stats.weight_mode = 0;
+        cx.stats.weight_bytes = 8;
+        cx.w[0].ensure(n * sizeof(int64_t));
+        cx.w[1].ensure(n * sizeof(int64_t));
+        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+            k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
+        });
+        run_levels<D, int64_t, uint64_t>(cx, st, n, iter_count, tol, wrap32, d_out, (int64_t *)cx.w[0].ptr);
+    } else {
+        cx.stats.weight_mode = 1;
+        cx.stats.weight_bytes = 8;
+        cx.w[0].ensure(n * sizeof(double));
+        cx.w[1].ensure(n * sizeof(double));
+        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+            k_store_weights<WIN, double><<<grid, kBlock, 0, st>>>(win, (double *)cx.w[0].ptr, n);
+        });
+        run_levels<D, double, double>(cx, st, n, iter_count, tol, 0, d_out, (double *)cx.w[0].ptr);
+    }
+    check(hipStreamSynchronize(st));
+}
+
+template <int D>
+static void run_dim(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, const double *pts, int wkind,
+                    const void *w, size_t iter_count, double tol) {
+    switch (wkind) {
+        case COUPE_INT:
+            run_typed<D, int32_t>(cx, st, d_out, n, pts, (const int32_t *)w, wkind, iter_count, tol);
+            break;
+        case COUPE_INT64:
+            run_typed<D, int64_t>(cx, st, d_out, n, pts, (const int64_t *)w, wkind, iter_count, tol);
+            break;
+        default:
+            run_typed<D, double>(cx, st, d_out, n, pts, (const double *)w, wkind, iter_count, tol);
+            break;
+    }
+}
+
+enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
+                          const double *d_points, int wkind, const void *d_weights, size_t iter_count,
+                          double tol) {
+    if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
+    if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
+    memset(&cx.stats, 0, sizeof cx.stats);
+    cx.stats.points = n;
+    cx.prof.reset();
+    cx.swept_bytes = 0.0;
+    if (n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
+    if (n > 0xffffffffull) return COUPE_ERR_ALLOC;   // original indices are 32-bit
+    try {
+        check(hipSetDevice(cx.device));
+        if (!cx.host_scratch) check(hipHostMalloc(&cx.host_scratch, 4096, hipHostMallocDefault));
+        if (dim == 2) run_dim<2>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
+        else run_dim<3>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
+        cx.prof.collect();
+        cx.prof.out[COUPE_HIP_K_HIST].algo_bytes = cx.swept_bytes;
+    } catch (const HipFailure &f) {
+        cx.bins_clean = false;
+        (void)hipGetLastError();
+        return map_hip_error(f.err);
+    } catch (const std::bad_alloc &) {
+        cx.bins_clean = false;
+        return COUPE_ERR_ALLOC;
+    }
+    cx.stats.device_bytes = cx.device_bytes();
+    return COUPE_ERR_OK;
+}
+
+uint64_t Context::device_bytes() const {
+    uint64_t b = 0;
+    const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
+                           &ix[0], &ix[1], &parts[0], &parts[1], &units[0], &units[1], &bcnt, &bw, &bmin,
+                           &ucnt, &uleft, &nchild, &nunits, &bsum, &prologue, &counters, &in_points,
+                           &in_weights, &out_ids};
+    for (const DevBuf *d : all) b += d->bytes;
+    return b;
+}
+
+void Context::release_all() {
+    DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
+                     &ix[0], &ix[1], &parts[0], &parts[1], &units[0], &units[1], &bcnt, &bw, &bmin,
+                     &ucnt, &uleft, &nchild, &nunits, &bsum, &prologue, &counters, &in_points,
+                     &in_weights, &out_ids};
+    for (DevBuf *d : all) d->release();
+    if (host_scratch) (void)hipHostFree(host_scratch);
+    host_scratch = nullptr;
+    prof.release();
+}
+
+}  // namespace coupe_hip

@@ -1,0 +1,91 @@
+/*
+ * coupe_hip.h -- device-resident entry points of the MI355X RCB library.
+ *
+ * coupe.h keeps coupe's host-memory C ABI.  This header adds the calls a
+ * GPU application needs: a context that owns the device work buffers and
+ * entry points whose inputs and outputs already live in device memory (HBM),
+ * so no PCIe copy sits on the path.  Results are identical to coupe_rcb.
+ */
+#ifndef COUPE_HIP_H
+#define COUPE_HIP_H
+
+#include <stdint.h>
+#include "coupe.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * A per-device context: work buffers (grown on demand, reused across calls),
+ * kept until coupe_hip_ctx_destroy.  Calls on one context must not overlap.
+ * Returns NULL when the device cannot be used.
+ * No counterpart in coupe (rayon's global pool plays this role there).
+ */
+typedef struct coupe_hip_ctx coupe_hip_ctx;
+coupe_hip_ctx *coupe_hip_ctx_create(int device);
+void coupe_hip_ctx_destroy(coupe_hip_ctx *ctx);
+
+/*
+ * RCB on device-resident data.
+ *
+ *   d_points    N*dimension doubles (AoS, like PointND<D>), device memory
+ *   d_weights   N values of `weight_type` (int, int64_t or double), device
+ *   d_partition N uint64 part ids, device memory (written)
+ *   stream      hipStream_t to run on (NULL: the null stream)
+ *
+ * Same semantics, errors and ids as coupe_rcb.  Returns when the ids are
+ * written (the stream is synchronised).
+ * Replaces: Rcb::partition, coupe/src/algorithms/recursive_bisection.rs:805-811
+ * (through rcb(), :644-705), for data already in device memory.
+ */
+enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream,
+		uint64_t *d_partition, uintptr_t n, uintptr_t dimension,
+		const double *d_points, enum coupe_type weight_type, const void *d_weights,
+		uintptr_t iter_count, double tolerance);
+
+/*
+ * Counters of the last call on a context (for benchmarks and tests).
+ */
+struct coupe_hip_stats {
+	uint64_t points;          /* N of the last call */
+	uint32_t levels;          /* bisection levels run */
+	uint32_t passes;          /* histogram passes (all levels) */
+	uint32_t max_passes;      /* largest pass count of one level */
+	uint32_t weight_mode;     /* 0 int64 exact sums, 1 f64 sums */
+	uint32_t weight_bytes;    /* bytes per stored weight (4 or 8) */
+	uint32_t bins;            /* histogram bins per part per pass */
+	uint64_t unit_points;     /* points per work unit */
+	uint64_t device_bytes;    /* device memory held by the context */
+};
+enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out);
+
+/*
+ * Per-kernel timing of the last call, measured with HIP events recorded on
+ * the call's stream around every launch when profiling is enabled, and the
+ * algorithmic bytes each kernel class must move (its required reads and
+ * writes, counted once).  Profiling adds two event records per launch.
+ */
+enum coupe_hip_kernel {
+	COUPE_HIP_K_PROLOGUE,   /* AoS f64 -> SoA f32, bounding box, weight sums */
+	COUPE_HIP_K_HIST,       /* split-candidate histogram pass */
+	COUPE_HIP_K_DECIDE,     /* split decision replay */
+	COUPE_HIP_K_PARTITION,  /* stable partition of all parts of a level */
+	COUPE_HIP_K_IDS,        /* part ids through the original index */
+	COUPE_HIP_K_OTHER,      /* tables, scans, weight conversion */
+	COUPE_HIP_K_COUNT
+};
+struct coupe_hip_kernel_time {
+	uint32_t launches;
+	double total_ms;          /* sum of launch durations */
+	double algo_bytes;        /* algorithmic bytes, summed over launches */
+};
+enum coupe_err coupe_hip_set_profiling(coupe_hip_ctx *ctx, int enable);
+enum coupe_err coupe_hip_kernel_times(const coupe_hip_ctx *ctx, struct coupe_hip_kernel_time *out,
+		int count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COUPE_HIP_H */

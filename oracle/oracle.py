@@ -1,0 +1,177 @@
+"""ctypes wrapper of the CPU oracle (oracle/rcb_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker, and by ``bench.py``'s cpu_baseline leg.  The product
+(``coupe_amd``) never imports this module.
+
+The oracle restates coupe's ``Rcb`` (coupe/src/algorithms/recursive_bisection.rs
+:456-705, coupe/src/geometry.rs:33-78) in C.  How each function is pinned:
+
+* ``rcb`` / ``rcb_par`` -- the golden vectors in tests/golden/rcb_vectors.json
+  (the reference's ``test_rcb_basic`` :1077-1115, the ``Rcb`` doctest
+  :739-768, ``coupe-ffi/examples/rcb.c``, and hand-traced cases that hit
+  every exit of the split loop); the closed form of dyadic grids (interleaved
+  cell-index bits); an independent pure-Python transcription on small seeded
+  inputs (tests/pyref.py); invariants of ``test_par_rcb_split``.
+* ``bbox`` -- the reference's ``test_aabb_2d``/``test_aabb_3d`` values
+  (coupe/src/geometry.rs:341-355, :489-503) and numpy min/max.
+* ``split_1d`` -- the invariants of ``test_par_rcb_split`` (:1041-1075).
+* ``reorder_1d`` -- the invariants of ``test_reorder_split_scalar``
+  (:952-983).
+* Non-integer f64 weights: the reference's summation order is rayon's and
+  unspecified, so ids for real weights are "parity unpinned" beyond the
+  1e-12 relative per-part weight tolerance used by the GPU tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "librcb_oracle.so")
+
+W_INT, W_INT64, W_DOUBLE = 0, 1, 2
+
+
+class OracleNode(C.Structure):
+    _fields_ = [
+        ("heap_id", C.c_uint64),
+        ("count", C.c_uint64),
+        ("count_left", C.c_uint64),
+        ("weight_left", C.c_double),
+        ("split_pos", C.c_float),
+        ("steps", C.c_uint32),
+        ("level", C.c_uint32),
+        ("all_left", C.c_uint32),
+    ]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [
+        ("ambiguous_pivots", C.c_uint64),
+        ("candidates", C.c_uint64),
+        ("nodes", C.c_uint64),
+        ("trace", C.POINTER(OracleNode)),
+        ("trace_cap", C.c_uint64),
+        ("trace_len", C.c_uint64),
+    ]
+
+
+def build() -> None:
+    """Compile the oracle library in place (gcc, see oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        sig = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+               C.c_size_t, C.c_int, C.c_size_t, C.c_double, C.POINTER(OracleStats)]
+        L.oracle_rcb.argtypes = sig
+        L.oracle_rcb.restype = C.c_int
+        L.oracle_rcb_par.argtypes = sig + [C.c_int]
+        L.oracle_rcb_par.restype = C.c_int
+        L.oracle_bbox.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+        L.oracle_bbox.restype = C.c_int
+        L.oracle_split_1d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_float,
+                                      C.c_float, C.POINTER(C.c_uint64), C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_float)]
+        L.oracle_split_1d.restype = C.c_int
+        L.oracle_reorder_1d.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t]
+        L.oracle_reorder_1d.restype = C.c_size_t
+        L.oracle_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _kind_of(weights) -> int:
+    dt = np.asarray(weights).dtype
+    if dt == np.int32:
+        return W_INT
+    if dt == np.int64:
+        return W_INT64
+    if dt == np.float64:
+        return W_DOUBLE
+    raise TypeError(f"weights must be int32, int64 or float64, got {dt}")
+
+
+def rcb(points, weights, iter_count, tolerance, *, n_part=None, constant=False,
+        trace=False, parallel=False, nthreads=0):
+    """Run the oracle.  Returns ``(err, ids, stats)``.
+
+    ``points`` is (N, D) float64 (AoS), ``weights`` an int32/int64/float64
+    array of N values (or of one value with ``constant=True``, like
+    ``coupe_data_constant``).  ``ids`` is a uint64 array (coupe's usize ids).
+    """
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    n = pts.shape[0]
+    dim = pts.shape[1] if pts.ndim == 2 else 0
+    w = np.ascontiguousarray(weights)
+    kind = _kind_of(w)
+    n_w = n if constant else w.shape[0]
+    if n_part is None:
+        n_part = n
+    ids = np.zeros(max(n_part, 1), dtype=np.uint64)[:n_part]
+    st = OracleStats()
+    nodes = None
+    if trace:
+        cap = 1 << 16
+        nodes = (OracleNode * cap)()
+        st.trace = C.cast(nodes, C.POINTER(OracleNode))
+        st.trace_cap = cap
+    args = [ids.ctypes.data, n_part, dim, pts.ctypes.data, n, kind, w.ctypes.data, n_w,
+            1 if constant else 0, iter_count, float(tolerance), C.byref(st)]
+    if parallel:
+        err = lib().oracle_rcb_par(*args, int(nthreads))
+    else:
+        err = lib().oracle_rcb(*args)
+    out = {"ambiguous_pivots": st.ambiguous_pivots, "candidates": st.candidates, "nodes": st.nodes}
+    if trace:
+        out["trace"] = [
+            dict(heap_id=x.heap_id, count=x.count, count_left=x.count_left,
+                 weight_left=x.weight_left, split_pos=x.split_pos, steps=x.steps,
+                 level=x.level, all_left=x.all_left)
+            for x in nodes[: st.trace_len]
+        ]
+    return err, ids, out
+
+
+def bbox(points):
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    dim = pts.shape[1]
+    lo = np.zeros(dim)
+    hi = np.zeros(dim)
+    ok = lib().oracle_bbox(pts.ctypes.data, pts.shape[0], dim, lo.ctypes.data, hi.ctypes.data)
+    return (lo, hi) if ok else None
+
+
+def split_1d(coords, weights, tolerance, vmin, vmax):
+    """One split of one node (``par_rcb_split``).  Returns the permuted
+    coordinates and weights plus ``(count_left, weight_left, split_pos)``."""
+    c = np.ascontiguousarray(coords, dtype=np.float32).copy()
+    w = np.ascontiguousarray(weights, dtype=np.int64).copy()
+    cl = C.c_uint64()
+    wl = C.c_int64()
+    sp = C.c_float()
+    lib().oracle_split_1d(c.ctypes.data, w.ctypes.data, c.shape[0], float(tolerance),
+                          float(vmin), float(vmax), C.byref(cl), C.byref(wl), C.byref(sp))
+    return c, w, cl.value, wl.value, sp.value
+
+
+def reorder_1d(coords, pivot):
+    c = np.ascontiguousarray(coords, dtype=np.float32).copy()
+    left = lib().oracle_reorder_1d(c.ctypes.data, c.shape[0], int(pivot))
+    return c, left
+
+
+def num_threads() -> int:
+    return lib().oracle_num_threads()

@@ -1,0 +1,199 @@
+"""Parity of the HIP path with the oracle, through the C ABI.
+
+Integer and integer-valued weights: ids bit for bit.  Non-integer f64
+weights: ids equal unless a split decision lies within rounding of its
+threshold (then per-part weights must agree to 1e-12 relative).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import coupe_amd
+from coupe_amd import _native, workloads
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gpu_rcb(pts, w, iters, tol):
+    part = np.zeros(pts.shape[0], dtype=np.uint64)
+    coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(part, (pts, w))
+    return part
+
+
+def oracle_rcb(oracle, pts, w, iters, tol, parallel=False):
+    err, ids, st = oracle.rcb(pts, w, iters, tol, parallel=parallel, nthreads=16 if parallel else 0)
+    assert err == 0
+    return ids, st
+
+
+def part_weights(ids, w):
+    k = int(ids.max()) + 1 if ids.size else 0
+    return np.bincount(ids.astype(np.int64), weights=w.astype(np.float64), minlength=k)
+
+
+def assert_parity(oracle, pts, w, iters, tol, label, parallel_oracle=False):
+    say(f"{label}: n={pts.shape[0]} d={pts.shape[1]} L={iters} tol={tol} w={w.dtype}")
+    got = gpu_rcb(pts, w, iters, tol)
+    exp, st = oracle_rcb(oracle, pts, w, iters, tol, parallel=parallel_oracle)
+    if cases.integral(w):
+        if not np.array_equal(got, exp):
+            bad = np.nonzero(got != exp)[0]
+            raise AssertionError(f"{label}: {bad.size} ids differ, first at {bad[:5]}: "
+                                 f"gpu {got[bad[:5]]} oracle {exp[bad[:5]]}")
+    else:
+        if not np.array_equal(got, exp):
+            a, b = part_weights(got, w), part_weights(exp, w)
+            assert a.shape == b.shape, label
+            np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg=label)
+    return got
+
+
+def test_golden_vectors(oracle_mod):
+    vecs, _ = cases.golden()
+    L = _native.lib()
+    for v, pts, w in vecs:
+        say(v["id"])
+        n = pts.shape[0]
+        part = np.zeros(n, dtype=np.uint64)
+        dp = L.coupe_data_array(n, _native.COUPE_DOUBLE, pts.ctypes.data)
+        wt = {np.dtype(np.int32): 0, np.dtype(np.int64): 1, np.dtype(np.float64): 2}[w.dtype]
+        dw = (L.coupe_data_constant(n, wt, w.ctypes.data) if v["constant"]
+              else L.coupe_data_array(n, wt, w.ctypes.data))
+        err = L.coupe_rcb(part.ctypes.data, 2, dp, dw, v["iter_count"], v["tolerance"])
+        L.coupe_data_free(dp)
+        L.coupe_data_free(dw)
+        assert err == 0
+        assert part.tolist() == v["expected"], v["id"]
+
+
+def test_callback_data_sets():
+    # coupe_data_fn for points and weights (coupe-ffi/src/data.rs:97-144).
+    L = _native.lib()
+    pts = np.random.default_rng(3).random((500, 3))
+    w = np.arange(1, 501, dtype=np.int64)
+    keep = []
+    p_cb = _native.ITH_FN(lambda ctx, i: pts.ctypes.data + 24 * i)
+    w_cb = _native.ITH_FN(lambda ctx, i: w.ctypes.data + 8 * i)
+    keep += [p_cb, w_cb]
+    dp = L.coupe_data_fn(None, 500, _native.COUPE_DOUBLE, p_cb)
+    dw = L.coupe_data_fn(None, 500, _native.COUPE_INT64, w_cb)
+    part = np.zeros(500, dtype=np.uint64)
+    assert L.coupe_rcb(part.ctypes.data, 3, dp, dw, 5, 0.02) == 0
+    L.coupe_data_free(dp)
+    L.coupe_data_free(dw)
+    assert np.array_equal(part, gpu_rcb(pts, w, 5, 0.02))
+
+
+@pytest.mark.parametrize("seed", range(160))
+def test_random_small_cases(oracle_mod, seed):
+    pts, w, iters, tol = cases.random_case(seed, n_max=300)
+    assert_parity(oracle_mod, pts, w, iters, tol, f"random seed {seed}")
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_medium_cases(oracle_mod, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(10_000, 300_000))
+    dim = 2 + seed % 2
+    if seed % 3 == 0:
+        pts = rng.random((n, dim))
+    elif seed % 3 == 1:
+        pts = rng.normal(0, 1, (n, dim)) ** 3
+    else:
+        pts = np.round(rng.random((n, dim)) * 50) / 50
+    w = rng.integers(1, 1000, n).astype([np.int32, np.int64, np.float64][seed % 3])
+    assert_parity(oracle_mod, pts, w, int(rng.integers(3, 14)), float(rng.choice([0.0, 0.01, 0.05])),
+                  f"medium seed {seed}")
+
+
+def test_edge_cases(oracle_mod):
+    rng = np.random.default_rng(7)
+    one = rng.random((1, 3))
+    assert_parity(oracle_mod, one, np.ones(1), 6, 0.05, "single point")
+    assert_parity(oracle_mod, rng.random((2, 2)), np.ones(2, np.int32), 3, 0.0, "two points")
+    same = np.tile([[0.25, -3.0, 7.0]], (1000, 1))
+    assert_parity(oracle_mod, same, np.ones(1000), 5, 0.05, "all points equal")
+    assert_parity(oracle_mod, rng.random((5000, 3)), np.ones(5000), 0, 0.05, "iter_count 0")
+    assert_parity(oracle_mod, rng.random((300, 2)), np.ones(300), 20, 0.05, "deep: 2^20 leaves, 300 points")
+    assert_parity(oracle_mod, rng.random((3000, 3)), np.ones(3000), 9, -1.0, "negative tolerance")
+    big = rng.integers(2 ** 28, 2 ** 31 - 1, 4000).astype(np.int32)
+    assert_parity(oracle_mod, rng.random((4000, 2)), big, 6, 0.01, "i32 sum wraps")
+    huge = rng.integers(2 ** 40, 2 ** 41, 4000).astype(np.int64)
+    assert_parity(oracle_mod, rng.random((4000, 3)), huge, 6, 0.01, "int64 beyond int32")
+    wide = rng.normal(0, 1, (4000, 3)) * 10.0 ** rng.integers(-30, 30, (4000, 3))
+    assert_parity(oracle_mod, wide, np.ones(4000), 8, 0.01, "wide dynamic range")
+    tiny = rng.random((4000, 2)) * 1e-40   # f32 denormals after the cast
+    assert_parity(oracle_mod, tiny, np.ones(4000), 8, 0.01, "denormal coordinates")
+    zw = np.zeros(2000)
+    assert_parity(oracle_mod, rng.random((2000, 2)), zw, 4, 0.05, "zero weights (NaN imbalance)")
+    negw = rng.integers(-50, 50, 2000).astype(np.float64)
+    assert_parity(oracle_mod, rng.random((2000, 3)), negw, 5, 0.05, "negative weights")
+
+
+def test_real_weights(oracle_mod):
+    rng = np.random.default_rng(9)
+    for k in range(4):
+        n = 50_000
+        pts = rng.random((n, 2 + k % 2))
+        w = rng.random(n) * 3.0 + 0.1
+        assert_parity(oracle_mod, pts, w, 8, 0.01, f"real weights {k}")
+
+
+def test_uniform2d_full(oracle_mod):
+    pts, w, it, tol = workloads.standard("uniform2d")
+    assert_parity(oracle_mod, pts, w, it, tol, "uniform2d 2^20")
+
+
+def test_gauss3d_scaled(oracle_mod):
+    pts, w, it, tol = workloads.standard("gauss3d", scale=4)
+    assert_parity(oracle_mod, pts, w, it, tol, "gauss3d 2^20")
+
+
+def test_jgrid_scaled(oracle_mod):
+    pts, w, it, tol = workloads.standard("jgrid512", scale=6)   # 128^3
+    assert_parity(oracle_mod, pts, w, it, tol, "jgrid 128^3")
+
+
+@pytest.mark.slow
+def test_gauss3d_full(oracle_mod):
+    pts, w, it, tol = workloads.standard("gauss3d")
+    assert_parity(oracle_mod, pts, w, it, tol, "gauss3d 2^24", parallel_oracle=True)
+
+
+@pytest.mark.slow
+def test_grid512_closed_form():
+    # 512^3 cell centres, unit weights, 4096 parts: ids are the interleaved
+    # cell-index bits (axis 0, 1, 2, 0, ...), MSB first.
+    say("grid512 closed form: n=2^27 L=12")
+    pts, w = workloads.jgrid(512, jitter=False)
+    got = gpu_rcb(pts, w, 12, 0.01)
+    del pts
+    idx = np.arange(512 ** 3, dtype=np.int64)
+    exp = np.zeros(512 ** 3, dtype=np.uint64)
+    for level in range(12):
+        a = level % 3
+        coord = (idx // 512 ** a) % 512
+        bit = ((coord >> (8 - level // 3)) & 1).astype(np.uint64)
+        exp |= bit << np.uint64(11 - level)
+    assert np.array_equal(got, exp)
+
+
+def test_device_entry_matches_host_entry():
+    torch = pytest.importorskip("torch")
+    pts, w, it, tol = workloads.standard("gauss3d", scale=6)
+    host = gpu_rcb(pts, w, it, tol)
+    dp = torch.from_numpy(pts).cuda()
+    dw = torch.from_numpy(w).cuda()
+    for _ in range(2):   # the context's buffers are reused across calls
+        out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
+        coupe_amd.Rcb(iter_count=it, tolerance=tol).partition(out, (dp, dw))
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy().astype(np.uint64), host)
