@@ -60,6 +60,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m coupe_amd.build`")
+        # torch ships its own libamdhip64.so.7.  Loading it first makes this
+        # library bind to the same HIP runtime instead of a second copy from
+        # /opt/rocm (two runtimes in one process hide the GPU from torch).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.coupe_strerror.argtypes = [C.c_int]
         L.coupe_strerror.restype = C.c_char_p
