@@ -1,10 +1,10 @@
 // rcb_common.h -- shared types of the MI355X RCB engine (device and host).
 //
-// Vocabulary follows coupe: a *part* (segment) is a node of the bisection
-// tree, i.e. a contiguous range of the working arrays; a *level* is one depth
-// of the tree; a *pass* is one histogram sweep that evaluates several split
-// candidates of every part still searching; a *unit* is the range of one
-// part that one workgroup sweeps.
+// Vocabulary follows coupe: a *part* is a node of the bisection tree, i.e. a
+// contiguous range of the working arrays; a *level* is one depth of the tree;
+// a *pass* is one sweep that evaluates several split candidates of every part
+// still searching; a *tile* is a range of at most kTile points of one part,
+// the unit of work of a workgroup.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,9 +14,10 @@ namespace coupe_hip {
 
 constexpr int kMaxDim = 3;
 constexpr int kBlock = 256;          // threads per workgroup of the sweeps
-constexpr int kItems = 8;            // points per thread per partition tile
+constexpr int kItems = 8;            // points per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kMaxBins = 256;        // histogram bins per part per pass
+constexpr int kFirstBins = 4;        // bins of the register sweep (2 bisection steps)
 
 // Part status during a level.
 enum : uint32_t {
@@ -43,12 +44,13 @@ struct Part {
     int32_t kstar;         // in-order index of the stopping candidate in the last pass
     uint64_t wl;           // weight_left, accumulator bits
     uint64_t cnt_left;     // points going left
-    uint32_t unit_begin;   // first unit of this part in the unit table
-    uint32_t unit_count;
+    uint32_t tile_begin;   // first tile of this part in the tile table
+    uint32_t tile_count;
+    int32_t child[2];      // indices of the left / right child in the next level (-1: empty)
 };
 
-// A unit: the range [begin, begin + count) of part `part`.
-struct Unit {
+// A tile: the range [begin, begin + count) of part `part`, count <= kTile.
+struct Tile {
     uint64_t begin;
     uint32_t count;
     uint32_t part;
@@ -65,9 +67,6 @@ struct Prologue {
     uint32_t non_integral;      // any weight with a fractional part or |w| > 2^53
     uint32_t not_i32;           // any weight outside int32
 };
-
-// Storage / accumulation modes chosen after the prologue.
-enum WeightStore : int { kStoreI32 = 0, kStoreI64 = 1, kStoreF64 = 2 };
 
 __host__ __device__ inline uint32_t f32_key(float x) {
     uint32_t u = __builtin_bit_cast(uint32_t, x);

@@ -44,15 +44,17 @@ struct Context {
     DevBuf cx[2][3];                 // SoA f32 coordinates, two sides
     DevBuf w[2];                     // stored weights, two sides
     DevBuf ix[2];                    // original indices, two sides
-    DevBuf parts[2], units[2];       // part / unit tables, current and next level
+    DevBuf parts[2], tiles[2];       // part / tile tables, current and next level
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
-    DevBuf ucnt;                     // per-unit bin counts (partition offsets)
-    DevBuf uleft, nchild, nunits;    // scan inputs / outputs
+    DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
     DevBuf bsum;                     // scan block sums
+    DevBuf status;                   // look-back status words, one per tile
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
     bool bins_clean = false;
+    uint32_t epoch = 0;              // look-back epoch of the last partition launch
+    int cu_count = 0;                // compute units of the device
     double swept_bytes = 0.0;        // HIST algorithmic bytes of the last call
     coupe_hip_stats stats{};
     Profiler prof;

@@ -99,15 +99,18 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 }
 
 // Small device counters shared by the kernels of one call.
+constexpr int kMaxTickets = 4096;
 struct Counters {
     uint32_t n_parts[2];
-    uint32_t n_units[2];
+    uint32_t n_tiles[2];
     uint32_t searching;
-    uint32_t pad;
+    uint32_t error;
     uint64_t parts_total;
-    uint64_t units_total;
+    uint64_t tiles_total;
     uint64_t min_id;
-    unsigned long long swept;   // points swept by histogram passes
+    unsigned long long swept;                   // points swept by standalone passes
+    unsigned long long swept_fused;             // points binned inside partitions
+    unsigned long long ticket[kMaxTickets];     // partition ticket counter, one per level
 };
 
 static inline unsigned grid_for(uint64_t items, unsigned per_block) {
@@ -129,54 +132,56 @@ static void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t
     });
 }
 
-// Points per unit: about 2048 units at level 0 (8 workgroups per CU on 256
-// CUs), in whole partition tiles.
-static uint64_t choose_unit(uint64_t n) {
-    uint64_t u = (n + 2047) / 2048;
-    u = (u + kTile - 1) / kTile * kTile;
-    if (u < (uint64_t)kTile) u = kTile;
-    if (u > (1ull << 20)) u = 1ull << 20;
-    return u;
-}
-
-// Histogram bins per part, bounded by the bin buffers.
-static int choose_bins(uint64_t parts_ub, uint64_t units_ub) {
+// Bins per part of the standalone LDS passes, bounded by the bin buffers.
+static int choose_bins(uint64_t parts_ub) {
     const uint64_t cap = 1ull << 23;   // entries of each bin array
     int nb = kMaxBins;
-    while (nb > 2 && (parts_ub * nb > cap || units_ub * nb > cap)) nb >>= 1;
+    while (nb > kFirstBins && parts_ub * nb > cap) nb >>= 1;
     return nb;
+}
+
+// Resident workgroups of a persistent kernel: CUs x blocks per CU.
+template <typename K>
+static unsigned persistent_grid(Context &cx, K kernel, uint64_t work_items) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 2;
+    uint64_t g = (uint64_t)cx.cu_count * (uint64_t)per_cu;
+    if (g > work_items) g = work_items;
+    return (unsigned)(g < 1 ? 1 : g);
 }
 
 template <int D, typename WS, typename Acc>
 static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_count, double tol, int wrap32,
                        uint64_t *d_out, WS *w_first) {
-    const uint64_t unit = choose_unit(n);
     Counters *ctr = (Counters *)cx.counters.ptr;
     uint32_t *h_search = (uint32_t *)cx.host_scratch;
 
-    // Table capacities: parts at depth l <= min(2^l, n); units <= parts + n / unit.
+    // Table capacities: parts at depth l <= min(2^l, n); tiles <= parts + n / kTile.
     uint64_t parts_cap = 1;
     for (size_t l = 0; l < iter_count && parts_cap < n; l++) parts_cap *= 2;
     if (parts_cap > n) parts_cap = n;
-    const uint64_t units_cap = parts_cap + (n + unit - 1) / unit;
+    const uint64_t tiles_cap = parts_cap + (n + kTile - 1) / kTile;
+    if (tiles_cap > 0xffffffffull) throw HipFailure{hipErrorOutOfMemory};
     for (int b = 0; b < 2; b++) {
         cx.parts[b].ensure(parts_cap * sizeof(Part));
-        cx.units[b].ensure(units_cap * sizeof(Unit));
+        cx.tiles[b].ensure(tiles_cap * sizeof(Tile));
     }
-    cx.uleft.ensure(units_cap * sizeof(uint64_t));
     cx.nchild.ensure(parts_cap * sizeof(uint64_t));
-    cx.nunits.ensure(parts_cap * sizeof(uint64_t));
-    // Bin buffers: the largest parts x bins and units x bins over all levels
-    // (each level picks its bin count from its own table bounds).
-    uint64_t bins = 1, ucnt_entries = 1;
+    cx.ntiles.ensure(parts_cap * sizeof(uint64_t));
+    const bool fresh_status = cx.status.bytes < tiles_cap * 8;
+    cx.status.ensure(tiles_cap * 8);
+    if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(iter_count, 1u << 20) - 2) {
+        check(hipMemsetAsync(cx.status.ptr, 0, cx.status.bytes, st));
+        cx.epoch = 0;
+    }
+    // Bins: parts x bins of the widest pass of any level.
+    uint64_t bins = 1;
     {
         uint64_t pub = 1;
         for (size_t level = 0; level < iter_count; level++) {
-            const uint64_t uub = std::min<uint64_t>(units_cap, pub + (n + unit - 1) / unit);
-            const int nbl = choose_bins(pub, uub);
-            bins = std::max<uint64_t>(bins, pub * nbl);
-            ucnt_entries = std::max<uint64_t>(ucnt_entries, uub * nbl);
-            if (pub >= parts_cap) break;   // later levels repeat this bound
+            bins = std::max<uint64_t>(bins, pub * (uint64_t)choose_bins(pub));
+            if (pub >= parts_cap) break;
             pub = std::min<uint64_t>(2 * pub, parts_cap);
         }
     }
@@ -190,9 +195,11 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         check(hipMemsetAsync(cx.bcnt.ptr, 0, cx.bcnt.bytes, st));
         check(hipMemsetAsync(cx.bw.ptr, 0, cx.bw.bytes, st));
         check(hipMemsetAsync(cx.bmin.ptr, 0xff, cx.bmin.bytes, st));
-        cx.bins_clean = true;
     }
-    cx.ucnt.ensure(ucnt_entries * 4);
+    cx.bins_clean = false;   // until this call completes
+    uint32_t *bcnt = (uint32_t *)cx.bcnt.ptr;
+    Acc *bw = (Acc *)cx.bw.ptr;
+    uint32_t *bmin = (uint32_t *)cx.bmin.ptr;
 
     // Working arrays: slot 0 holds the prologue output, slot 1 the other side.
     float *cs[2][3];
@@ -207,98 +214,145 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
 
     const int real_mode = std::is_same<Acc, double>::value;
     const double payload = 4.0 * D + sizeof(WS) + 4.0;   // bytes of one point in the working arrays
-    check(hipMemsetAsync(&ctr->swept, 0, sizeof(ctr->swept), st));
+    check(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_init_root<Acc><<<grid_for(units_cap, kBlock), kBlock, 0, st>>>(
-            (Part *)cx.parts[0].ptr, &ctr->n_parts[0], (Unit *)cx.units[0].ptr, &ctr->n_units[0],
-            (const Prologue *)cx.prologue.ptr, n, D, unit, real_mode);
+        k_init_root<<<grid_for(tiles_cap, kBlock), kBlock, 0, st>>>(
+            (Part *)cx.parts[0].ptr, &ctr->n_parts[0], (Tile *)cx.tiles[0].ptr, &ctr->n_tiles[0],
+            (const Prologue *)cx.prologue.ptr, n, D, real_mode);
     });
 
-    int cur = 0;
-    uint64_t parts_ub = 1;
     cx.stats.levels = 0;
     cx.stats.passes = 0;
     cx.stats.max_passes = 0;
-    cx.stats.unit_points = unit;
-    for (size_t level = 0; level < iter_count; level++) {
-        const int nxt = cur ^ 1;
-        Part *parts = (Part *)cx.parts[cur].ptr;
-        Unit *units = (Unit *)cx.units[cur].ptr;
-        uint32_t *np = &ctr->n_parts[cur];
-        uint32_t *nu = &ctr->n_units[cur];
-        const uint64_t units_ub = std::min<uint64_t>(units_cap, parts_ub + (n + unit - 1) / unit);
-        const int nb = choose_bins(parts_ub, units_ub);
-        cx.stats.bins = nb;
+    cx.stats.unit_points = kTile;
+    cx.stats.bins = kFirstBins;
 
-        // Split search: passes until no part is searching.
-        uint32_t passes = 0;
+    // Split search of the parts of slot `sl` after a first pass (k_sweep or
+    // the fused partition) filled their kFirstBins bins.
+    auto search = [&](int sl, uint64_t parts_ub, uint64_t tiles_ub, bool first_standalone) {
+        Part *parts = (Part *)cx.parts[sl].ptr;
+        Tile *tiles = (Tile *)cx.tiles[sl].ptr;
+        uint32_t *np = &ctr->n_parts[sl];
+        uint32_t *ntl = &ctr->n_tiles[sl];
+        check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
+        timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, kFirstBins, tol, wrap32,
+                                                           &ctr->searching,
+                                                           first_standalone ? &ctr->swept : &ctr->swept_fused);
+        });
+        uint32_t passes = 1;
+        const int nb = choose_bins(parts_ub);
         for (;;) {
-            check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
-            // bytes of HIST launches are set from the swept-points counter at the end
-            timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
-                k_hist<WS, Acc><<<(unsigned)units_ub, kBlock, 0, st>>>(
-                    parts, units, nu, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur], nb, (uint32_t *)cx.bcnt.ptr,
-                    (Acc *)cx.bw.ptr, (uint32_t *)cx.bmin.ptr, (uint32_t *)cx.ucnt.ptr);
-            });
-            timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
-                    parts, np, (uint32_t *)cx.bcnt.ptr, (Acc *)cx.bw.ptr, (uint32_t *)cx.bmin.ptr, nb, tol,
-                    wrap32, &ctr->searching, &ctr->swept);
-            });
-            passes++;
             check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             check(hipStreamSynchronize(st));
             if (*h_search == 0) break;
-            // f32 bisection ends within a few hundred candidates; b >= 1 per pass
+            // f32 bisection ends within a few hundred candidates; each pass takes 8
             if (passes >= 4096) throw HipFailure{hipErrorUnknown};
+            check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
+            const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
+            cx.stats.bins = nb;
+            timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
+                k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[sl][0], cs[sl][1], cs[sl][2], ws[sl],
+                                                     nb, bcnt, bw, bmin);
+            });
+            timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, tol, wrap32,
+                                                               &ctr->searching, &ctr->swept);
+            });
+            passes++;
         }
         cx.stats.passes += passes;
         cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
-        cx.stats.levels++;
+    };
 
-        if (level + 1 < iter_count) {
-            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_unit_left<<<grid_for(units_ub, kBlock), kBlock, 0, st>>>(
-                    parts, units, nu, (const uint32_t *)cx.ucnt.ptr, nb, (uint64_t *)cx.uleft.ptr);
-            });
-            scan_u64(cx, st, (uint64_t *)cx.uleft.ptr, nu, units_ub, nullptr);
-            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-                k_partition<D, WS><<<(unsigned)units_ub, kBlock, 0, st>>>(
-                    parts, units, nu, (const uint64_t *)cx.uleft.ptr, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur],
-                    ixs[cur], cs[nxt][0], cs[nxt][1], cs[nxt][2], ws[nxt], ixs[nxt], n);
-            });
-            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_count<Acc><<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
-                    parts, np, unit, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.nunits.ptr);
-            });
-            scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
-            scan_u64(cx, st, (uint64_t *)cx.nunits.ptr, np, parts_ub, &ctr->units_total);
-            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
-                    parts, np, D, unit, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.nunits.ptr,
-                    (Part *)cx.parts[nxt].ptr, (Unit *)cx.units[nxt].ptr);
-                k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->units_total, &ctr->n_parts[nxt],
-                                              &ctr->n_units[nxt]);
-            });
-            cur = nxt;
-            parts_ub = std::min<uint64_t>(2 * parts_ub, parts_cap);
-        } else {
+    // Level 0: register sweep of the root, then its split search.
+    uint64_t parts_ub = 1;
+    uint64_t tiles_ub = (n + kTile - 1) / kTile;
+    {
+        const unsigned g = persistent_grid(cx, k_sweep<WS, Acc>, tiles_ub);
+        timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
+            k_sweep<WS, Acc><<<g, kBlock, 0, st>>>((const Part *)cx.parts[0].ptr, (const Tile *)cx.tiles[0].ptr,
+                                                  &ctr->n_tiles[0], cs[0][0], cs[0][1], cs[0][2], ws[0], bcnt, bw,
+                                                  bmin);
+        });
+        search(0, parts_ub, tiles_ub, true);
+    }
+
+    int cur = 0;
+    for (size_t level = 0; level < iter_count; level++) {
+        cx.stats.levels++;
+        const int nxt = cur ^ 1;
+        Part *parts = (Part *)cx.parts[cur].ptr;
+        Tile *tiles = (Tile *)cx.tiles[cur].ptr;
+        uint32_t *np = &ctr->n_parts[cur];
+        uint32_t *ntl = &ctr->n_tiles[cur];
+        const bool first = level == 0;   // working arrays still in input order
+
+        if (level + 1 == iter_count) {
             check(hipMemsetAsync(&ctr->min_id, 0xff, sizeof(uint64_t), st));
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_leaf_min<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &ctr->min_id);
             });
             // reads the axis coordinate and the index, writes the 8-byte id
-            timed(cx, st, COUPE_HIP_K_IDS, 16.0 * (double)n, [&] {
-                k_write_ids<<<(unsigned)units_ub, kBlock, 0, st>>>(parts, units, nu, cs[cur][0], cs[cur][1],
-                                                                 cs[cur][2], ixs[cur], &ctr->min_id, d_out, n);
+            const unsigned g = (unsigned)std::min<uint64_t>(tiles_ub, (uint64_t)cx.cu_count * 8);
+            timed(cx, st, COUPE_HIP_K_IDS, (first ? 12.0 : 16.0) * (double)n, [&] {
+                if (first)
+                    k_write_ids<true><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[cur][0], cs[cur][1], cs[cur][2],
+                                                          ixs[cur], &ctr->min_id, d_out, n);
+                else
+                    k_write_ids<false><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[cur][0], cs[cur][1], cs[cur][2],
+                                                           ixs[cur], &ctr->min_id, d_out, n);
             });
+            break;
         }
+
+        // Children: next level's parts and tiles.
+        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+            k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
+                                                                          (uint64_t *)cx.ntiles.ptr);
+        });
+        scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
+        scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
+        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+            k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
+                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr);
+            k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
+                                          &ctr->n_tiles[nxt]);
+        });
+
+        // Partition of this level, fused with the children's first pass.
+        const uint32_t epoch = ++cx.epoch;
+        unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
+        if (level > 0 && level % kMaxTickets == 0)
+            check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
+        auto launch_partition = [&](auto kernel) {
+            const unsigned g = persistent_grid(cx, kernel, tiles_ub);
+            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n - (first ? 4.0 * n : 0.0), [&] {
+                kernel<<<g, kBlock, 0, st>>>(parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch,
+                                             &ctr->error, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur], ixs[cur],
+                                             cs[nxt][0], cs[nxt][1], cs[nxt][2], ws[nxt], ixs[nxt], n, bcnt, bw,
+                                             bmin);
+            });
+        };
+        if (first) launch_partition(k_partition<D, WS, Acc, true>);
+        else launch_partition(k_partition<D, WS, Acc, false>);
+
+        cur = nxt;
+        parts_ub = std::min<uint64_t>(2 * parts_ub, parts_cap);
+        tiles_ub = std::min<uint64_t>(tiles_cap, parts_ub + (n + kTile - 1) / kTile);
+        search(cur, parts_ub, tiles_ub, false);
     }
-    // HIST bytes: the swept points times (axis coordinate + stored weight).
+
+    // Standalone-pass bytes: swept points times (axis coordinate + stored weight).
     unsigned long long *h_swept = (unsigned long long *)((char *)cx.host_scratch + 64);
+    uint32_t *h_error = (uint32_t *)((char *)cx.host_scratch + 128);
     check(hipMemcpyAsync(h_swept, &ctr->swept, sizeof(*h_swept), hipMemcpyDeviceToHost, st));
+    check(hipMemcpyAsync(h_error, &ctr->error, sizeof(*h_error), hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
+    if (*h_error) throw HipFailure{hipErrorUnknown};
     cx.swept_bytes = (double)*h_swept * (4.0 + sizeof(WS));
+    cx.bins_clean = true;
 }
 
 template <int D, typename WIN>
@@ -310,14 +364,13 @@ static void prologue(Context &cx, hipStream_t st, const double *pts, const WIN *
         init.bmax[k] = 0;
     }
     check(hipMemcpyAsync(cx.prologue.ptr, &init, sizeof init, hipMemcpyHostToDevice, st));
-    unsigned grid = (unsigned)std::min<uint64_t>(grid_for(n, kBlock), 4096);
-    // reads AoS f64 points and the input weights; writes SoA f32, index, int32 weight
-    const double bytes = (double)n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0 + 4.0);
+    const unsigned g = persistent_grid(cx, k_prologue<D, WIN>, (n + kProPoints - 1) / kProPoints);
+    // reads AoS f64 points and the input weights; writes SoA f32 and int32 weights
+    const double bytes = (double)n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0);
     timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
-        k_prologue<D, WIN><<<grid, kBlock, 0, st>>>(
-            pts, win, n, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
-            (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr), (uint32_t *)cx.ix[0].ptr, (int32_t *)cx.w[0].ptr,
-            (Prologue *)cx.prologue.ptr);
+        k_prologue<D, WIN><<<g, kBlock, 0, st>>>(pts, win, n, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
+                                                 (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
+                                                 (int32_t *)cx.w[0].ptr, (Prologue *)cx.prologue.ptr);
     });
     check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
@@ -411,6 +464,10 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
     try {
         check(hipSetDevice(cx.device));
         if (!cx.host_scratch) check(hipHostMalloc(&cx.host_scratch, 4096, hipHostMallocDefault));
+        if (cx.cu_count <= 0) {
+            check(hipDeviceGetAttribute(&cx.cu_count, hipDeviceAttributeMultiprocessorCount, cx.device));
+            if (cx.cu_count <= 0) cx.cu_count = 1;
+        }
         if (dim == 2) run_dim<2>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
         else run_dim<3>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
         cx.prof.collect();
@@ -430,18 +487,18 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
-                           &ix[0], &ix[1], &parts[0], &parts[1], &units[0], &units[1], &bcnt, &bw, &bmin,
-                           &ucnt, &uleft, &nchild, &nunits, &bsum, &prologue, &counters, &in_points,
-                           &in_weights, &out_ids};
+                           &ix[0], &ix[1], &parts[0], &parts[1], &tiles[0], &tiles[1], &bcnt, &bw, &bmin,
+                           &nchild, &ntiles, &bsum, &status, &prologue, &counters, &in_points, &in_weights,
+                           &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
 
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
-                     &ix[0], &ix[1], &parts[0], &parts[1], &units[0], &units[1], &bcnt, &bw, &bmin,
-                     &ucnt, &uleft, &nchild, &nunits, &bsum, &prologue, &counters, &in_points,
-                     &in_weights, &out_ids};
+                     &ix[0], &ix[1], &parts[0], &parts[1], &tiles[0], &tiles[1], &bcnt, &bw, &bmin,
+                     &nchild, &ntiles, &bsum, &status, &prologue, &counters, &in_points, &in_weights,
+                     &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

@@ -1,24 +1,26 @@
 // rcb_kernels.h -- HIP kernels (gfx950) of the RCB engine.
 //
-// Level structure (one level = one depth of coupe's recursion, for all parts
-// of that depth at once):
-//   k_hist    one pass: every workgroup sweeps one unit of a searching part
-//             and bins its axis coordinate against the 2^b - 1 candidates
-//             that coupe's split loop can reach in its next b steps
-//             (par_rcb_split, recursive_bisection.rs:471-520); bins hold
-//             count, weight sum and smallest coordinate, in LDS, then are
-//             added into the part's bins with one atomic per bin.
-//   k_decide  one wavefront per part: prefix sums / suffix minima over the
-//             bins, then lane 0 replays coupe's decision loop exactly
-//             (:522-571) for up to b steps.
-//   k_unit_left + scans: where each unit's left/right points start.
-//   k_partition  stable split of every part into {c < t} and {c >= t}:
-//             wave ballots give ranks, each wave store covers two contiguous
-//             runs.  Replaces reorder_split (:437-453, :122-181).
-//   k_children   builds the next level's parts (:603-641) and units.
+// One level = one depth of coupe's recursion, processed for all its parts at
+// once.  The working set is SoA: f32 coordinates per axis, stored weights, and
+// the u32 original index, in two copies (each partition writes the other one).
+//
+//   k_prologue   AoS f64 -> SoA f32, bounding box, weight sums (:674-688)
+//   k_sweep      first split pass of the root: 3 candidates (2 bisection steps),
+//                bins accumulated in registers, no atomics per point
+//   k_hist       further passes for parts still searching: 255 candidates
+//                (8 steps), bins in LDS
+//   k_decide     one wavefront per searching part: prefix sums over the bins,
+//                then lane 0 replays coupe's split loop exactly (:470-571)
+//   k_children   next level's parts and tiles (:603-641)
+//   k_partition  stable split of every part into {c < t} | {c >= t} with a
+//                decoupled look-back across the part's tiles and LDS-staged
+//                contiguous stores; in the same sweep it bins the children's
+//                next-axis coordinates (their first split pass)
 //   k_write_ids  last level: part ids through the original index (:589-601,
-//             :698-702) without moving the data again.
+//                :698-702)
 #pragma once
+
+#include <type_traits>
 
 #include "rcb_common.h"
 
@@ -55,6 +57,11 @@ template <> struct AccOps<double> {
     __device__ static double sub(double a, double b) { return a - b; }
 };
 
+__device__ inline void atomic_add_acc(uint64_t *p, uint64_t v) {
+    atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+__device__ inline void atomic_add_acc(double *p, double v) { atomicAdd(p, v); }
+
 __device__ inline int lane_id() { return (int)__lane_id(); }
 
 template <typename T> __device__ inline T wave_incl_add(T v) {
@@ -65,6 +72,29 @@ template <typename T> __device__ inline T wave_incl_add(T v) {
         if (lane >= o) v += n;
     }
     return v;
+}
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t x = __shfl_xor(v, o, 64);
+        v = x < v ? x : v;
+    }
+    return v;
+}
+
+// v[a] for a uniform runtime axis a, as bit masks the compiler cannot turn
+// into an indexed (scratch) access to a register array.
+__device__ __forceinline__ float sel3(int a, float v0, float v1, float v2) {
+    uint32_t m0 = a == 0 ? 0xffffffffu : 0u, m1 = a == 1 ? 0xffffffffu : 0u, m2 = a == 2 ? 0xffffffffu : 0u;
+    asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2));
+    return __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, v0) & m0) |
+                                         (__builtin_bit_cast(uint32_t, v1) & m1) |
+                                         (__builtin_bit_cast(uint32_t, v2) & m2));
 }
 
 // Candidate `p` (in-order index among nb-1) of the bisection tree rooted at
@@ -83,69 +113,90 @@ __device__ inline float candidate(float lo, float hi, int p, int nb) {
 
 // Number of candidates <= x (sorted ascending): the bin of x.  Bin j holds
 // cand[j-1] <= c < cand[j], so count(c < cand[q]) = sum of bins 0..q.
+// A NaN coordinate lands in bin 0, like !(c >= t) in the partition.
 __device__ inline int bin_of(const float *cand, int nb, float x) {
     int pos = 0;
     for (int step = nb >> 1; step >= 1; step >>= 1)
         if (x >= cand[pos + step - 1]) pos += step;
     return pos;
 }
+// Same for the 3 register candidates of the first pass.
+__device__ inline int bin4(float c0, float c1, float c2, float x) {
+    int b = (x >= c1) ? 2 : 0;
+    b += (x >= (b ? c2 : c0)) ? 1 : 0;
+    return b;
+}
 
 // ---------------------------------------------------------------------------
-// Prologue: AoS f64 -> SoA f32 (`as f32`, :674-679), original index, bounding
-// box (geometry.rs:42-76), weight sums (:684) and the weight-kind checks that
-// select exact integer accumulation.
+// Prologue: AoS f64 -> SoA f32 (`as f32`, :674-679), bounding box
+// (geometry.rs:42-76), weight sums (:684) and the weight-kind checks that
+// select exact integer accumulation.  Points are read through LDS so every
+// global load is contiguous.
 // ---------------------------------------------------------------------------
+constexpr int kProPoints = kBlock * 4;   // points per workgroup iteration
+
 template <int D, typename WIN>
 __global__ void __launch_bounds__(kBlock)
 k_prologue(const double *__restrict__ pts, const WIN *__restrict__ win, uint64_t n,
            float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
-           uint32_t *__restrict__ ix, int32_t *__restrict__ w32, Prologue *out) {
+           int32_t *__restrict__ w32, Prologue *out) {
+    __shared__ double sp[kProPoints * D];
+    const int tid = threadIdx.x;
     double mn[D], mx[D];
 #pragma unroll
     for (int k = 0; k < D; k++) { mn[k] = 1.7976931348623157e308; mx[k] = -1.7976931348623157e308; }
     uint64_t su = 0;
     double sf = 0.0, sa = 0.0, ma = 0.0;
     uint32_t nonint = 0, not32 = 0;
-    float *cs[3] = {c0, c1, c2};
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (uint64_t base = (uint64_t)blockIdx.x * kProPoints; base < n; base += (uint64_t)gridDim.x * kProPoints) {
+        const uint32_t cnt = (uint32_t)((n - base) < (uint64_t)kProPoints ? (n - base) : kProPoints);
+        const double *src = pts + base * D;
+        for (uint32_t j = tid; j < cnt * D; j += kBlock) sp[j] = src[j];
+        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < D; k++) {
-            double v = pts[i * D + k];
-            if (v < mn[k]) mn[k] = v;
-            if (mx[k] < v) mx[k] = v;
-            cs[k][i] = (float)v;
-        }
-        ix[i] = (uint32_t)i;
-        WIN w = win[i];
-        if constexpr (sizeof(WIN) == 8 && ((WIN)0.5) != 0) {   // double
-            double a = fabs(w);
-            bool integral = (w == trunc(w)) && a <= 9007199254740992.0;
-            nonint |= !integral;
-            sf += w;
-            sa += a;
-            ma = a > ma ? a : ma;
-            if (integral) {
-                su += (uint64_t)(int64_t)w;
-                bool fits = w >= -2147483648.0 && w <= 2147483647.0;
-                not32 |= !fits;
-                w32[i] = fits ? (int32_t)w : 0;
-            } else {
-                not32 = 1;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t p = k * kBlock + tid;
+            if (p < cnt) {
+                const uint64_t i = base + p;
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    double v = sp[p * D + a];
+                    if (v < mn[a]) mn[a] = v;
+                    if (mx[a] < v) mx[a] = v;
+                    float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
+                    cs[i] = (float)v;
+                }
+                WIN w = win[i];
+                if constexpr (std::is_same<WIN, double>::value) {
+                    double aw = fabs(w);
+                    bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
+                    nonint |= !integral;
+                    sf += w;
+                    sa += aw;
+                    ma = aw > ma ? aw : ma;
+                    if (integral) {
+                        su += (uint64_t)(int64_t)w;
+                        bool fits = w >= -2147483648.0 && w <= 2147483647.0;
+                        not32 |= !fits;
+                        w32[i] = fits ? (int32_t)w : 0;
+                    } else {
+                        not32 = 1;
+                    }
+                } else {
+                    int64_t wi = (int64_t)w;
+                    su += (uint64_t)wi;
+                    double aw = fabs((double)wi);
+                    sf += (double)wi;
+                    sa += aw;
+                    ma = aw > ma ? aw : ma;
+                    bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
+                    not32 |= !fits;
+                    w32[i] = (int32_t)wi;
+                }
             }
-        } else {
-            int64_t wi = (int64_t)w;
-            su += (uint64_t)wi;
-            double a = fabs((double)wi);
-            sf += (double)wi;
-            sa += a;
-            ma = a > ma ? a : ma;
-            bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
-            not32 |= !fits;
-            w32[i] = (int32_t)wi;
         }
+        __syncthreads();
     }
-    // wave reductions
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
 #pragma unroll
@@ -189,49 +240,169 @@ k_store_weights(const WIN *__restrict__ win, WS *__restrict__ ws, uint64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Histogram pass over all searching parts.
+// Register bins: block reduction of per-thread bins, then one global atomic
+// per non-empty bin.  Bins [0, NB) go to part slot `slot0`, [NB, 2NB) to
+// `slot1` (the two children of a part); a negative slot drops the bins.
+// ---------------------------------------------------------------------------
+template <int NR, typename Acc>
+struct RegBins {
+    uint32_t cnt[NR];
+    Acc w[NR];
+    uint32_t mn[NR];
+    __device__ __forceinline__ void reset() {
+#pragma unroll
+        for (int j = 0; j < NR; j++) { cnt[j] = 0; w[j] = (Acc)0; mn[j] = 0xffffffffu; }
+    }
+    __device__ __forceinline__ void add(int b, Acc wv, uint32_t key) {
+#pragma unroll
+        for (int j = 0; j < NR; j++) {
+            const bool s = b == j;
+            cnt[j] += s ? 1u : 0u;
+            w[j] += s ? wv : (Acc)0;
+            const uint32_t k = s ? key : 0xffffffffu;
+            mn[j] = k < mn[j] ? k : mn[j];
+        }
+    }
+};
+
+template <int NR, typename Acc>
+struct RegBinsShared {
+    uint32_t cnt[kBlock / 64][NR];
+    Acc w[kBlock / 64][NR];
+    uint32_t mn[kBlock / 64][NR];
+};
+
+template <int NR, int NB, typename Acc>
+__device__ __forceinline__ void flush_bins(RegBins<NR, Acc> &r, RegBinsShared<NR, Acc> &sh, int64_t slot0, int64_t slot1,
+                           uint32_t *__restrict__ gcnt, Acc *__restrict__ gw, uint32_t *__restrict__ gmn) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NR; j++) {
+        uint32_t c = wave_sum(r.cnt[j]);
+        Acc x = wave_sum(r.w[j]);
+        uint32_t m = wave_min(r.mn[j]);
+        if (lane == 0) { sh.cnt[wid][j] = c; sh.w[wid][j] = x; sh.mn[wid][j] = m; }
+    }
+    __syncthreads();
+    const int j = threadIdx.x;
+    if (j < NR) {
+        uint32_t c = 0, m = 0xffffffffu;
+        Acc x = (Acc)0;
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; v++) {
+            c += sh.cnt[v][j];
+            x += sh.w[v][j];
+            m = sh.mn[v][j] < m ? sh.mn[v][j] : m;
+        }
+        const int64_t slot = j < NB ? slot0 : slot1;
+        if (c && slot >= 0) {
+            const uint64_t g = (uint64_t)slot * NB + (j % NB);
+            atomicAdd(&gcnt[g], c);
+            atomic_add_acc(&gw[g], x);
+            atomicMin(&gmn[g], m);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// First pass of the root part: the 3 candidates of 2 bisection steps, bins in
+// registers.  Workgroups stride over the tile table and flush when the part
+// changes.
 // ---------------------------------------------------------------------------
 template <typename WS, typename Acc>
 __global__ void __launch_bounds__(kBlock)
-k_hist(const Part *__restrict__ parts, const Unit *__restrict__ units, const uint32_t *n_units,
+k_sweep(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
+        const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw, uint32_t *__restrict__ bmin) {
+    __shared__ RegBinsShared<kFirstBins, Acc> sh;
+    RegBins<kFirstBins, Acc> r;
+    r.reset();
+    const uint32_t nt = *n_tiles;
+    uint32_t cur = 0xffffffffu;
+    bool active = false;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    const float *c = c0;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {
+            if (active) { flush_bins<kFirstBins, kFirstBins>(r, sh, cur, -1, bcnt, bw, bmin); r.reset(); }
+            cur = T.part;
+            const Part &P = parts[cur];
+            active = P.status == kSearching;
+            e0 = candidate(P.blo, P.bhi, 0, kFirstBins);
+            e1 = candidate(P.blo, P.bhi, 1, kFirstBins);
+            e2 = candidate(P.blo, P.bhi, 2, kFirstBins);
+            c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
+        }
+        if (!active) continue;
+        const uint64_t end = T.begin + T.count;
+        for (uint64_t i = T.begin + threadIdx.x; i < end; i += kBlock) {
+            const float x = c[i];
+            r.add(bin4(e0, e1, e2, x), AccOps<Acc>::of(w[i]), f32_key(x));
+        }
+    }
+    if (active) flush_bins<kFirstBins, kFirstBins>(r, sh, cur, -1, bcnt, bw, bmin);
+}
+
+// ---------------------------------------------------------------------------
+// Later passes: nb - 1 candidates (nb <= 256) binned in LDS.
+// ---------------------------------------------------------------------------
+template <typename Acc>
+__device__ inline void flush_lds_bins(uint64_t base, int nb, const uint32_t *scnt, const Acc *sw,
+                                      const uint32_t *smin, uint32_t *bcnt, Acc *bw, uint32_t *bmin) {
+    for (int j = threadIdx.x; j < nb; j += kBlock) {
+        if (scnt[j]) {
+            atomicAdd(&bcnt[base + j], scnt[j]);
+            atomic_add_acc(&bw[base + j], sw[j]);
+            atomicMin(&bmin[base + j], smin[j]);
+        }
+    }
+}
+
+template <typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-       uint32_t *__restrict__ bmin, uint32_t *__restrict__ ucnt) {
+       uint32_t *__restrict__ bmin) {
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
     __shared__ Acc sw[kMaxBins];
     __shared__ uint32_t smin[kMaxBins];
-    const uint32_t u = blockIdx.x;
-    if (u >= *n_units) return;
-    const Unit un = units[u];
-    const Part &P = parts[un.part];
-    if (P.status != kSearching) return;
     const int tid = threadIdx.x;
-    const float blo = P.blo, bhi = P.bhi;
-    for (int p = tid; p < nb - 1; p += kBlock) cand[p] = candidate(blo, bhi, p, nb);
-    for (int j = tid; j < nb; j += kBlock) { scnt[j] = 0; sw[j] = (Acc)0; smin[j] = 0xffffffffu; }
-    __syncthreads();
-    const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
-    const uint64_t end = un.begin + un.count;
-    for (uint64_t i = un.begin + tid; i < end; i += kBlock) {
-        float x = c[i];
-        Acc wv = AccOps<Acc>::of(w[i]);
-        int b = bin_of(cand, nb, x);
-        atomicAdd(&scnt[b], 1u);
-        atomicAdd(&sw[b], wv);
-        atomicMin(&smin[b], f32_key(x));
-    }
-    __syncthreads();
-    const uint64_t base = (uint64_t)un.part * nb;
-    for (int j = tid; j < nb; j += kBlock) {
-        uint32_t k = scnt[j];
-        ucnt[(uint64_t)u * nb + j] = k;
-        if (k) {
-            atomicAdd(&bcnt[base + j], k);
-            atomicAdd(&bw[base + j], sw[j]);
-            atomicMin(&bmin[base + j], smin[j]);
+    const uint32_t nt = *n_tiles;
+    uint32_t cur = 0xffffffffu;
+    bool active = false;
+    const float *c = c0;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {
+            __syncthreads();
+            if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin);
+            __syncthreads();
+            cur = T.part;
+            const Part &P = parts[cur];
+            active = P.status == kSearching;
+            if (active) {
+                for (int p = tid; p < nb - 1; p += kBlock) cand[p] = candidate(P.blo, P.bhi, p, nb);
+                for (int j = tid; j < nb; j += kBlock) { scnt[j] = 0; sw[j] = (Acc)0; smin[j] = 0xffffffffu; }
+            }
+            c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
+            __syncthreads();
+        }
+        if (!active) continue;
+        const uint64_t end = T.begin + T.count;
+        for (uint64_t i = T.begin + tid; i < end; i += kBlock) {
+            const float x = c[i];
+            const int b = bin_of(cand, nb, x);
+            atomicAdd(&scnt[b], 1u);
+            atomic_add_acc(&sw[b], AccOps<Acc>::of(w[i]));
+            atomicMin(&smin[b], f32_key(x));
         }
     }
+    __syncthreads();
+    if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,7 +443,6 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) { lc += c[k]; lw += w[k]; }
-    // exclusive prefix of lane totals
     uint64_t ic = wave_incl_add(lc);
     Acc iw = wave_incl_add(lw);
     uint64_t ec = ic - lc;
@@ -288,7 +458,6 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
             }
         }
     }
-    // suffix minima: lane-local, then across lanes from the right
 #pragma unroll
     for (int k = 3; k >= 0; k--) lm = m[k] < lm ? m[k] : lm;
     uint32_t suf = lm;
@@ -351,25 +520,6 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     }
     P->blo = lo; P->bhi = hi; P->prev = prev;             // next pass continues here
     atomicAdd(n_searching, 1u);
-}
-
-// Points of each unit that go left (all of them for kAllLeft parts).
-__global__ void __launch_bounds__(kBlock)
-k_unit_left(const Part *__restrict__ parts, const Unit *__restrict__ units, const uint32_t *n_units,
-            const uint32_t *__restrict__ ucnt, int nb, uint64_t *__restrict__ left) {
-    const uint32_t u = blockIdx.x * kBlock + threadIdx.x;
-    if (u >= *n_units) return;
-    const Unit un = units[u];
-    const Part &P = parts[un.part];
-    uint64_t l;
-    if (P.status == kAllLeft) {
-        l = un.count;
-    } else {
-        l = 0;
-        const uint32_t *row = ucnt + (uint64_t)u * nb;
-        for (int j = 0; j <= P.kstar; j++) l += row[j];
-    }
-    left[u] = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -447,109 +597,28 @@ k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const 
 }
 
 // ---------------------------------------------------------------------------
-// Stable partition of every part: left = {c < t} (or all), right = the rest.
-// ---------------------------------------------------------------------------
-template <int D, typename WS>
-__global__ void __launch_bounds__(kBlock)
-k_partition(const Part *__restrict__ parts, const Unit *__restrict__ units, const uint32_t *n_units,
-            const uint64_t *__restrict__ uleft,
-            const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
-            const WS *__restrict__ iw, const uint32_t *__restrict__ iix,
-            float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
-            WS *__restrict__ ow, uint32_t *__restrict__ oix, uint64_t n_total) {
-    __shared__ uint32_t wtot[kItems][kBlock / 64];
-    __shared__ uint32_t tile_left;
-    const uint32_t u = blockIdx.x;
-    if (u >= *n_units) return;
-    const Unit un = units[u];
-    const Part &P = parts[un.part];
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
-    const bool all_left = P.status == kAllLeft;
-    const float t = P.split;
-    const uint64_t lprefix = uleft[u] - uleft[P.unit_begin];
-    const uint64_t lbase = P.begin + lprefix;
-    const uint64_t rbase = P.begin + P.cnt_left + ((un.begin - P.begin) - lprefix);
-    const float *ins[3] = {i0, i1, i2};
-    float *outs[3] = {o0, o1, o2};
-    const float *ca = ins[P.axis];
-    uint64_t runL = 0, runR = 0;
-    const uint64_t end = un.begin + un.count;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint64_t t0 = un.begin; t0 < end; t0 += kTile) {
-        bool f[kItems];
-        uint32_t pre[kItems];
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            uint64_t p = t0 + (uint64_t)k * kBlock + tid;
-            bool valid = p < end;
-            // !(c >= t) is c < t for numbers and sends NaN left, as bin_of does
-            f[k] = valid && (all_left || !(ca[p] >= t));
-            uint64_t bal = __ballot(f[k]);
-            pre[k] = __popcll(bal & lt_mask);
-            if (lane == 0) wtot[k][wid] = __popcll(bal);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < kItems; k++)
-                for (int wv = 0; wv < kBlock / 64; wv++) { uint32_t x = wtot[k][wv]; wtot[k][wv] = run; run += x; }
-            tile_left = run;
-        }
-        __syncthreads();
-        uint64_t dst[kItems];
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            uint64_t off = (uint64_t)k * kBlock + tid;   // position in tile
-            uint32_t rl = wtot[k][wid] + pre[k];          // left items before it
-            dst[k] = f[k] ? lbase + runL + rl : rbase + runR + (off - rl);
-        }
-        const uint64_t valid_n = (end - t0) < (uint64_t)kTile ? (end - t0) : (uint64_t)kTile;
-        const uint32_t tl = tile_left;
-#pragma unroll
-        for (int a = 0; a < D; a++) {
-#pragma unroll
-            for (int k = 0; k < kItems; k++) {
-                uint64_t p = t0 + (uint64_t)k * kBlock + tid;
-                if (p < end && dst[k] < n_total) outs[a][dst[k]] = ins[a][p];
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            uint64_t p = t0 + (uint64_t)k * kBlock + tid;
-            if (p < end && dst[k] < n_total) { ow[dst[k]] = iw[p]; oix[dst[k]] = iix[p]; }
-        }
-        runL += tl;
-        runR += valid_n - tl;
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Next level's parts.  Children of part s: left (2i+1, box hi = split, sum wl)
 // and right (2i+2, box lo = split, sum S - wl), kept when non-empty
-// (:586-588, :603-641).
+// (:586-588, :603-641).  Their split search starts on the next axis.
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t units_for(uint64_t count, uint64_t unit) {
-    return (uint32_t)((count + unit - 1) / unit);
-}
+__device__ inline uint32_t tiles_for(uint64_t count) { return (uint32_t)((count + kTile - 1) / kTile); }
 
-template <typename Acc>
 __global__ void __launch_bounds__(kBlock)
-k_children_count(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t unit,
-                 uint64_t *__restrict__ nchild, uint64_t *__restrict__ nunits) {
+k_children_count(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *__restrict__ nchild,
+                 uint64_t *__restrict__ ntiles) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= *n_parts) return;
     const Part &P = parts[s];
     const uint64_t nl = P.cnt_left, nr = P.count - P.cnt_left;
     nchild[s] = (nl > 0) + (nr > 0);
-    nunits[s] = units_for(nl, unit) + units_for(nr, unit);
+    ntiles[s] = tiles_for(nl) + tiles_for(nr);
 }
 
 template <typename Acc>
 __global__ void __launch_bounds__(64)
-k_children_write(const Part *__restrict__ parts, const uint32_t *n_parts, int dim, uint64_t unit,
-                 const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ unit_off,
-                 Part *__restrict__ next, Unit *__restrict__ next_units) {
+k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
+                 const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
+                 Part *__restrict__ next, Tile *__restrict__ next_tiles) {
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     const Part P = parts[s];
@@ -559,44 +628,254 @@ k_children_write(const Part *__restrict__ parts, const uint32_t *n_parts, int di
     const Acc S = AccOps<Acc>::from_bits(P.sum);
     const Acc wl = AccOps<Acc>::from_bits(P.wl);
     uint64_t co = child_off[s];
-    uint64_t uo = unit_off[s];
+    uint64_t to = tile_off[s];
+    int32_t id0 = -1, id1 = -1;
+#pragma unroll
     for (int side = 0; side < 2; side++) {
         const uint64_t cnt = side == 0 ? nl : nr;
         if (cnt == 0) continue;
         const uint64_t b = side == 0 ? P.begin : P.begin + nl;
-        const uint32_t nu = units_for(cnt, unit);
+        const uint32_t nt = tiles_for(cnt);
+        if (side == 0) id0 = (int32_t)co; else id1 = (int32_t)co;
         if (threadIdx.x == 0) {
             Part C;
             C.begin = b;
             C.count = cnt;
             C.heap_id = 2 * P.heap_id + 1 + side;
             C.sum = side == 0 ? AccOps<Acc>::to_bits(wl) : AccOps<Acc>::to_bits(AccOps<Acc>::sub(S, wl));
-            for (int k = 0; k < kMaxDim; k++) { C.lo[k] = P.lo[k]; C.hi[k] = P.hi[k]; }
-            if (side == 0) C.hi[a] = P.split; else C.lo[a] = P.split;
+#pragma unroll
+            for (int k = 0; k < kMaxDim; k++) {
+                float lo = P.lo[k], hi = P.hi[k];
+                if (k == (int)a) {
+                    if (side == 0) hi = P.split; else lo = P.split;
+                }
+                C.lo[k] = lo;
+                C.hi[k] = hi;
+            }
             C.axis = na;
-            C.blo = C.lo[na];
-            C.bhi = C.hi[na];
+            C.blo = na == 0 ? C.lo[0] : (na == 1 ? C.lo[1] : C.lo[2]);
+            C.bhi = na == 0 ? C.hi[0] : (na == 1 ? C.hi[1] : C.hi[2]);
             C.prev = ~0ull;
             C.status = kSearching;
             C.split = 0.0f;
             C.kstar = -1;
             C.wl = 0;
             C.cnt_left = 0;
-            C.unit_begin = (uint32_t)uo;
-            C.unit_count = nu;
+            C.tile_begin = (uint32_t)to;
+            C.tile_count = nt;
+            C.child[0] = -1;
+            C.child[1] = -1;
             next[co] = C;
         }
-        for (uint32_t k = threadIdx.x; k < nu; k += 64) {
-            Unit U;
-            U.begin = b + (uint64_t)k * unit;
-            uint64_t rem = cnt - (uint64_t)k * unit;
-            U.count = (uint32_t)(rem < unit ? rem : unit);
+        for (uint32_t k = threadIdx.x; k < nt; k += 64) {
+            Tile U;
+            U.begin = b + (uint64_t)k * kTile;
+            uint64_t rem = cnt - (uint64_t)k * kTile;
+            U.count = (uint32_t)(rem < (uint64_t)kTile ? rem : (uint64_t)kTile);
             U.part = (uint32_t)co;
-            next_units[uo + k] = U;
+            next_tiles[to + k] = U;
         }
         co++;
-        uo += nu;
+        to += nt;
     }
+    if (threadIdx.x == 0) {
+        parts[s].child[0] = id0;
+        parts[s].child[1] = id1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable partition of every part with a decoupled look-back over the part's
+// tiles, fused with the children's first split pass.
+//
+// Tiles are taken in ticket order, so a tile's predecessors are already being
+// processed.  Status word of a tile: bits 0-31 value, 32-33 flag
+// (1 aggregate: this tile's left count; 2 inclusive: left count of the part up
+// to and including this tile), 34-63 epoch of the launch.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1ull << 32;
+constexpr uint64_t kFlagInc = 2ull << 32;
+
+__device__ inline uint64_t lb_word(uint32_t epoch, uint64_t flag, uint32_t value) {
+    return ((uint64_t)epoch << 34) | flag | value;
+}
+
+template <int D, typename WS, typename Acc, bool FIRST>
+__global__ void __launch_bounds__(kBlock)
+k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+            unsigned long long *ticket, unsigned long long *status, uint32_t epoch, uint32_t *error,
+            const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
+            const WS *__restrict__ iw, const uint32_t *__restrict__ iix,
+            float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
+            WS *__restrict__ ow, uint32_t *__restrict__ oix, uint64_t n_total,
+            uint32_t *__restrict__ bcnt, Acc *__restrict__ bw, uint32_t *__restrict__ bmin) {
+    constexpr int NB = kFirstBins;
+    __shared__ uint64_t stage[kTile];
+    __shared__ uint32_t wtot[kItems][kBlock / 64];
+    __shared__ uint32_t s_tile_left, s_prefix, s_t;
+    __shared__ RegBinsShared<2 * NB, Acc> sh;
+    float *stage_f = reinterpret_cast<float *>(stage);
+    uint32_t *stage_u = reinterpret_cast<uint32_t *>(stage);
+    WS *stage_w = reinterpret_cast<WS *>(stage);
+
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nt = *n_tiles;
+    RegBins<2 * NB, Acc> r;
+    r.reset();
+    uint32_t cur = 0xffffffffu;
+    int32_t ch0 = -1, ch1 = -1;
+    int a2 = 0;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+
+    for (;;) {
+        if (tid == 0) s_t = (uint32_t)atomicAdd(ticket, 1ull);
+        __syncthreads();
+        const uint32_t t = s_t;
+        if (t >= nt) break;
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) { flush_bins<2 * NB, NB>(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
+            cur = T.part;
+            ch0 = P.child[0];
+            ch1 = P.child[1];
+            a2 = (int)((P.axis + 1) % D);
+            // both children keep the parent's box on the next axis
+            const float blo = a2 == 0 ? P.lo[0] : (a2 == 1 ? P.lo[1] : P.lo[2]);
+            const float bhi = a2 == 0 ? P.hi[0] : (a2 == 1 ? P.hi[1] : P.hi[2]);
+            e0 = candidate(blo, bhi, 0, NB);
+            e1 = candidate(blo, bhi, 1, NB);
+            e2 = candidate(blo, bhi, 2, NB);
+        }
+        const int ax = (int)P.axis;
+        const bool all_left = P.status == kAllLeft;
+        const float split = P.split;
+        const uint64_t end = T.begin + T.count;
+
+        float x0[kItems], x1[kItems], x2[kItems];
+        WS wv[kItems];
+        uint32_t xi[kItems];
+        bool f[kItems];
+        uint32_t pre[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
+            const bool valid = p < end;
+            x0[k] = valid ? i0[p] : 0.f;
+            x1[k] = valid ? i1[p] : 0.f;
+            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
+            wv[k] = valid ? iw[p] : (WS)0;
+            xi[k] = valid ? (FIRST ? (uint32_t)p : iix[p]) : 0u;
+            const float xa = sel3(ax, x0[k], x1[k], x2[k]);
+            // !(c >= t) is c < t for numbers and sends NaN left, as bin_of does
+            f[k] = valid && (all_left || !(xa >= split));
+            const uint64_t bal = __ballot(f[k]);
+            pre[k] = __popcll(bal & lt_mask);
+            if (lane == 0) wtot[k][wid] = __popcll(bal);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kItems; k++)
+                for (int v = 0; v < kBlock / 64; v++) { uint32_t c = wtot[k][v]; wtot[k][v] = run; run += c; }
+            s_tile_left = run;
+            // decoupled look-back within the part
+            uint64_t prefix = 0;
+            if (t == P.tile_begin) {
+                __hip_atomic_store(&status[t], lb_word(epoch, kFlagInc, run), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(&status[t], lb_word(epoch, kFlagAgg, run), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t j = t - 1;
+                uint32_t spins = 0;
+                for (;;) {
+                    const uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t fl = w & (3ull << 32);
+                    if ((uint32_t)(w >> 34) == epoch && fl != 0) {
+                        prefix += (uint32_t)w;
+                        if (fl == kFlagInc || j == P.tile_begin) break;
+                        j--;
+                    } else {
+                        // bounded wait: a lost predecessor becomes an error, not a hang
+                        if (++spins > (1u << 22)) { atomicOr(error, 1u); break; }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                __hip_atomic_store(&status[t], lb_word(epoch, kFlagInc, (uint32_t)(prefix + run)),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            s_prefix = (uint32_t)prefix;
+        }
+        __syncthreads();
+        const uint32_t tl = s_tile_left;
+        const uint64_t prefix = s_prefix;
+        const uint64_t lbase = P.begin + prefix;
+        const uint64_t rbase = P.begin + P.cnt_left + ((T.begin - P.begin) - prefix);
+        uint32_t cp[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t off = k * kBlock + tid;
+            const uint32_t rl = wtot[k][wid] + pre[k];
+            cp[k] = f[k] ? rl : tl + (off - rl);
+        }
+        const uint32_t cnt = T.count;
+        // Each array goes through LDS in output order, then out in contiguous runs.
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) stage_f[cp[k]] = a == 0 ? x0[k] : (a == 1 ? x1[k] : x2[k]);
+            __syncthreads();
+            float *o = a == 0 ? o0 : (a == 1 ? o1 : o2);
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const uint32_t pos = k * kBlock + tid;
+                if (pos < cnt) {
+                    const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
+                    if (d < n_total) o[d] = stage_f[pos];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++)
+            if (k * kBlock + tid < (int)cnt) stage_w[cp[k]] = wv[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t pos = k * kBlock + tid;
+            if (pos < cnt) {
+                const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
+                if (d < n_total) ow[d] = stage_w[pos];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++)
+            if (k * kBlock + tid < (int)cnt) stage_u[cp[k]] = xi[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t pos = k * kBlock + tid;
+            if (pos < cnt) {
+                const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
+                if (d < n_total) oix[d] = stage_u[pos];
+            }
+        }
+        // children's first pass: bin the next-axis coordinate
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            if (k * kBlock + tid < (int)cnt) {
+                const float xb = sel3(a2, x0[k], x1[k], x2[k]);
+                const int b = bin4(e0, e1, e2, xb) + (f[k] ? 0 : NB);
+                r.add(b, AccOps<Acc>::of(wv[k]), f32_key(xb));
+            }
+        }
+        __syncthreads();
+    }
+    if (cur != 0xffffffffu) flush_bins<2 * NB, NB>(r, sh, ch0, ch1, bcnt, bw, bmin);
 }
 
 // Smallest leaf heap id (:698-699).
@@ -621,25 +900,27 @@ k_leaf_min(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *mi
 }
 
 // Last level: part id of every point (:589-601) minus the offset (:700-702).
+template <bool FIRST>
 __global__ void __launch_bounds__(kBlock)
-k_write_ids(const Part *__restrict__ parts, const Unit *__restrict__ units, const uint32_t *n_units,
+k_write_ids(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
             const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
             const uint32_t *__restrict__ ix, const uint64_t *min_id, uint64_t *__restrict__ out,
             uint64_t n_total) {
-    const uint32_t u = blockIdx.x;
-    if (u >= *n_units) return;
-    const Unit un = units[u];
-    const Part &P = parts[un.part];
-    const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
-    const bool all_left = P.status == kAllLeft;
-    const float t = P.split;
+    const uint32_t nt = *n_tiles;
     const uint64_t off = *min_id;
-    const uint64_t idl = 2 * P.heap_id + 1 - off, idr = 2 * P.heap_id + 2 - off;
-    const uint64_t end = un.begin + un.count;
-    for (uint64_t i = un.begin + threadIdx.x; i < end; i += kBlock) {
-        bool l = all_left || !(c[i] >= t);
-        const uint32_t o = ix[i];
-        if (o < n_total) out[o] = l ? idl : idr;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
+        const bool all_left = P.status == kAllLeft;
+        const float split = P.split;
+        const uint64_t idl = 2 * P.heap_id + 1 - off, idr = 2 * P.heap_id + 2 - off;
+        const uint64_t end = T.begin + T.count;
+        for (uint64_t i = T.begin + threadIdx.x; i < end; i += kBlock) {
+            const bool l = all_left || !(c[i] >= split);
+            const uint64_t o = FIRST ? i : ix[i];
+            if (o < n_total) out[o] = l ? idl : idr;
+        }
     }
 }
 
@@ -650,10 +931,9 @@ __global__ void __launch_bounds__(kBlock) k_fill_u64(uint64_t *out, uint64_t n, 
 }
 
 // Root part: whole array, box from the prologue, sum in the accumulator.
-template <typename Acc>
-__global__ void k_init_root(Part *parts, uint32_t *n_parts, Unit *units, uint32_t *n_units,
-                            const Prologue *pro, uint64_t n, int dim, uint64_t unit, int real_mode) {
-    const uint32_t nu = units_for(n, unit);
+__global__ void k_init_root(Part *parts, uint32_t *n_parts, Tile *tiles, uint32_t *n_tiles,
+                            const Prologue *pro, uint64_t n, int dim, int real_mode) {
+    const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile);
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         Part R;
         R.begin = 0;
@@ -673,29 +953,31 @@ __global__ void k_init_root(Part *parts, uint32_t *n_parts, Unit *units, uint32_
         R.kstar = -1;
         R.wl = 0;
         R.cnt_left = 0;
-        R.unit_begin = 0;
-        R.unit_count = nu;
+        R.tile_begin = 0;
+        R.tile_count = nt;
+        R.child[0] = -1;
+        R.child[1] = -1;
         parts[0] = R;
         *n_parts = 1;
-        *n_units = nu;
+        *n_tiles = nt;
     }
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nu;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nt;
          k += (uint64_t)gridDim.x * blockDim.x) {
-        Unit U;
-        U.begin = k * unit;
-        uint64_t rem = n - k * unit;
-        U.count = (uint32_t)(rem < unit ? rem : unit);
+        Tile U;
+        U.begin = k * kTile;
+        uint64_t rem = n - k * kTile;
+        U.count = (uint32_t)(rem < (uint64_t)kTile ? rem : (uint64_t)kTile);
         U.part = 0;
-        units[k] = U;
+        tiles[k] = U;
     }
 }
 
 // Copies the 64-bit totals of the scans into the 32-bit table sizes.
-__global__ void k_set_sizes(const uint64_t *parts_total, const uint64_t *units_total,
-                            uint32_t *n_parts, uint32_t *n_units) {
+__global__ void k_set_sizes(const uint64_t *parts_total, const uint64_t *tiles_total, uint32_t *n_parts,
+                            uint32_t *n_tiles) {
     if (threadIdx.x == 0) {
         *n_parts = (uint32_t)*parts_total;
-        *n_units = (uint32_t)*units_total;
+        *n_tiles = (uint32_t)*tiles_total;
     }
 }
 
