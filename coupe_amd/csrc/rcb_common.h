@@ -56,6 +56,15 @@ struct Tile {
     uint32_t part;
 };
 
+// One split of the bisection tree, kept for the final descent that gives
+// every point its part id in input order.
+constexpr uint32_t kNodeAllLeft = 1u << 8;
+struct Node {
+    float split;           // split_pos
+    uint32_t meta;         // axis | kNodeAllLeft
+    int32_t child[2];      // node index of the left / right child (-1: empty or leaf level)
+};
+
 // Results of the prologue reductions.
 struct Prologue {
     uint64_t bmin[kMaxDim];     // ordered keys of the f64 bounding box

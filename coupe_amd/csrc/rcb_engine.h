@@ -41,14 +41,14 @@ struct Profiler {
 struct Context {
     int device = 0;
     std::mutex lock;                 // one call at a time per context
-    DevBuf cx[2][3];                 // SoA f32 coordinates, two sides
+    DevBuf cx[3][3];                 // SoA f32 coordinates: input order, then two sides
     DevBuf w[2];                     // stored weights, two sides
-    DevBuf ix[2];                    // original indices, two sides
     DevBuf parts[2], tiles[2];       // part / tile tables, current and next level
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
     DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
     DevBuf bsum;                     // scan block sums
     DevBuf status;                   // look-back status words, one per tile
+    DevBuf nodes, leaf;              // split tree of all levels, part ids of the leaves
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks

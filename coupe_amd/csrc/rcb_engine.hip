@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 #include <type_traits>
+#include <vector>
 
 #include "rcb_engine.h"
 #include "rcb_kernels.h"
@@ -169,6 +170,19 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
     }
     cx.nchild.ensure(parts_cap * sizeof(uint64_t));
     cx.ntiles.ensure(parts_cap * sizeof(uint64_t));
+    // Split tree: level l holds at most min(2^l, n) nodes, at node_base[l].
+    std::vector<uint64_t> node_base(iter_count + 1, 0);
+    {
+        uint64_t pub = 1;
+        for (size_t l = 0; l < iter_count; l++) {
+            node_base[l + 1] = node_base[l] + pub;
+            pub = std::min<uint64_t>(2 * pub, parts_cap);
+        }
+    }
+    const uint64_t n_nodes = node_base[iter_count];
+    if (n_nodes > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
+    cx.nodes.ensure(n_nodes * sizeof(Node));
+    cx.leaf.ensure(2 * parts_cap * sizeof(uint64_t));
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
     if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(iter_count, 1u << 20) - 2) {
@@ -201,19 +215,20 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
     Acc *bw = (Acc *)cx.bw.ptr;
     uint32_t *bmin = (uint32_t *)cx.bmin.ptr;
 
-    // Working arrays: slot 0 holds the prologue output, slot 1 the other side.
-    float *cs[2][3];
+    // Working arrays.  Coordinates: slot 0 is the prologue output in input
+    // order and is kept for the final descent; levels 1, 3, ... use slot 1,
+    // levels 2, 4, ... slot 2.  Weights alternate between two slots.
+    float *cs[3][3];
     WS *ws[2];
-    uint32_t *ixs[2];
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < 3; b++)
         for (int k = 0; k < 3; k++) cs[b][k] = k < D ? (float *)cx.cx[b][k].ptr : nullptr;
-        ixs[b] = (uint32_t *)cx.ix[b].ptr;
-    }
     ws[0] = w_first;
     ws[1] = (WS *)cx.w[1].ptr;
+    auto cslot = [](size_t level) { return level == 0 ? 0 : (level % 2 == 1 ? 1 : 2); };
+    auto wslot = [](size_t level) { return (int)(level % 2); };
 
     const int real_mode = std::is_same<Acc, double>::value;
-    const double payload = 4.0 * D + sizeof(WS) + 4.0;   // bytes of one point in the working arrays
+    const double payload = 4.0 * D + sizeof(WS);   // bytes of one point in the working arrays
     check(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
         k_init_root<<<grid_for(tiles_cap, kBlock), kBlock, 0, st>>>(
@@ -229,15 +244,17 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
 
     // Split search of the parts of slot `sl` after a first pass (k_sweep or
     // the fused partition) filled their kFirstBins bins.
-    auto search = [&](int sl, uint64_t parts_ub, uint64_t tiles_ub, bool first_standalone) {
+    auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, bool first_standalone) {
         Part *parts = (Part *)cx.parts[sl].ptr;
+        float *const *cl = cs[cslot(level)];
+        WS *wl = ws[wslot(level)];
         Tile *tiles = (Tile *)cx.tiles[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
         uint32_t *ntl = &ctr->n_tiles[sl];
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, kFirstBins, tol, wrap32,
-                                                           &ctr->searching,
+            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_standalone ? kFirstBins : 2,
+                                                           tol, wrap32, &ctr->searching,
                                                            first_standalone ? &ctr->swept : &ctr->swept_fused);
         });
         uint32_t passes = 1;
@@ -252,8 +269,8 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
             const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
             cx.stats.bins = nb;
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
-                k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[sl][0], cs[sl][1], cs[sl][2], ws[sl],
-                                                     nb, bcnt, bw, bmin);
+                k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
+                                                     bmin);
             });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, tol, wrap32,
@@ -275,7 +292,7 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
                                                   &ctr->n_tiles[0], cs[0][0], cs[0][1], cs[0][2], ws[0], bcnt, bw,
                                                   bmin);
         });
-        search(0, parts_ub, tiles_ub, true);
+        search(0, 0, parts_ub, tiles_ub, true);
     }
 
     int cur = 0;
@@ -286,27 +303,26 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         Tile *tiles = (Tile *)cx.tiles[cur].ptr;
         uint32_t *np = &ctr->n_parts[cur];
         uint32_t *ntl = &ctr->n_tiles[cur];
-        const bool first = level == 0;   // working arrays still in input order
 
         if (level + 1 == iter_count) {
+            // Last level: leaf ids, then every point walks the tree in input order.
             check(hipMemsetAsync(&ctr->min_id, 0xff, sizeof(uint64_t), st));
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_leaf_min<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &ctr->min_id);
+                k_last_nodes<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                    parts, np, node_base[level], &ctr->min_id, (Node *)cx.nodes.ptr, (uint64_t *)cx.leaf.ptr);
             });
-            // reads the axis coordinate and the index, writes the 8-byte id
-            const unsigned g = (unsigned)std::min<uint64_t>(tiles_ub, (uint64_t)cx.cu_count * 8);
-            timed(cx, st, COUPE_HIP_K_IDS, (first ? 12.0 : 16.0) * (double)n, [&] {
-                if (first)
-                    k_write_ids<true><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[cur][0], cs[cur][1], cs[cur][2],
-                                                          ixs[cur], &ctr->min_id, d_out, n);
-                else
-                    k_write_ids<false><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cs[cur][0], cs[cur][1], cs[cur][2],
-                                                           ixs[cur], &ctr->min_id, d_out, n);
+            const unsigned g = persistent_grid(cx, k_descend<D>, (n + kBlock - 1) / kBlock);
+            // reads the input-order f32 coordinates, writes the 8-byte id
+            timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
+                k_descend<D><<<g, kBlock, 0, st>>>((const Node *)cx.nodes.ptr, n_nodes, (uint32_t)iter_count,
+                                                   node_base[level], (const uint64_t *)cx.leaf.ptr, parts_ub,
+                                                   cs[0][0], cs[0][1], cs[0][2], n, d_out, &ctr->error);
             });
             break;
         }
 
-        // Children: next level's parts and tiles.
+        // Children: next level's parts and tiles, and this level's split nodes.
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
                                                                           (uint64_t *)cx.ntiles.ptr);
@@ -316,7 +332,8 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                 parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr);
+                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, (Node *)cx.nodes.ptr, node_base[level],
+                node_base[level + 1]);
             k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                           &ctr->n_tiles[nxt]);
         });
@@ -326,22 +343,19 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
         if (level > 0 && level % kMaxTickets == 0)
             check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
-        auto launch_partition = [&](auto kernel) {
-            const unsigned g = persistent_grid(cx, kernel, tiles_ub);
-            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n - (first ? 4.0 * n : 0.0), [&] {
-                kernel<<<g, kBlock, 0, st>>>(parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch,
-                                             &ctr->error, cs[cur][0], cs[cur][1], cs[cur][2], ws[cur], ixs[cur],
-                                             cs[nxt][0], cs[nxt][1], cs[nxt][2], ws[nxt], ixs[nxt], n, bcnt, bw,
-                                             bmin);
-            });
-        };
-        if (first) launch_partition(k_partition<D, WS, Acc, true>);
-        else launch_partition(k_partition<D, WS, Acc, false>);
+        float *const *ci = cs[cslot(level)];
+        float *const *co = cs[cslot(level + 1)];
+        const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
+        timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+            k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0], ci[1],
+                ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
+        });
 
         cur = nxt;
         parts_ub = std::min<uint64_t>(2 * parts_ub, parts_cap);
         tiles_ub = std::min<uint64_t>(tiles_cap, parts_ub + (n + kTile - 1) / kTile);
-        search(cur, parts_ub, tiles_ub, false);
+        search(cur, level + 1, parts_ub, tiles_ub, false);
     }
 
     // Standalone-pass bytes: swept points times (axis coordinate + stored weight).
@@ -379,11 +393,9 @@ static void prologue(Context &cx, hipStream_t st, const double *pts, const WIN *
 template <int D, typename WIN>
 static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, const double *pts, const WIN *win,
                       int wkind, size_t iter_count, double tol) {
-    // Working arrays for slot 0 (prologue output) and slot 1.
-    for (int b = 0; b < 2; b++) {
+    // Coordinate slots: 0 (prologue output, input order), 1 and 2 (levels).
+    for (int b = 0; b < 3; b++)
         for (int k = 0; k < D; k++) cx.cx[b][k].ensure(n * sizeof(float));
-        cx.ix[b].ensure(n * sizeof(uint32_t));
-    }
     cx.w[0].ensure(n * sizeof(int32_t));
     cx.prologue.ensure(sizeof(Prologue));
     cx.counters.ensure(sizeof(Counters));
@@ -486,19 +498,19 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
 
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
-    const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
-                           &ix[0], &ix[1], &parts[0], &parts[1], &tiles[0], &tiles[1], &bcnt, &bw, &bmin,
-                           &nchild, &ntiles, &bsum, &status, &prologue, &counters, &in_points, &in_weights,
-                           &out_ids};
+    const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
+                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
+                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &prologue,
+                           &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
 
 void Context::release_all() {
-    DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &w[0], &w[1],
-                     &ix[0], &ix[1], &parts[0], &parts[1], &tiles[0], &tiles[1], &bcnt, &bw, &bmin,
-                     &nchild, &ntiles, &bsum, &status, &prologue, &counters, &in_points, &in_weights,
-                     &out_ids};
+    DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
+                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
+                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &prologue,
+                     &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

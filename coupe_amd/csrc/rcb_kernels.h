@@ -16,7 +16,7 @@
 //                decoupled look-back across the part's tiles and LDS-staged
 //                contiguous stores; in the same sweep it bins the children's
 //                next-axis coordinates (their first split pass)
-//   k_write_ids  last level: part ids through the original index (:589-601,
+//   k_descend    part ids in input order by walking the split tree (:589-601,
 //                :698-702)
 #pragma once
 
@@ -618,7 +618,8 @@ template <typename Acc>
 __global__ void __launch_bounds__(64)
 k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
-                 Part *__restrict__ next, Tile *__restrict__ next_tiles) {
+                 Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
+                 uint64_t node_base, uint64_t next_node_base) {
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     const Part P = parts[s];
@@ -681,6 +682,12 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     if (threadIdx.x == 0) {
         parts[s].child[0] = id0;
         parts[s].child[1] = id1;
+        Node nd;                                  // this split, for the final descent
+        nd.split = P.split;
+        nd.meta = P.axis | (P.status == kAllLeft ? kNodeAllLeft : 0u);
+        nd.child[0] = id0 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id0);
+        nd.child[1] = id1 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id1);
+        nodes[node_base + s] = nd;
     }
 }
 
@@ -700,33 +707,102 @@ __device__ inline uint64_t lb_word(uint32_t epoch, uint64_t flag, uint32_t value
     return ((uint64_t)epoch << 34) | flag | value;
 }
 
-template <int D, typename WS, typename Acc, bool FIRST>
+// Fused first pass of the children: one candidate t2 (the midpoint of their
+// box on the next axis, the same for both children).  Per child only the
+// count and weight of {c < t2} and the smallest c >= t2 are needed by the
+// decision at t2, so the bins stay in 8 registers (nb = 2 layout).
+template <typename Acc>
+struct ChildBins {
+    uint32_t cnt0, cnt1, mn0, mn1;
+    Acc w0, w1;
+    __device__ __forceinline__ void reset() {
+        cnt0 = cnt1 = 0;
+        mn0 = mn1 = 0xffffffffu;
+        w0 = w1 = (Acc)0;
+    }
+    __device__ __forceinline__ void add(bool right_child, bool below, Acc wv, uint32_t key) {
+        const bool l0 = !right_child && below, l1 = right_child && below;
+        cnt0 += l0 ? 1u : 0u;
+        cnt1 += l1 ? 1u : 0u;
+        w0 += l0 ? wv : (Acc)0;
+        w1 += l1 ? wv : (Acc)0;
+        const uint32_t k0 = (!right_child && !below) ? key : 0xffffffffu;
+        const uint32_t k1 = (right_child && !below) ? key : 0xffffffffu;
+        mn0 = k0 < mn0 ? k0 : mn0;
+        mn1 = k1 < mn1 ? k1 : mn1;
+    }
+};
+
+template <typename Acc>
+struct ChildBinsShared {
+    uint32_t cnt[kBlock / 64][2];
+    Acc w[kBlock / 64][2];
+    uint32_t mn[kBlock / 64][2];
+};
+
+template <typename Acc>
+__device__ __forceinline__ void flush_child_bins(ChildBins<Acc> &r, ChildBinsShared<Acc> &sh, int32_t ch0,
+                                                 int32_t ch1, uint32_t *__restrict__ gcnt, Acc *__restrict__ gw,
+                                                 uint32_t *__restrict__ gmn) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t c0 = wave_sum(r.cnt0), c1 = wave_sum(r.cnt1);
+    const Acc x0 = wave_sum(r.w0), x1 = wave_sum(r.w1);
+    const uint32_t m0 = wave_min(r.mn0), m1 = wave_min(r.mn1);
+    if (lane == 0) {
+        sh.cnt[wid][0] = c0; sh.cnt[wid][1] = c1;
+        sh.w[wid][0] = x0; sh.w[wid][1] = x1;
+        sh.mn[wid][0] = m0; sh.mn[wid][1] = m1;
+    }
+    __syncthreads();
+    const int j = threadIdx.x;
+    if (j < 2) {
+        const int32_t ch = j == 0 ? ch0 : ch1;
+        uint32_t c = 0, m = 0xffffffffu;
+        Acc x = (Acc)0;
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; v++) {
+            c += sh.cnt[v][j];
+            x += sh.w[v][j];
+            m = sh.mn[v][j] < m ? sh.mn[v][j] : m;
+        }
+        if (ch >= 0) {
+            const uint64_t g = (uint64_t)ch * 2;
+            if (c) {
+                atomicAdd(&gcnt[g], c);
+                atomic_add_acc(&gw[g], x);
+            }
+            if (m != 0xffffffffu) atomicMin(&gmn[g + 1], m);
+        }
+    }
+    __syncthreads();
+}
+
+template <int D, typename WS, typename Acc>
 __global__ void __launch_bounds__(kBlock)
 k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
             unsigned long long *ticket, unsigned long long *status, uint32_t epoch, uint32_t *error,
             const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
-            const WS *__restrict__ iw, const uint32_t *__restrict__ iix,
-            float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
-            WS *__restrict__ ow, uint32_t *__restrict__ oix, uint64_t n_total,
-            uint32_t *__restrict__ bcnt, Acc *__restrict__ bw, uint32_t *__restrict__ bmin) {
-    constexpr int NB = kFirstBins;
-    __shared__ uint64_t stage[kTile];
+            const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
+            WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
+            uint32_t *__restrict__ bmin) {
+    __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
     __shared__ uint32_t wtot[kItems][kBlock / 64];
     __shared__ uint32_t s_tile_left, s_prefix, s_t;
-    __shared__ RegBinsShared<2 * NB, Acc> sh;
-    float *stage_f = reinterpret_cast<float *>(stage);
-    uint32_t *stage_u = reinterpret_cast<uint32_t *>(stage);
-    WS *stage_w = reinterpret_cast<WS *>(stage);
+    __shared__ ChildBinsShared<Acc> sh;
+    float *sf0 = reinterpret_cast<float *>(stage);
+    float *sf1 = sf0 + kTile;
+    WS *sw = reinterpret_cast<WS *>(stage);
+    WS *sw1 = reinterpret_cast<WS *>(sf1);       // second half, for 4-byte weights
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nt = *n_tiles;
-    RegBins<2 * NB, Acc> r;
+    ChildBins<Acc> r;
     r.reset();
     uint32_t cur = 0xffffffffu;
     int32_t ch0 = -1, ch1 = -1;
     int a2 = 0;
-    float e0 = 0.f, e1 = 0.f, e2 = 0.f;
+    float t2 = 0.f;
 
     for (;;) {
         if (tid == 0) s_t = (uint32_t)atomicAdd(ticket, 1ull);
@@ -736,17 +812,15 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (T.part != cur) {
-            if (cur != 0xffffffffu) { flush_bins<2 * NB, NB>(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
+            if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
             cur = T.part;
             ch0 = P.child[0];
             ch1 = P.child[1];
             a2 = (int)((P.axis + 1) % D);
-            // both children keep the parent's box on the next axis
-            const float blo = a2 == 0 ? P.lo[0] : (a2 == 1 ? P.lo[1] : P.lo[2]);
-            const float bhi = a2 == 0 ? P.hi[0] : (a2 == 1 ? P.hi[1] : P.hi[2]);
-            e0 = candidate(blo, bhi, 0, NB);
-            e1 = candidate(blo, bhi, 1, NB);
-            e2 = candidate(blo, bhi, 2, NB);
+            // both children keep the parent's box on the next axis (:613-616)
+            const float blo = sel3(a2, P.lo[0], P.lo[1], P.lo[2]);
+            const float bhi = sel3(a2, P.hi[0], P.hi[1], P.hi[2]);
+            t2 = (blo + bhi) / 2.0f;                     // their first candidate (:472)
         }
         const int ax = (int)P.axis;
         const bool all_left = P.status == kAllLeft;
@@ -755,7 +829,6 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
 
         float x0[kItems], x1[kItems], x2[kItems];
         WS wv[kItems];
-        uint32_t xi[kItems];
         bool f[kItems];
         uint32_t pre[kItems];
 #pragma unroll
@@ -766,10 +839,12 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
             x1[k] = valid ? i1[p] : 0.f;
             x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
             wv[k] = valid ? iw[p] : (WS)0;
-            xi[k] = valid ? (FIRST ? (uint32_t)p : iix[p]) : 0u;
-            const float xa = sel3(ax, x0[k], x1[k], x2[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
             // !(c >= t) is c < t for numbers and sends NaN left, as bin_of does
-            f[k] = valid && (all_left || !(xa >= split));
+            f[k] = p < end && (all_left || !(sel3(ax, x0[k], x1[k], x2[k]) >= split));
             const uint64_t bal = __ballot(f[k]);
             pre[k] = __popcll(bal & lt_mask);
             if (lane == 0) wtot[k][wid] = __popcll(bal);
@@ -813,6 +888,7 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
         const uint64_t prefix = s_prefix;
         const uint64_t lbase = P.begin + prefix;
         const uint64_t rbase = P.begin + P.cnt_left + ((T.begin - P.begin) - prefix);
+        const uint32_t cnt = T.count;
         uint32_t cp[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
@@ -820,62 +896,65 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
             const uint32_t rl = wtot[k][wid] + pre[k];
             cp[k] = f[k] ? rl : tl + (off - rl);
         }
-        const uint32_t cnt = T.count;
-        // Each array goes through LDS in output order, then out in contiguous runs.
+        // LDS holds the tile in output order; each store then covers
+        // contiguous runs (the left run, then the right run).
+#define COUPE_STORE_RUN(EXPR)                                                  \
+    _Pragma("unroll") for (int k = 0; k < kItems; k++) {                       \
+        const uint32_t pos = k * kBlock + tid;                                 \
+        if (pos < cnt) {                                                       \
+            const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);    \
+            if (d < n_total) { EXPR; }                                         \
+        }                                                                      \
+    }
+        // round 1: axes 0 and 1
 #pragma unroll
-        for (int a = 0; a < D; a++) {
+        for (int k = 0; k < kItems; k++)
+            if (k * kBlock + tid < (int)cnt) { sf0[cp[k]] = x0[k]; sf1[cp[k]] = x1[k]; }
+        __syncthreads();
+        COUPE_STORE_RUN(o0[d] = sf0[pos]; o1[d] = sf1[pos])
+        __syncthreads();
+        // round 2: axis 2 and 4-byte weights together; 8-byte weights alone
+        if constexpr (sizeof(WS) == 4) {
 #pragma unroll
             for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) stage_f[cp[k]] = a == 0 ? x0[k] : (a == 1 ? x1[k] : x2[k]);
-            __syncthreads();
-            float *o = a == 0 ? o0 : (a == 1 ? o1 : o2);
-#pragma unroll
-            for (int k = 0; k < kItems; k++) {
-                const uint32_t pos = k * kBlock + tid;
-                if (pos < cnt) {
-                    const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
-                    if (d < n_total) o[d] = stage_f[pos];
+                if (k * kBlock + tid < (int)cnt) {
+                    if constexpr (D == 3) sf0[cp[k]] = x2[k];
+                    sw1[cp[k]] = wv[k];
                 }
+            __syncthreads();
+            if constexpr (D == 3) {
+                COUPE_STORE_RUN(o2[d] = sf0[pos]; ow[d] = sw1[pos])
+            } else {
+                COUPE_STORE_RUN(ow[d] = sw1[pos])
             }
             __syncthreads();
-        }
+        } else {
+            if constexpr (D == 3) {
 #pragma unroll
-        for (int k = 0; k < kItems; k++)
-            if (k * kBlock + tid < (int)cnt) stage_w[cp[k]] = wv[k];
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            const uint32_t pos = k * kBlock + tid;
-            if (pos < cnt) {
-                const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
-                if (d < n_total) ow[d] = stage_w[pos];
+                for (int k = 0; k < kItems; k++)
+                    if (k * kBlock + tid < (int)cnt) sf0[cp[k]] = x2[k];
+                __syncthreads();
+                COUPE_STORE_RUN(o2[d] = sf0[pos])
+                __syncthreads();
             }
-        }
-        __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kItems; k++)
-            if (k * kBlock + tid < (int)cnt) stage_u[cp[k]] = xi[k];
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            const uint32_t pos = k * kBlock + tid;
-            if (pos < cnt) {
-                const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);
-                if (d < n_total) oix[d] = stage_u[pos];
-            }
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) sw[cp[k]] = wv[k];
+            __syncthreads();
+            COUPE_STORE_RUN(ow[d] = sw[pos])
+            __syncthreads();
         }
-        // children's first pass: bin the next-axis coordinate
+#undef COUPE_STORE_RUN
+        // children's first pass
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             if (k * kBlock + tid < (int)cnt) {
                 const float xb = sel3(a2, x0[k], x1[k], x2[k]);
-                const int b = bin4(e0, e1, e2, xb) + (f[k] ? 0 : NB);
-                r.add(b, AccOps<Acc>::of(wv[k]), f32_key(xb));
+                r.add(!f[k], !(xb >= t2), AccOps<Acc>::of(wv[k]), f32_key(xb));
             }
         }
-        __syncthreads();
     }
-    if (cur != 0xffffffffu) flush_bins<2 * NB, NB>(r, sh, ch0, ch1, bcnt, bw, bmin);
+    if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
 }
 
 // Smallest leaf heap id (:698-699).
@@ -899,28 +978,60 @@ k_leaf_min(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *mi
     if (lane_id() == 0 && v != ~0ull) atomicMin((unsigned long long *)min_id, (unsigned long long)v);
 }
 
-// Last level: part id of every point (:589-601) minus the offset (:700-702).
-template <bool FIRST>
+// Split records of the last level and the part id of each of its leaves
+// (leaf heap index minus the smallest one, :589-601, :698-702).
 __global__ void __launch_bounds__(kBlock)
-k_write_ids(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
-            const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
-            const uint32_t *__restrict__ ix, const uint64_t *min_id, uint64_t *__restrict__ out,
-            uint64_t n_total) {
-    const uint32_t nt = *n_tiles;
+k_last_nodes(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t node_base,
+             const uint64_t *min_id, Node *__restrict__ nodes, uint64_t *__restrict__ leaf) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= *n_parts) return;
+    const Part &P = parts[s];
     const uint64_t off = *min_id;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
-        const Tile T = tiles[t];
-        const Part &P = parts[T.part];
-        const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
-        const bool all_left = P.status == kAllLeft;
-        const float split = P.split;
-        const uint64_t idl = 2 * P.heap_id + 1 - off, idr = 2 * P.heap_id + 2 - off;
-        const uint64_t end = T.begin + T.count;
-        for (uint64_t i = T.begin + threadIdx.x; i < end; i += kBlock) {
-            const bool l = all_left || !(c[i] >= split);
-            const uint64_t o = FIRST ? i : ix[i];
-            if (o < n_total) out[o] = l ? idl : idr;
+    Node nd;
+    nd.split = P.split;
+    nd.meta = P.axis | (P.status == kAllLeft ? kNodeAllLeft : 0u);
+    nd.child[0] = -1;
+    nd.child[1] = -1;
+    nodes[node_base + s] = nd;
+    leaf[2 * (uint64_t)s] = 2 * P.heap_id + 1 - off;
+    leaf[2 * (uint64_t)s + 1] = 2 * P.heap_id + 2 - off;
+}
+
+// Part ids in input order: every point walks the split tree from the root
+// with its own coordinates (the prologue's SoA copy), taking the side the
+// partitions gave it.  The top of the tree is cached in LDS.
+constexpr int kNodeCache = 2048;
+
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_descend(const Node *__restrict__ nodes, uint64_t n_nodes, uint32_t levels, uint64_t last_base,
+          const uint64_t *__restrict__ leaf, uint64_t n_leaf_parts, const float *__restrict__ c0,
+          const float *__restrict__ c1, const float *__restrict__ c2, uint64_t n, uint64_t *__restrict__ out,
+          uint32_t *error) {
+    __shared__ Node cache[kNodeCache];
+    const uint32_t cached = (uint32_t)(n_nodes < (uint64_t)kNodeCache ? n_nodes : (uint64_t)kNodeCache);
+    for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = nodes[j];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const float x0 = c0[i], x1 = c1[i], x2 = D == 3 ? c2[i] : 0.f;
+        uint64_t node = 0;
+        uint64_t id = ~0ull;
+        for (uint32_t l = 0; l < levels; l++) {
+            const Node nd = node < cached ? cache[node] : nodes[node];
+            const bool all_left = (nd.meta & kNodeAllLeft) != 0;
+            const int side = (all_left || !(sel3((int)(nd.meta & 3u), x0, x1, x2) >= nd.split)) ? 0 : 1;
+            if (l + 1 == levels) {
+                const uint64_t s = node - last_base;
+                if (node >= last_base && s < n_leaf_parts) id = leaf[2 * s + side];
+                break;
+            }
+            const int32_t next = side ? nd.child[1] : nd.child[0];
+            if (next < 0 || (uint64_t)next >= n_nodes) break;   // cannot happen for a consistent tree
+            node = (uint64_t)next;
         }
+        if (id == ~0ull) atomicOr(error, 2u);
+        out[i] = id;
     }
 }
 
