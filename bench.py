@@ -57,6 +57,8 @@ def main():
     ap.add_argument("--workload", default="jgrid512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the partitioner runs on a "
@@ -84,6 +86,25 @@ def main():
 
     def step():
         rcb.partition(d_out, (d_pts, d_w), context=ctx, stream=stream.cuda_stream)
+
+    copy_probe = None
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        c0.record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        c1.record(stream)
+        torch.cuda.synchronize()
+        ms = c0.elapsed_time(c1) / reps
+        copy_probe = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": round(ms, 4),
+                      "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+        del src, dst
+        log(f"copy probe {args.copy_probe_mib} MiB: {ms:.3f} ms, {copy_probe['GBps_read_plus_write']} GB/s")
 
     for i in range(args.warmup):
         step()
@@ -205,6 +226,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "whole_job": whole_job,
+        "copy_probe": copy_probe,
         "step_ms": {"median": round(step_ms[len(step_ms) // 2], 4), "min": round(step_ms[0], 4),
                     "max": round(step_ms[-1], 4), "profiled_median": round(sorted(prof_ms)[len(prof_ms) // 2], 4)},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}

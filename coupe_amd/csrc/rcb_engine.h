@@ -49,6 +49,7 @@ struct Context {
     DevBuf bsum;                     // scan block sums
     DevBuf status;                   // look-back status words, one per tile
     DevBuf nodes, leaf;              // split tree of all levels, part ids of the leaves
+    DevBuf heap_split;               // split values by heap index (shallow trees)
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks

@@ -183,6 +183,10 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
     if (n_nodes > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
     cx.nodes.ensure(n_nodes * sizeof(Node));
     cx.leaf.ensure(2 * parts_cap * sizeof(uint64_t));
+    // Dense heap of split values for shallow trees (the fast descent).
+    const bool dense = iter_count <= (size_t)kDenseLevels;
+    if (dense) cx.heap_split.ensure(((1ull << iter_count) - 1) * sizeof(float));
+    float *heap_split = dense ? (float *)cx.heap_split.ptr : nullptr;
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
     if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(iter_count, 1u << 20) - 2) {
@@ -310,15 +314,24 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_leaf_min<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &ctr->min_id);
                 k_last_nodes<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
-                    parts, np, node_base[level], &ctr->min_id, (Node *)cx.nodes.ptr, (uint64_t *)cx.leaf.ptr);
+                    parts, np, node_base[level], &ctr->min_id, (Node *)cx.nodes.ptr, (uint64_t *)cx.leaf.ptr,
+                    heap_split);
             });
-            const unsigned g = persistent_grid(cx, k_descend<D>, (n + kBlock - 1) / kBlock);
             // reads the input-order f32 coordinates, writes the 8-byte id
-            timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
-                k_descend<D><<<g, kBlock, 0, st>>>((const Node *)cx.nodes.ptr, n_nodes, (uint32_t)iter_count,
-                                                   node_base[level], (const uint64_t *)cx.leaf.ptr, parts_ub,
-                                                   cs[0][0], cs[0][1], cs[0][2], n, d_out, &ctr->error);
-            });
+            if (dense) {
+                const unsigned g = persistent_grid(cx, k_descend_dense<D>, (n + kBlock - 1) / kBlock);
+                timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
+                    k_descend_dense<D><<<g, kBlock, 0, st>>>(heap_split, (uint32_t)iter_count, &ctr->min_id,
+                                                             cs[0][0], cs[0][1], cs[0][2], n, d_out);
+                });
+            } else {
+                const unsigned g = persistent_grid(cx, k_descend<D>, (n + kBlock - 1) / kBlock);
+                timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
+                    k_descend<D><<<g, kBlock, 0, st>>>((const Node *)cx.nodes.ptr, n_nodes, (uint32_t)iter_count,
+                                                       node_base[level], (const uint64_t *)cx.leaf.ptr, parts_ub,
+                                                       cs[0][0], cs[0][1], cs[0][2], n, d_out, &ctr->error);
+                });
+            }
             break;
         }
 
@@ -333,7 +346,7 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
             k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                 parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                 (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, (Node *)cx.nodes.ptr, node_base[level],
-                node_base[level + 1]);
+                node_base[level + 1], heap_split);
             k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                           &ctr->n_tiles[nxt]);
         });
@@ -500,7 +513,7 @@ uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
-                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &prologue,
+                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split, &prologue,
                            &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
@@ -509,7 +522,7 @@ uint64_t Context::device_bytes() const {
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
-                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &prologue,
+                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split, &prologue,
                      &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);

@@ -601,6 +601,12 @@ k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const 
 // and right (2i+2, box lo = split, sum S - wl), kept when non-empty
 // (:586-588, :603-641).  Their split search starts on the next axis.
 // ---------------------------------------------------------------------------
+// Split value of a node in the dense heap of the descent: NaN when every
+// point went left, since !(c >= NaN) holds for every c.
+__device__ inline float dense_split(const Part &P) {
+    return P.status == kAllLeft ? __builtin_nanf("") : P.split;
+}
+
 __device__ inline uint32_t tiles_for(uint64_t count) { return (uint32_t)((count + kTile - 1) / kTile); }
 
 __global__ void __launch_bounds__(kBlock)
@@ -619,7 +625,7 @@ __global__ void __launch_bounds__(64)
 k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
-                 uint64_t node_base, uint64_t next_node_base) {
+                 uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split) {
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     const Part P = parts[s];
@@ -688,6 +694,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
         nd.child[0] = id0 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id0);
         nd.child[1] = id1 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id1);
         nodes[node_base + s] = nd;
+        if (heap_split) heap_split[P.heap_id] = dense_split(P);
     }
 }
 
@@ -982,7 +989,8 @@ k_leaf_min(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *mi
 // (leaf heap index minus the smallest one, :589-601, :698-702).
 __global__ void __launch_bounds__(kBlock)
 k_last_nodes(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t node_base,
-             const uint64_t *min_id, Node *__restrict__ nodes, uint64_t *__restrict__ leaf) {
+             const uint64_t *min_id, Node *__restrict__ nodes, uint64_t *__restrict__ leaf,
+             float *__restrict__ heap_split) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= *n_parts) return;
     const Part &P = parts[s];
@@ -995,6 +1003,7 @@ k_last_nodes(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t n
     nodes[node_base + s] = nd;
     leaf[2 * (uint64_t)s] = 2 * P.heap_id + 1 - off;
     leaf[2 * (uint64_t)s + 1] = 2 * P.heap_id + 2 - off;
+    if (heap_split) heap_split[P.heap_id] = dense_split(P);
 }
 
 // Part ids in input order: every point walks the split tree from the root
@@ -1032,6 +1041,46 @@ k_descend(const Node *__restrict__ nodes, uint64_t n_nodes, uint32_t levels, uin
         }
         if (id == ~0ull) atomicOr(error, 2u);
         out[i] = id;
+    }
+}
+
+// Dense variant for iter_count <= kDenseLevels: the split of node h (heap
+// index) is heap_split[h], level l splits axis l mod D (:624, :636, :696),
+// the children of h are 2h+1 and 2h+2, and the leaf's heap index minus the
+// offset is the part id.  The top kDenseCached levels sit in LDS.
+constexpr int kDenseLevels = 24;
+constexpr int kDenseCached = 13;   // 8191 splits, 32 KB of LDS
+
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_descend_dense(const float *__restrict__ heap_split, uint32_t levels, const uint64_t *min_id,
+                const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
+                uint64_t n, uint64_t *__restrict__ out) {
+    __shared__ float cache[(1u << kDenseCached) - 1];
+    const uint32_t cached_levels = levels < (uint32_t)kDenseCached ? levels : (uint32_t)kDenseCached;
+    const uint32_t cached = (1u << cached_levels) - 1;
+    for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = heap_split[j];
+    __syncthreads();
+    const uint64_t off = *min_id;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const float x[3] = {c0[i], c1[i], D == 3 ? c2[i] : 0.f};
+        uint32_t h = 0;
+        int a = 0;
+#pragma unroll 4
+        for (uint32_t l = 0; l < cached_levels; l++) {
+            const float t = cache[h];
+            const float xa = sel3(a, x[0], x[1], x[2]);
+            h = 2 * h + (xa >= t ? 2u : 1u);
+            a = a + 1 == D ? 0 : a + 1;
+        }
+        for (uint32_t l = cached_levels; l < levels; l++) {
+            const float t = heap_split[h];
+            const float xa = sel3(a, x[0], x[1], x[2]);
+            h = 2 * h + (xa >= t ? 2u : 1u);
+            a = a + 1 == D ? 0 : a + 1;
+        }
+        out[i] = (uint64_t)h - off;
     }
 }
 
