@@ -27,7 +27,8 @@ import numpy as np
 from . import _native
 from ._native import strerror
 
-__all__ = ["Rcb", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror"]
+__all__ = ["Rcb", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
+           "partition_batch"]
 
 
 class Error(Exception):
@@ -197,3 +198,45 @@ class Rcb(Partition):
         if err != _native.OK:
             raise CoupeError(err)
         return None
+
+
+def partition_batch(part_ids, points, weights, instances, *, stream=None, context: DeviceContext = None):
+    """Batched RCB on device tensors (coupe_hip_rcb_batch, include/coupe_hip.h).
+
+    ``instances`` is a sequence of ``(n, iter_count, tolerance)`` (or of
+    ``(n, Rcb)``).  Instance q owns the next n rows of ``points`` (N, D)
+    float64, of ``weights`` and of ``part_ids`` (int64/uint64), in order;
+    its ids are those ``Rcb(iter_count, tolerance)`` gives it alone.
+    """
+    import torch
+
+    specs = []
+    for it in instances:
+        if len(it) == 2 and isinstance(it[1], Rcb):
+            specs.append((int(it[0]), int(it[1].iter_count), float(it[1].tolerance)))
+        else:
+            n, iters, tol = it
+            specs.append((int(n), int(iters), float(tol)))
+    total = sum(n for n, _, _ in specs)
+    for name, t in (("part_ids", part_ids), ("points", points), ("weights", weights)):
+        if not _is_device_tensor(t):
+            raise TypeError(f"{name} must be a device tensor")
+        if t.shape[0] != total:
+            raise InputLenMismatch(total, t.shape[0])
+        if not t.is_contiguous():
+            raise TypeError(f"{name} must be contiguous")
+    if points.dtype != torch.float64:
+        raise TypeError("points must be float64")
+    arr = (_native.Instance * max(len(specs), 1))()
+    for q, (n, iters, tol) in enumerate(specs):
+        arr[q].n, arr[q].iter_count, arr[q].tolerance = n, iters, tol
+    dev = points.device.index or 0
+    ctx = context or _context_for(dev)
+    st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
+    dim = points.shape[1] if points.dim() == 2 else 0
+    err = _native.lib().coupe_hip_rcb_batch(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), dim,
+                                            points.data_ptr(), _weight_type(weights.dtype), weights.data_ptr(),
+                                            arr, len(specs))
+    if err != _native.OK:
+        raise CoupeError(err)
+    return None

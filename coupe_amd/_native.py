@@ -21,9 +21,13 @@ COUPE_INT, COUPE_INT64, COUPE_DOUBLE = 0, 1, 2
 EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb",
-    "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_last_stats",
-    "coupe_hip_set_profiling", "coupe_hip_kernel_times",
+    "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_batch",
+    "coupe_hip_last_stats", "coupe_hip_set_profiling", "coupe_hip_kernel_times",
 ]
+
+
+class Instance(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("iter_count", C.c_size_t), ("tolerance", C.c_double)]
 
 KERNELS = ["prologue", "hist", "decide", "partition", "ids", "other"]
 
@@ -87,6 +91,9 @@ def lib():
         L.coupe_hip_rcb_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                            C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_double]
         L.coupe_hip_rcb_device.restype = C.c_int
+        L.coupe_hip_rcb_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.POINTER(Instance), C.c_size_t]
+        L.coupe_hip_rcb_batch.restype = C.c_int
         L.coupe_hip_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.coupe_hip_last_stats.restype = C.c_int
         L.coupe_hip_set_profiling.argtypes = [C.c_void_p, C.c_int]

@@ -164,6 +164,30 @@ extern "C" enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream,
                                  d_weights, iter_count, tolerance);
 }
 
+extern "C" enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,
+                                              uintptr_t dimension, const double *d_points,
+                                              enum coupe_type weight_type, const void *d_weights,
+                                              const struct coupe_hip_instance *instances, uintptr_t n_instances) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    if (n_instances > 0 && !instances) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    try {
+        std::vector<uint64_t> counts(n_instances);
+        std::vector<size_t> levels(n_instances);
+        std::vector<double> tols(n_instances);
+        for (uintptr_t q = 0; q < n_instances; q++) {
+            counts[q] = instances[q].n;
+            levels[q] = instances[q].iter_count;
+            tols[q] = instances[q].tolerance;
+        }
+        return coupe_hip::rcb_batch(*c, (hipStream_t)stream, d_partition, dimension, d_points, weight_type,
+                                    d_weights, counts.data(), levels.data(), tols.data(), n_instances);
+    } catch (const std::bad_alloc &) {
+        return COUPE_ERR_ALLOC;
+    }
+}
+
 extern "C" enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out) {
     if (!ctx || !out) return COUPE_ERR_CRASH;
     *out = reinterpret_cast<const Context *>(ctx)->stats;

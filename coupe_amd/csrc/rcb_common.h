@@ -24,6 +24,7 @@ enum : uint32_t {
     kSearching = 0,   // split position not found yet
     kSplitAtT = 1,    // split found: left = {c < t}
     kAllLeft = 2,     // no point at or right of t twice: every point goes left
+    kDone = 3,        // nothing to split (iter_count 0 or no point)
 };
 
 // One part of the current level.  `sum` and `wl` hold the bits of the
@@ -47,6 +48,28 @@ struct Part {
     uint32_t tile_begin;   // first tile of this part in the tile table
     uint32_t tile_count;
     int32_t child[2];      // indices of the left / right child in the next level (-1: empty)
+    uint32_t inst;         // instance of a batch (0 for a single call)
+    uint32_t depth;        // level of this part in its instance's tree
+    uint32_t levels;       // iter_count of its instance
+    uint32_t pad;
+    double tol;            // tolerance of its instance
+};
+
+// One independent RCB problem of a batch: its points are the range
+// [begin, begin + count) of the packed arrays.
+struct Instance {
+    uint64_t begin;
+    uint64_t count;
+    uint32_t levels;       // iter_count
+    uint32_t tile_begin;   // its first tile at level 0
+    double tol;
+    uint64_t heap_base;    // its dense split heap: heap_split[heap_base + h]
+    // filled on the device
+    uint64_t bmin[kMaxDim];     // ordered keys of the f64 bounding box
+    uint64_t bmax[kMaxDim];
+    uint64_t sum_u64;           // wrapping integer sum of the weights
+    double sum_f64;             // f64 sum (real weights)
+    uint64_t min_id;            // smallest leaf heap id (the id offset)
 };
 
 // A tile: the range [begin, begin + count) of part `part`, count <= kTile.
@@ -65,12 +88,8 @@ struct Node {
     int32_t child[2];      // node index of the left / right child (-1: empty or leaf level)
 };
 
-// Results of the prologue reductions.
+// Results of the prologue reductions over the whole batch.
 struct Prologue {
-    uint64_t bmin[kMaxDim];     // ordered keys of the f64 bounding box
-    uint64_t bmax[kMaxDim];
-    uint64_t sum_u64;           // wrapping integer sum of the weights
-    double sum_f64;             // f64 sum (real weights)
     double sum_abs;             // sum of |w| (exactness guard)
     uint64_t max_abs_bits;      // bits of max |w| (non-negative doubles order as integers)
     uint32_t non_integral;      // any weight with a fractional part or |w| > 2^53

@@ -50,6 +50,8 @@ struct Context {
     DevBuf status;                   // look-back status words, one per tile
     DevBuf nodes, leaf;              // split tree of all levels, part ids of the leaves
     DevBuf heap_split;               // split values by heap index (shallow trees)
+    DevBuf instances;                // instance table of the current call
+    DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
@@ -64,6 +66,12 @@ struct Context {
 };
 
 enum coupe_err map_hip_error(hipError_t e);
+
+// RCB of a batch of independent instances packed in input order (see
+// coupe_hip_rcb_batch).  counts / levels / tols: one entry per instance.
+enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
+                         int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
+                         const double *tols, size_t n_inst);
 
 // RCB on device-resident inputs (see coupe_hip_rcb_device).
 enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,

@@ -152,57 +152,108 @@ static unsigned persistent_grid(Context &cx, K kernel, uint64_t work_items) {
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// Host view of a batch: the instance table and the bounds derived from it.
+struct BatchPlan {
+    std::vector<Instance> inst;   // begin, count, levels, tile_begin, tol, heap_base
+    uint64_t n = 0;               // points of all instances
+    uint32_t max_levels = 0;
+    uint64_t tiles0 = 0;          // level-0 tiles
+    std::vector<uint64_t> parts_ub;   // parts at each level, upper bound
+    uint64_t parts_cap = 1;
+    uint64_t tiles_cap = 1;
+    uint64_t heap_total = 0;      // dense split heap entries
+    bool dense = true;            // every instance has iter_count <= kDenseLevels
+};
+
+static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const double *tols, size_t n_inst) {
+    BatchPlan b;
+    b.inst.resize(n_inst);
+    uint64_t begin = 0, tile = 0;
+    for (size_t q = 0; q < n_inst; q++) {
+        Instance &I = b.inst[q];
+        memset(&I, 0, sizeof I);
+        I.begin = begin;
+        I.count = counts[q];
+        I.levels = (uint32_t)std::min<size_t>(levels[q], 0xffffffffu);
+        I.tile_begin = (uint32_t)tile;
+        I.tol = tols[q];
+        begin += counts[q];
+        tile += (counts[q] + kTile - 1) / kTile;
+        if (counts[q] > 0) b.max_levels = std::max<uint32_t>(b.max_levels, I.levels);
+        if (I.levels > (uint32_t)kDenseLevels) b.dense = false;
+    }
+    b.n = begin;
+    b.tiles0 = tile;
+    // parts at level l: one root per instance at level 0, then at most
+    // min(2^l, n_i) for the instances still splitting
+    b.parts_ub.assign((size_t)b.max_levels + 1, 0);
+    b.parts_ub[0] = n_inst;
+    for (size_t q = 0; q < n_inst; q++) {
+        const uint64_t c = counts[q];
+        uint64_t p = 1;
+        for (uint32_t l = 1; l < b.inst[q].levels && c > 0; l++) {
+            p = std::min<uint64_t>(2 * p, c);
+            b.parts_ub[l] += p;
+        }
+    }
+    for (uint64_t v : b.parts_ub) b.parts_cap = std::max<uint64_t>(b.parts_cap, v);
+    b.tiles_cap = b.parts_cap + (b.n + kTile - 1) / kTile;
+    if (b.dense) {
+        for (auto &I : b.inst) {
+            I.heap_base = b.heap_total;
+            b.heap_total += (I.levels ? (1ull << I.levels) - 1 : 0);
+        }
+    }
+    return b;
+}
+
 template <int D, typename WS, typename Acc>
-static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_count, double tol, int wrap32,
-                       uint64_t *d_out, WS *w_first) {
+static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap32, uint64_t *d_out, WS *w_first) {
     Counters *ctr = (Counters *)cx.counters.ptr;
     uint32_t *h_search = (uint32_t *)cx.host_scratch;
+    Instance *insts = (Instance *)cx.instances.ptr;
+    const uint32_t n_inst = (uint32_t)B.inst.size();
+    const uint64_t n = B.n;
+    const size_t L = B.max_levels;
+    const uint64_t parts_cap = B.parts_cap, tiles_cap = B.tiles_cap;
+    if (tiles_cap > 0xffffffffull || parts_cap > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
 
-    // Table capacities: parts at depth l <= min(2^l, n); tiles <= parts + n / kTile.
-    uint64_t parts_cap = 1;
-    for (size_t l = 0; l < iter_count && parts_cap < n; l++) parts_cap *= 2;
-    if (parts_cap > n) parts_cap = n;
-    const uint64_t tiles_cap = parts_cap + (n + kTile - 1) / kTile;
-    if (tiles_cap > 0xffffffffull) throw HipFailure{hipErrorOutOfMemory};
     for (int b = 0; b < 2; b++) {
         cx.parts[b].ensure(parts_cap * sizeof(Part));
         cx.tiles[b].ensure(tiles_cap * sizeof(Tile));
     }
     cx.nchild.ensure(parts_cap * sizeof(uint64_t));
     cx.ntiles.ensure(parts_cap * sizeof(uint64_t));
-    // Split tree: level l holds at most min(2^l, n) nodes, at node_base[l].
-    std::vector<uint64_t> node_base(iter_count + 1, 0);
-    {
-        uint64_t pub = 1;
-        for (size_t l = 0; l < iter_count; l++) {
-            node_base[l + 1] = node_base[l] + pub;
-            pub = std::min<uint64_t>(2 * pub, parts_cap);
-        }
+    // Split tree for the descent: a dense heap per instance for shallow
+    // trees; node records with child links otherwise (single instance).
+    const bool dense = B.dense;
+    std::vector<uint64_t> node_base(L + 1, 0);
+    uint64_t n_nodes = 0;
+    if (dense) {
+        cx.heap_split.ensure(std::max<uint64_t>(B.heap_total, 1) * sizeof(float));
+    } else {
+        for (size_t l = 0; l < L; l++) node_base[l + 1] = node_base[l] + B.parts_ub[l];
+        n_nodes = node_base[L];
+        if (n_nodes > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
+        cx.nodes.ensure(n_nodes * sizeof(Node));
+        cx.leaf.ensure(2 * parts_cap * sizeof(uint64_t));
     }
-    const uint64_t n_nodes = node_base[iter_count];
-    if (n_nodes > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
-    cx.nodes.ensure(n_nodes * sizeof(Node));
-    cx.leaf.ensure(2 * parts_cap * sizeof(uint64_t));
-    // Dense heap of split values for shallow trees (the fast descent).
-    const bool dense = iter_count <= (size_t)kDenseLevels;
-    if (dense) cx.heap_split.ensure(((1ull << iter_count) - 1) * sizeof(float));
     float *heap_split = dense ? (float *)cx.heap_split.ptr : nullptr;
+    Node *nodes = dense ? nullptr : (Node *)cx.nodes.ptr;
+    // Level 0 has its own tile table: the prologue and the final descent
+    // read it, while the two ping-pong tables serve levels 1, 2, ...
+    Tile *tiles0 = (Tile *)cx.tiles0.ptr;
+
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
-    if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(iter_count, 1u << 20) - 2) {
+    if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(L, 1u << 20) - 2) {
         check(hipMemsetAsync(cx.status.ptr, 0, cx.status.bytes, st));
         cx.epoch = 0;
     }
     // Bins: parts x bins of the widest pass of any level.
     uint64_t bins = 1;
-    {
-        uint64_t pub = 1;
-        for (size_t level = 0; level < iter_count; level++) {
-            bins = std::max<uint64_t>(bins, pub * (uint64_t)choose_bins(pub));
-            if (pub >= parts_cap) break;
-            pub = std::min<uint64_t>(2 * pub, parts_cap);
-        }
-    }
+    for (size_t l = 0; l < L; l++)
+        bins = std::max<uint64_t>(bins, std::max<uint64_t>(B.parts_ub[l], 1) * (uint64_t)choose_bins(B.parts_ub[l]));
     const bool fresh_bins = cx.bcnt.bytes < bins * 4 || cx.bw.bytes < bins * 8 || cx.bmin.bytes < bins * 4;
     cx.bcnt.ensure(bins * 4);
     cx.bw.ensure(bins * 8);
@@ -235,9 +286,8 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
     const double payload = 4.0 * D + sizeof(WS);   // bytes of one point in the working arrays
     check(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_init_root<<<grid_for(tiles_cap, kBlock), kBlock, 0, st>>>(
-            (Part *)cx.parts[0].ptr, &ctr->n_parts[0], (Tile *)cx.tiles[0].ptr, &ctr->n_tiles[0],
-            (const Prologue *)cx.prologue.ptr, n, D, real_mode);
+        k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(insts, n_inst, (Part *)cx.parts[0].ptr, D,
+                                                                    real_mode);
     });
 
     cx.stats.levels = 0;
@@ -247,22 +297,25 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
     cx.stats.bins = kFirstBins;
 
     // Split search of the parts of slot `sl` after a first pass (k_sweep or
-    // the fused partition) filled their kFirstBins bins.
-    auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, bool first_standalone) {
+    // the fused partition) filled their first bins (kFirstBins or 2).
+    auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, uint64_t n_parts_host,
+                      bool first_standalone) {
         Part *parts = (Part *)cx.parts[sl].ptr;
         float *const *cl = cs[cslot(level)];
         WS *wl = ws[wslot(level)];
-        Tile *tiles = (Tile *)cx.tiles[sl].ptr;
+        Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
         uint32_t *ntl = &ctr->n_tiles[sl];
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_standalone ? kFirstBins : 2,
-                                                           tol, wrap32, &ctr->searching,
+            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin,
+                                                           first_standalone ? kFirstBins : 2, wrap32,
+                                                           &ctr->searching,
                                                            first_standalone ? &ctr->swept : &ctr->swept_fused);
         });
         uint32_t passes = 1;
         const int nb = choose_bins(parts_ub);
+        (void)n_parts_host;
         for (;;) {
             check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             check(hipStreamSynchronize(st));
@@ -277,7 +330,7 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
                                                      bmin);
             });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, tol, wrap32,
+                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
                                                                &ctr->searching, &ctr->swept);
             });
             passes++;
@@ -286,56 +339,35 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
     };
 
-    // Level 0: register sweep of the root, then its split search.
-    uint64_t parts_ub = 1;
-    uint64_t tiles_ub = (n + kTile - 1) / kTile;
+    // Level 0: the root parts' tables come from k_init_roots (in prologue());
+    // register sweep of the roots, then their split search.
+    check(hipMemcpyAsync(&ctr->n_parts[0], &n_inst, sizeof(uint32_t), hipMemcpyHostToDevice, st));
     {
+        const uint32_t t0 = (uint32_t)B.tiles0;
+        check(hipMemcpyAsync(&ctr->n_tiles[0], &t0, sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    uint64_t parts_ub = n_inst;
+    uint64_t tiles_ub = std::max<uint64_t>(B.tiles0, 1);
+    if (L > 0) {
         const unsigned g = persistent_grid(cx, k_sweep<WS, Acc>, tiles_ub);
         timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
-            k_sweep<WS, Acc><<<g, kBlock, 0, st>>>((const Part *)cx.parts[0].ptr, (const Tile *)cx.tiles[0].ptr,
+            k_sweep<WS, Acc><<<g, kBlock, 0, st>>>((const Part *)cx.parts[0].ptr, tiles0,
                                                   &ctr->n_tiles[0], cs[0][0], cs[0][1], cs[0][2], ws[0], bcnt, bw,
                                                   bmin);
         });
-        search(0, 0, parts_ub, tiles_ub, true);
+        search(0, 0, parts_ub, tiles_ub, n_inst, true);
     }
 
     int cur = 0;
-    for (size_t level = 0; level < iter_count; level++) {
+    for (size_t level = 0; level < L; level++) {
         cx.stats.levels++;
         const int nxt = cur ^ 1;
         Part *parts = (Part *)cx.parts[cur].ptr;
-        Tile *tiles = (Tile *)cx.tiles[cur].ptr;
+        Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[cur].ptr;
         uint32_t *np = &ctr->n_parts[cur];
         uint32_t *ntl = &ctr->n_tiles[cur];
 
-        if (level + 1 == iter_count) {
-            // Last level: leaf ids, then every point walks the tree in input order.
-            check(hipMemsetAsync(&ctr->min_id, 0xff, sizeof(uint64_t), st));
-            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_leaf_min<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &ctr->min_id);
-                k_last_nodes<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
-                    parts, np, node_base[level], &ctr->min_id, (Node *)cx.nodes.ptr, (uint64_t *)cx.leaf.ptr,
-                    heap_split);
-            });
-            // reads the input-order f32 coordinates, writes the 8-byte id
-            if (dense) {
-                const unsigned g = persistent_grid(cx, k_descend_dense<D>, (n + kBlock - 1) / kBlock);
-                timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
-                    k_descend_dense<D><<<g, kBlock, 0, st>>>(heap_split, (uint32_t)iter_count, &ctr->min_id,
-                                                             cs[0][0], cs[0][1], cs[0][2], n, d_out);
-                });
-            } else {
-                const unsigned g = persistent_grid(cx, k_descend<D>, (n + kBlock - 1) / kBlock);
-                timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
-                    k_descend<D><<<g, kBlock, 0, st>>>((const Node *)cx.nodes.ptr, n_nodes, (uint32_t)iter_count,
-                                                       node_base[level], (const uint64_t *)cx.leaf.ptr, parts_ub,
-                                                       cs[0][0], cs[0][1], cs[0][2], n, d_out, &ctr->error);
-                });
-            }
-            break;
-        }
-
-        // Children: next level's parts and tiles, and this level's split nodes.
+        // Children (next level's parts and tiles), split records, id offsets.
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
                                                                           (uint64_t *)cx.ntiles.ptr);
@@ -345,11 +377,12 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                 parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, (Node *)cx.nodes.ptr, node_base[level],
-                node_base[level + 1], heap_split);
+                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
+                node_base[std::min(level + 1, L)], heap_split, insts);
             k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                           &ctr->n_tiles[nxt]);
         });
+        if (level + 1 == L) break;   // every part left is at its last level
 
         // Partition of this level, fused with the children's first pass.
         const uint32_t epoch = ++cx.epoch;
@@ -366,9 +399,33 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
         });
 
         cur = nxt;
-        parts_ub = std::min<uint64_t>(2 * parts_ub, parts_cap);
+        parts_ub = std::max<uint64_t>(B.parts_ub[level + 1], 1);
         tiles_ub = std::min<uint64_t>(tiles_cap, parts_ub + (n + kTile - 1) / kTile);
-        search(cur, level + 1, parts_ub, tiles_ub, false);
+        search(cur, level + 1, parts_ub, tiles_ub, 0, false);
+    }
+
+    // Part ids in input order (:589-601, :698-702).
+    if (dense) {
+        const unsigned g = persistent_grid(cx, k_descend_dense<D>, B.tiles0);
+        // reads the input-order f32 coordinates, writes the 8-byte id
+        timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
+            k_descend_dense<D><<<g, kBlock, 0, st>>>(tiles0, (uint32_t)B.tiles0, insts,
+                                                     heap_split, cs[0][0], cs[0][1], cs[0][2], d_out);
+        });
+    } else {
+        // single instance, deep tree: node records and leaf ids
+        Part *parts = (Part *)cx.parts[cur].ptr;
+        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+            k_last_nodes<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                parts, &ctr->n_parts[cur], node_base[L - 1], &insts[0].min_id, nodes, (uint64_t *)cx.leaf.ptr,
+                nullptr);
+        });
+        const unsigned g = persistent_grid(cx, k_descend<D>, (n + kBlock - 1) / kBlock);
+        timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
+            k_descend<D><<<g, kBlock, 0, st>>>(nodes, n_nodes, (uint32_t)L, node_base[L - 1],
+                                               (const uint64_t *)cx.leaf.ptr, parts_ub, cs[0][0], cs[0][1],
+                                               cs[0][2], n, d_out, &ctr->error);
+        });
     }
 
     // Standalone-pass bytes: swept points times (axis coordinate + stored weight).
@@ -383,46 +440,50 @@ static void run_levels(Context &cx, hipStream_t st, uint64_t n, size_t iter_coun
 }
 
 template <int D, typename WIN>
-static void prologue(Context &cx, hipStream_t st, const double *pts, const WIN *win, uint64_t n, Prologue *h) {
+static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const double *pts, const WIN *win,
+                     Prologue *h) {
+    const uint32_t n_inst = (uint32_t)B.inst.size();
+    check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
+    // bounding boxes start empty: min key = max, max key = 0
     Prologue init;
     memset(&init, 0, sizeof init);
-    for (int k = 0; k < kMaxDim; k++) {
-        init.bmin[k] = ~0ull;
-        init.bmax[k] = 0;
-    }
     check(hipMemcpyAsync(cx.prologue.ptr, &init, sizeof init, hipMemcpyHostToDevice, st));
-    const unsigned g = persistent_grid(cx, k_prologue<D, WIN>, (n + kProPoints - 1) / kProPoints);
-    // reads AoS f64 points and the input weights; writes SoA f32 and int32 weights
-    const double bytes = (double)n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0);
-    timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
-        k_prologue<D, WIN><<<g, kBlock, 0, st>>>(pts, win, n, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
-                                                 (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
-                                                 (int32_t *)cx.w[0].ptr, (Prologue *)cx.prologue.ptr);
+    timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+        k_reset_instances<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>((Instance *)cx.instances.ptr, n_inst);
+        k_init_roots<<<n_inst, 64, 0, st>>>((const Instance *)cx.instances.ptr, n_inst, (Part *)cx.parts[0].ptr,
+                                            (Tile *)cx.tiles0.ptr);
     });
+    const uint32_t nt = (uint32_t)B.tiles0;
+    if (nt > 0) {
+        const unsigned g = persistent_grid(cx, k_prologue<D, WIN>, nt);
+        // reads AoS f64 points and the input weights; writes SoA f32 and int32 weights
+        const double bytes = (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0);
+        timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
+            k_prologue<D, WIN><<<g, kBlock, 0, st>>>(
+                (const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
+                (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr), (int32_t *)cx.w[0].ptr,
+                (Instance *)cx.instances.ptr, (Prologue *)cx.prologue.ptr);
+        });
+    }
     check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
 }
 
 template <int D, typename WIN>
-static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, const double *pts, const WIN *win,
-                      int wkind, size_t iter_count, double tol) {
+static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts,
+                      const WIN *win, int wkind) {
+    const uint64_t n = B.n;
     // Coordinate slots: 0 (prologue output, input order), 1 and 2 (levels).
     for (int b = 0; b < 3; b++)
-        for (int k = 0; k < D; k++) cx.cx[b][k].ensure(n * sizeof(float));
-    cx.w[0].ensure(n * sizeof(int32_t));
+        for (int k = 0; k < D; k++) cx.cx[b][k].ensure(std::max<uint64_t>(n, 1) * sizeof(float));
+    cx.w[0].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
     cx.prologue.ensure(sizeof(Prologue));
     cx.counters.ensure(sizeof(Counters));
+    cx.instances.ensure(std::max<size_t>(B.inst.size(), 1) * sizeof(Instance));
+    cx.parts[0].ensure(std::max<uint64_t>(B.parts_cap, B.inst.size()) * sizeof(Part));
+    cx.tiles0.ensure(std::max<uint64_t>(B.tiles0, 1) * sizeof(Tile));
     Prologue h;
-    prologue<D, WIN>(cx, st, pts, win, n, &h);
-
-    if (iter_count == 0) {   // every point in part 0 (:589-601, offset 0)
-        timed(cx, st, COUPE_HIP_K_IDS, 8.0 * (double)n, [&] {
-            k_fill_u64<<<std::min<unsigned>(grid_for(n, kBlock), 4096), kBlock, 0, st>>>(d_out, n, 0);
-        });
-        check(hipStreamSynchronize(st));
-        cx.stats.levels = 0;
-        return;
-    }
+    prologue<D, WIN>(cx, st, B, pts, win, &h);
 
     // Weight mode.  Integer kinds and integer-valued doubles whose partial
     // sums stay exact in f64 (|sum| <= 2^53) use exact int64 sums: the same
@@ -435,8 +496,8 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, 
     if (integral && fits32) {
         cx.stats.weight_mode = 0;
         cx.stats.weight_bytes = 4;
-        cx.w[1].ensure(n * sizeof(int32_t));
-        run_levels<D, int32_t, uint64_t>(cx, st, n, iter_count, tol, wrap32, d_out, (int32_t *)cx.w[0].ptr);
+        cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
+        run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr);
     } else if (integral) {
         cx.stats.weight_mode = 0;
         cx.stats.weight_bytes = 8;
@@ -445,7 +506,7 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, 
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
         });
-        run_levels<D, int64_t, uint64_t>(cx, st, n, iter_count, tol, wrap32, d_out, (int64_t *)cx.w[0].ptr);
+        run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr);
     } else {
         cx.stats.weight_mode = 1;
         cx.stats.weight_bytes = 8;
@@ -454,47 +515,43 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, 
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_store_weights<WIN, double><<<grid, kBlock, 0, st>>>(win, (double *)cx.w[0].ptr, n);
         });
-        run_levels<D, double, double>(cx, st, n, iter_count, tol, 0, d_out, (double *)cx.w[0].ptr);
+        run_levels<D, double, double>(cx, st, B, 0, d_out, (double *)cx.w[0].ptr);
     }
     check(hipStreamSynchronize(st));
 }
 
 template <int D>
-static void run_dim(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, const double *pts, int wkind,
-                    const void *w, size_t iter_count, double tol) {
+static void run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts, int wkind,
+                    const void *w) {
     switch (wkind) {
-        case COUPE_INT:
-            run_typed<D, int32_t>(cx, st, d_out, n, pts, (const int32_t *)w, wkind, iter_count, tol);
-            break;
-        case COUPE_INT64:
-            run_typed<D, int64_t>(cx, st, d_out, n, pts, (const int64_t *)w, wkind, iter_count, tol);
-            break;
-        default:
-            run_typed<D, double>(cx, st, d_out, n, pts, (const double *)w, wkind, iter_count, tol);
-            break;
+        case COUPE_INT: run_typed<D, int32_t>(cx, st, d_out, B, pts, (const int32_t *)w, wkind); break;
+        case COUPE_INT64: run_typed<D, int64_t>(cx, st, d_out, B, pts, (const int64_t *)w, wkind); break;
+        default: run_typed<D, double>(cx, st, d_out, B, pts, (const double *)w, wkind); break;
     }
 }
 
-enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
-                          const double *d_points, int wkind, const void *d_weights, size_t iter_count,
-                          double tol) {
+enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
+                         int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
+                         const double *tols, size_t n_inst) {
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
     memset(&cx.stats, 0, sizeof cx.stats);
-    cx.stats.points = n;
     cx.prof.reset();
     cx.swept_bytes = 0.0;
-    if (n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
-    if (n > 0xffffffffull) return COUPE_ERR_ALLOC;   // original indices are 32-bit
     try {
+        BatchPlan B = plan_batch(counts, levels, tols, n_inst);
+        cx.stats.points = B.n;
+        if (B.n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
+        if (B.n > 0xffffffffull) return COUPE_ERR_ALLOC;
+        if (B.inst.size() > 1 && !B.dense) return COUPE_ERR_BAD_TYPE;
         check(hipSetDevice(cx.device));
         if (!cx.host_scratch) check(hipHostMalloc(&cx.host_scratch, 4096, hipHostMallocDefault));
         if (cx.cu_count <= 0) {
             check(hipDeviceGetAttribute(&cx.cu_count, hipDeviceAttributeMultiprocessorCount, cx.device));
             if (cx.cu_count <= 0) cx.cu_count = 1;
         }
-        if (dim == 2) run_dim<2>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
-        else run_dim<3>(cx, st, d_out, n, d_points, wkind, d_weights, iter_count, tol);
+        if (dim == 2) run_dim<2>(cx, st, d_out, B, d_points, wkind, d_weights);
+        else run_dim<3>(cx, st, d_out, B, d_points, wkind, d_weights);
         cx.prof.collect();
         cx.prof.out[COUPE_HIP_K_HIST].algo_bytes = cx.swept_bytes;
     } catch (const HipFailure &f) {
@@ -504,17 +561,26 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
     } catch (const std::bad_alloc &) {
         cx.bins_clean = false;
         return COUPE_ERR_ALLOC;
+    } catch (const std::length_error &) {
+        cx.bins_clean = false;
+        return COUPE_ERR_ALLOC;
     }
     cx.stats.device_bytes = cx.device_bytes();
     return COUPE_ERR_OK;
+}
+
+enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
+                          const double *d_points, int wkind, const void *d_weights, size_t iter_count,
+                          double tol) {
+    return rcb_batch(cx, st, d_out, dim, d_points, wkind, d_weights, &n, &iter_count, &tol, 1);
 }
 
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
-                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split, &prologue,
-                           &counters, &in_points, &in_weights, &out_ids};
+                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
+                           &instances, &tiles0, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -522,8 +588,8 @@ uint64_t Context::device_bytes() const {
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
-                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split, &prologue,
-                     &counters, &in_points, &in_weights, &out_ids};
+                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
+                     &instances, &tiles0, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

@@ -133,14 +133,70 @@ __device__ inline int bin4(float c0, float c1, float c2, float x) {
 // select exact integer accumulation.  Points are read through LDS so every
 // global load is contiguous.
 // ---------------------------------------------------------------------------
-constexpr int kProPoints = kBlock * 4;   // points per workgroup iteration
+constexpr int kProPoints = kBlock * 4;   // points per LDS transpose round
 
+// Per-instance bounding box and weight sums, reduced over the workgroup and
+// added to the instance with one atomic per field.
+template <int D>
+__device__ __forceinline__ void flush_instance(Instance *I, double (&mn)[D], double (&mx)[D], uint64_t &su,
+                                               double &sf, double *s_d, uint64_t *s_u) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            const double a = __shfl_xor(mn[k], o, 64);
+            const double b = __shfl_xor(mx[k], o, 64);
+            mn[k] = a < mn[k] ? a : mn[k];
+            mx[k] = mx[k] < b ? b : mx[k];
+        }
+        su += __shfl_xor(su, o, 64);
+        sf += __shfl_xor(sf, o, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < D; k++) { s_d[wid * 8 + k] = mn[k]; s_d[wid * 8 + 3 + k] = mx[k]; }
+        s_d[wid * 8 + 6] = sf;
+        s_u[wid] = su;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int v = 1; v < kBlock / 64; v++) {
+#pragma unroll
+            for (int k = 0; k < D; k++) {
+                const double a = s_d[v * 8 + k], b = s_d[v * 8 + 3 + k];
+                s_d[k] = a < s_d[k] ? a : s_d[k];
+                s_d[3 + k] = s_d[3 + k] < b ? b : s_d[3 + k];
+            }
+            s_d[6] += s_d[v * 8 + 6];
+            s_u[0] += s_u[v];
+        }
+#pragma unroll
+        for (int k = 0; k < D; k++) {
+            atomicMin((unsigned long long *)&I->bmin[k], (unsigned long long)f64_key(s_d[k]));
+            atomicMax((unsigned long long *)&I->bmax[k], (unsigned long long)f64_key(s_d[3 + k]));
+        }
+        atomicAdd((unsigned long long *)&I->sum_u64, (unsigned long long)s_u[0]);
+        atomicAdd(&I->sum_f64, s_d[6]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < D; k++) { mn[k] = 1.7976931348623157e308; mx[k] = -1.7976931348623157e308; }
+    su = 0;
+    sf = 0.0;
+}
+
+// Workgroups stride over the level-0 tiles (each inside one instance) and
+// read the AoS points of a tile through LDS so every global load is
+// contiguous.
 template <int D, typename WIN>
 __global__ void __launch_bounds__(kBlock)
-k_prologue(const double *__restrict__ pts, const WIN *__restrict__ win, uint64_t n,
-           float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
-           int32_t *__restrict__ w32, Prologue *out) {
+k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__restrict__ pts,
+           const WIN *__restrict__ win, float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
+           int32_t *__restrict__ w32, Instance *__restrict__ insts, Prologue *out) {
     __shared__ double sp[kProPoints * D];
+    __shared__ double s_d[kBlock / 64 * 8];
+    __shared__ uint64_t s_u[kBlock / 64];
     const int tid = threadIdx.x;
     double mn[D], mx[D];
 #pragma unroll
@@ -148,83 +204,76 @@ k_prologue(const double *__restrict__ pts, const WIN *__restrict__ win, uint64_t
     uint64_t su = 0;
     double sf = 0.0, sa = 0.0, ma = 0.0;
     uint32_t nonint = 0, not32 = 0;
-    for (uint64_t base = (uint64_t)blockIdx.x * kProPoints; base < n; base += (uint64_t)gridDim.x * kProPoints) {
-        const uint32_t cnt = (uint32_t)((n - base) < (uint64_t)kProPoints ? (n - base) : kProPoints);
-        const double *src = pts + base * D;
-        for (uint32_t j = tid; j < cnt * D; j += kBlock) sp[j] = src[j];
-        __syncthreads();
+    uint32_t cur = 0xffffffffu;
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, su, sf, s_d, s_u);
+            cur = T.part;
+        }
+        for (uint32_t r0 = 0; r0 < T.count; r0 += kProPoints) {
+            const uint64_t base = T.begin + r0;
+            const uint32_t cnt = T.count - r0 < (uint32_t)kProPoints ? T.count - r0 : (uint32_t)kProPoints;
+            const double *src = pts + base * D;
+            for (uint32_t j = tid; j < cnt * D; j += kBlock) sp[j] = src[j];
+            __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t p = k * kBlock + tid;
-            if (p < cnt) {
-                const uint64_t i = base + p;
+            for (int k = 0; k < 4; k++) {
+                const uint32_t p = k * kBlock + tid;
+                if (p < cnt) {
+                    const uint64_t i = base + p;
 #pragma unroll
-                for (int a = 0; a < D; a++) {
-                    double v = sp[p * D + a];
-                    if (v < mn[a]) mn[a] = v;
-                    if (mx[a] < v) mx[a] = v;
-                    float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
-                    cs[i] = (float)v;
-                }
-                WIN w = win[i];
-                if constexpr (std::is_same<WIN, double>::value) {
-                    double aw = fabs(w);
-                    bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
-                    nonint |= !integral;
-                    sf += w;
-                    sa += aw;
-                    ma = aw > ma ? aw : ma;
-                    if (integral) {
-                        su += (uint64_t)(int64_t)w;
-                        bool fits = w >= -2147483648.0 && w <= 2147483647.0;
-                        not32 |= !fits;
-                        w32[i] = fits ? (int32_t)w : 0;
-                    } else {
-                        not32 = 1;
+                    for (int a = 0; a < D; a++) {
+                        const double v = sp[p * D + a];
+                        if (v < mn[a]) mn[a] = v;
+                        if (mx[a] < v) mx[a] = v;
+                        float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
+                        cs[i] = (float)v;
                     }
-                } else {
-                    int64_t wi = (int64_t)w;
-                    su += (uint64_t)wi;
-                    double aw = fabs((double)wi);
-                    sf += (double)wi;
-                    sa += aw;
-                    ma = aw > ma ? aw : ma;
-                    bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
-                    not32 |= !fits;
-                    w32[i] = (int32_t)wi;
+                    const WIN w = win[i];
+                    if constexpr (std::is_same<WIN, double>::value) {
+                        const double aw = fabs(w);
+                        const bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
+                        nonint |= !integral;
+                        sf += w;
+                        sa += aw;
+                        ma = aw > ma ? aw : ma;
+                        if (integral) {
+                            su += (uint64_t)(int64_t)w;
+                            const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
+                            not32 |= !fits;
+                            w32[i] = fits ? (int32_t)w : 0;
+                        } else {
+                            not32 = 1;
+                        }
+                    } else {
+                        const int64_t wi = (int64_t)w;
+                        su += (uint64_t)wi;
+                        const double aw = fabs((double)wi);
+                        sf += (double)wi;
+                        sa += aw;
+                        ma = aw > ma ? aw : ma;
+                        const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
+                        not32 |= !fits;
+                        w32[i] = (int32_t)wi;
+                    }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
+    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, su, sf, s_d, s_u);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            double a = __shfl_xor(mn[k], o, 64);
-            double b = __shfl_xor(mx[k], o, 64);
-            mn[k] = a < mn[k] ? a : mn[k];
-            mx[k] = mx[k] < b ? b : mx[k];
-        }
-        su += __shfl_xor(su, o, 64);
-        sf += __shfl_xor(sf, o, 64);
         sa += __shfl_xor(sa, o, 64);
-        double m2 = __shfl_xor(ma, o, 64);
+        const double m2 = __shfl_xor(ma, o, 64);
         ma = m2 > ma ? m2 : ma;
         nonint |= __shfl_xor(nonint, o, 64);
         not32 |= __shfl_xor(not32, o, 64);
     }
     if (lane_id() == 0) {
-#pragma unroll
-        for (int k = 0; k < D; k++) {
-            atomicMin((unsigned long long *)&out->bmin[k], (unsigned long long)f64_key(mn[k]));
-            atomicMax((unsigned long long *)&out->bmax[k], (unsigned long long)f64_key(mx[k]));
-        }
-        atomicAdd((unsigned long long *)&out->sum_u64, (unsigned long long)su);
-        atomicAdd(&out->sum_f64, sf);
         atomicAdd(&out->sum_abs, sa);
-        atomicMax((unsigned long long *)&out->max_abs_bits,
-                  (unsigned long long)__builtin_bit_cast(uint64_t, ma));
+        atomicMax((unsigned long long *)&out->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, ma));
         if (nonint) atomicOr(&out->non_integral, 1u);
         if (not32) atomicOr(&out->not_i32, 1u);
     }
@@ -411,7 +460,7 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
 template <typename Acc>
 __global__ void __launch_bounds__(64)
 k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
-         Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, double tol, int wrap32,
+         Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
          uint32_t *n_searching, unsigned long long *points_swept) {
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
@@ -483,6 +532,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     using Ops = AccOps<Acc>;
     const Acc S = Ops::from_bits(P->sum);
     const uint64_t n = P->count;
+    const double tol = P->tol;
     float lo = P->blo, hi = P->bhi;
     uint64_t prev = P->prev;
     int step = nb >> 1;
@@ -615,7 +665,8 @@ k_children_count(const Part *__restrict__ parts, const uint32_t *n_parts, uint64
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= *n_parts) return;
     const Part &P = parts[s];
-    const uint64_t nl = P.cnt_left, nr = P.count - P.cnt_left;
+    const bool inner = P.status != kDone && P.depth + 1 < P.levels;
+    const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;
     nchild[s] = (nl > 0) + (nr > 0);
     ntiles[s] = tiles_for(nl) + tiles_for(nr);
 }
@@ -625,11 +676,15 @@ __global__ void __launch_bounds__(64)
 k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
-                 uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split) {
+                 uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split,
+                 Instance *__restrict__ insts) {
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     const Part P = parts[s];
-    const uint64_t nl = P.cnt_left, nr = P.count - P.cnt_left;
+    if (P.status == kDone) return;
+    // A part at its instance's last level has leaves, not parts (:589-601).
+    const bool inner = P.depth + 1 < P.levels;
+    const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;
     const uint32_t a = P.axis;
     const uint32_t na = (a + 1) % dim;
     const Acc S = AccOps<Acc>::from_bits(P.sum);
@@ -672,6 +727,11 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             C.tile_count = nt;
             C.child[0] = -1;
             C.child[1] = -1;
+            C.inst = P.inst;
+            C.depth = P.depth + 1;
+            C.levels = P.levels;
+            C.pad = 0;
+            C.tol = P.tol;
             next[co] = C;
         }
         for (uint32_t k = threadIdx.x; k < nt; k += 64) {
@@ -688,14 +748,93 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     if (threadIdx.x == 0) {
         parts[s].child[0] = id0;
         parts[s].child[1] = id1;
-        Node nd;                                  // this split, for the final descent
-        nd.split = P.split;
-        nd.meta = P.axis | (P.status == kAllLeft ? kNodeAllLeft : 0u);
-        nd.child[0] = id0 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id0);
-        nd.child[1] = id1 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id1);
-        nodes[node_base + s] = nd;
-        if (heap_split) heap_split[P.heap_id] = dense_split(P);
+        if (nodes) {                              // this split, for the tree descent
+            Node nd;
+            nd.split = P.split;
+            nd.meta = P.axis | (P.status == kAllLeft ? kNodeAllLeft : 0u);
+            nd.child[0] = id0 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id0);
+            nd.child[1] = id1 < 0 ? -1 : (int32_t)(next_node_base + (uint64_t)id1);
+            nodes[node_base + s] = nd;
+        }
+        if (heap_split) heap_split[insts[P.inst].heap_base + P.heap_id] = dense_split(P);
+        if (!inner) {                             // smallest leaf of the instance (:698-699)
+            uint64_t v = P.cnt_left > 0 ? 2 * P.heap_id + 1 : 2 * P.heap_id + 2;
+            atomicMin((unsigned long long *)&insts[P.inst].min_id, (unsigned long long)v);
+        }
     }
+}
+
+// Empty per-instance reductions before the prologue.
+__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n_inst) return;
+    Instance &I = insts[q];
+    for (int k = 0; k < kMaxDim; k++) {
+        I.bmin[k] = ~0ull;
+        I.bmax[k] = 0;
+    }
+    I.sum_u64 = 0;
+    I.sum_f64 = 0.0;
+    I.min_id = ~0ull;
+}
+
+// Root parts, one per instance (part index = instance index), and their
+// tiles.  The box and the sum are set once the prologue has reduced them.
+__global__ void __launch_bounds__(64)
+k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts,
+             Tile *__restrict__ tiles) {
+    const uint32_t q = blockIdx.x;
+    if (q >= n_inst) return;
+    const Instance I = insts[q];
+    const uint32_t nt = tiles_for(I.count);
+    if (threadIdx.x == 0) {
+        Part R;
+        memset(&R, 0, sizeof R);
+        R.begin = I.begin;
+        R.count = I.count;
+        R.heap_id = 0;
+        R.prev = ~0ull;
+        R.status = (I.count == 0 || I.levels == 0) ? kDone : kSearching;
+        R.kstar = -1;
+        R.tile_begin = I.tile_begin;
+        R.tile_count = nt;
+        R.child[0] = -1;
+        R.child[1] = -1;
+        R.inst = q;
+        R.depth = 0;
+        R.levels = I.levels;
+        R.tol = I.tol;
+        parts[q] = R;
+    }
+    for (uint32_t k = threadIdx.x; k < nt; k += 64) {
+        Tile U;
+        U.begin = I.begin + (uint64_t)k * kTile;
+        const uint64_t rem = I.count - (uint64_t)k * kTile;
+        U.count = (uint32_t)(rem < (uint64_t)kTile ? rem : (uint64_t)kTile);
+        U.part = q;
+        tiles[I.tile_begin + k] = U;
+    }
+}
+
+// Box (`as f32`, :604-605) and sum (:684) of every root part from the
+// prologue's per-instance reductions; the id offset starts at "none".
+__global__ void __launch_bounds__(kBlock)
+k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts, int dim,
+                 int real_mode) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= n_inst) return;
+    Instance &I = insts[q];
+    Part &R = parts[q];
+    R.sum = real_mode ? __builtin_bit_cast(uint64_t, I.sum_f64) : I.sum_u64;
+    for (int k = 0; k < kMaxDim; k++) {
+        R.lo[k] = k < dim ? (float)key_f64(I.bmin[k]) : 0.0f;
+        R.hi[k] = k < dim ? (float)key_f64(I.bmax[k]) : 0.0f;
+    }
+    R.axis = 0;
+    R.blo = R.lo[0];
+    R.bhi = R.hi[0];
+    // iter_count 0: every point in part 0 (offset 0)
+    I.min_id = I.levels == 0 ? 0ull : ~0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -828,6 +967,12 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
             const float blo = sel3(a2, P.lo[0], P.lo[1], P.lo[2]);
             const float bhi = sel3(a2, P.hi[0], P.hi[1], P.hi[2]);
             t2 = (blo + bhi) / 2.0f;                     // their first candidate (:472)
+        }
+        // parts at their instance's last level are not moved (their leaves
+        // are read back by the descent)
+        if (P.status == kDone || P.depth + 1 >= P.levels) {
+            __syncthreads();
+            continue;
         }
         const int ax = (int)P.axis;
         const bool all_left = P.status == kAllLeft;
@@ -1053,34 +1198,43 @@ constexpr int kDenseCached = 13;   // 8191 splits, 32 KB of LDS
 
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_descend_dense(const float *__restrict__ heap_split, uint32_t levels, const uint64_t *min_id,
-                const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
-                uint64_t n, uint64_t *__restrict__ out) {
+k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance *__restrict__ insts,
+                const float *__restrict__ heap_split, const float *__restrict__ c0, const float *__restrict__ c1,
+                const float *__restrict__ c2, uint64_t *__restrict__ out) {
     __shared__ float cache[(1u << kDenseCached) - 1];
-    const uint32_t cached_levels = levels < (uint32_t)kDenseCached ? levels : (uint32_t)kDenseCached;
-    const uint32_t cached = (1u << cached_levels) - 1;
-    for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = heap_split[j];
-    __syncthreads();
-    const uint64_t off = *min_id;
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const float x[3] = {c0[i], c1[i], D == 3 ? c2[i] : 0.f};
-        uint32_t h = 0;
-        int a = 0;
-#pragma unroll 4
-        for (uint32_t l = 0; l < cached_levels; l++) {
-            const float t = cache[h];
-            const float xa = sel3(a, x[0], x[1], x[2]);
-            h = 2 * h + (xa >= t ? 2u : 1u);
-            a = a + 1 == D ? 0 : a + 1;
+    uint32_t cur = 0xffffffffu, levels = 0, cached_levels = 0;
+    uint64_t hb = 0, off = 0;
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {             // level-0 tiles: part index = instance index
+            __syncthreads();
+            cur = T.part;
+            const Instance &I = insts[cur];
+            levels = I.levels;
+            hb = I.heap_base;
+            off = I.min_id;
+            cached_levels = levels < (uint32_t)kDenseCached ? levels : (uint32_t)kDenseCached;
+            const uint32_t cached = (1u << cached_levels) - 1;
+            for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = heap_split[hb + j];
+            __syncthreads();
         }
-        for (uint32_t l = cached_levels; l < levels; l++) {
-            const float t = heap_split[h];
-            const float xa = sel3(a, x[0], x[1], x[2]);
-            h = 2 * h + (xa >= t ? 2u : 1u);
-            a = a + 1 == D ? 0 : a + 1;
+        const uint64_t end = T.begin + T.count;
+        for (uint64_t i = T.begin + threadIdx.x; i < end; i += kBlock) {
+            const float x0 = c0[i], x1 = c1[i], x2 = D == 3 ? c2[i] : 0.f;
+            uint32_t h = 0;
+            int a = 0;
+            for (uint32_t l = 0; l < cached_levels; l++) {
+                const float t = cache[h];
+                h = 2 * h + (sel3(a, x0, x1, x2) >= t ? 2u : 1u);
+                a = a + 1 == D ? 0 : a + 1;
+            }
+            for (uint32_t l = cached_levels; l < levels; l++) {
+                const float t = heap_split[hb + h];
+                h = 2 * h + (sel3(a, x0, x1, x2) >= t ? 2u : 1u);
+                a = a + 1 == D ? 0 : a + 1;
+            }
+            out[i] = (uint64_t)h - off;
         }
-        out[i] = (uint64_t)h - off;
     }
 }
 
@@ -1088,48 +1242,6 @@ k_descend_dense(const float *__restrict__ heap_split, uint32_t levels, const uin
 __global__ void __launch_bounds__(kBlock) k_fill_u64(uint64_t *out, uint64_t n, uint64_t v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = v;
-}
-
-// Root part: whole array, box from the prologue, sum in the accumulator.
-__global__ void k_init_root(Part *parts, uint32_t *n_parts, Tile *tiles, uint32_t *n_tiles,
-                            const Prologue *pro, uint64_t n, int dim, int real_mode) {
-    const uint32_t nt = (uint32_t)((n + kTile - 1) / kTile);
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        Part R;
-        R.begin = 0;
-        R.count = n;
-        R.heap_id = 0;
-        R.sum = real_mode ? __builtin_bit_cast(uint64_t, pro->sum_f64) : pro->sum_u64;
-        for (int k = 0; k < kMaxDim; k++) {
-            R.lo[k] = k < dim ? (float)key_f64(pro->bmin[k]) : 0.0f;   // `as f32`, :604-605
-            R.hi[k] = k < dim ? (float)key_f64(pro->bmax[k]) : 0.0f;
-        }
-        R.axis = 0;
-        R.blo = R.lo[0];
-        R.bhi = R.hi[0];
-        R.prev = ~0ull;
-        R.status = kSearching;
-        R.split = 0.0f;
-        R.kstar = -1;
-        R.wl = 0;
-        R.cnt_left = 0;
-        R.tile_begin = 0;
-        R.tile_count = nt;
-        R.child[0] = -1;
-        R.child[1] = -1;
-        parts[0] = R;
-        *n_parts = 1;
-        *n_tiles = nt;
-    }
-    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nt;
-         k += (uint64_t)gridDim.x * blockDim.x) {
-        Tile U;
-        U.begin = k * kTile;
-        uint64_t rem = n - k * kTile;
-        U.count = (uint32_t)(rem < (uint64_t)kTile ? rem : (uint64_t)kTile);
-        U.part = 0;
-        tiles[k] = U;
-    }
 }
 
 // Copies the 64-bit totals of the scans into the 32-bit table sizes.

@@ -45,6 +45,29 @@ enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream,
 		uintptr_t iter_count, double tolerance);
 
 /*
+ * Batched RCB: many independent instances in one call.
+ *
+ * Instance q holds instances[q].n points; the instances are packed one after
+ * the other in d_points (AoS, `dimension` doubles per point), d_weights and
+ * d_partition, in the order given.  Instance q is partitioned exactly as
+ * coupe_hip_rcb_device would partition it alone, with its own iter_count and
+ * tolerance: its ids start at 0 and are written at its own offsets.
+ * All instances share the dimension and the weight type (split a mixed batch
+ * by dimension, one call each).  Every instance of a batch of more than one
+ * needs iter_count <= 24 (else COUPE_ERR_BAD_TYPE).
+ * No counterpart in coupe: it replaces a loop of Rcb::partition calls
+ * (coupe/src/algorithms/recursive_bisection.rs:805-811), one per instance.
+ */
+struct coupe_hip_instance {
+	uintptr_t n;            /* points of this instance */
+	uintptr_t iter_count;   /* Rcb::iter_count */
+	double tolerance;       /* Rcb::tolerance */
+};
+enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,
+		uintptr_t dimension, const double *d_points, enum coupe_type weight_type,
+		const void *d_weights, const struct coupe_hip_instance *instances, uintptr_t n_instances);
+
+/*
  * Counters of the last call on a context (for benchmarks and tests).
  */
 struct coupe_hip_stats {

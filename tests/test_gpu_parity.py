@@ -197,3 +197,51 @@ def test_device_entry_matches_host_entry():
         coupe_amd.Rcb(iter_count=it, tolerance=tol).partition(out, (dp, dw))
         torch.cuda.synchronize()
         assert np.array_equal(out.cpu().numpy().astype(np.uint64), host)
+
+
+# ---------------------------------------------------------------------------
+# Batched RCB (coupe_hip_rcb_batch): every instance equals its lone run.
+# ---------------------------------------------------------------------------
+def _batch_case(seed, dim, count):
+    rng = np.random.default_rng(seed)
+    specs, pts, ws = [], [], []
+    for q in range(count):
+        n = int(rng.choice([0, 1, 2, 17, 300, 2048, 2049, 5000, 40000]))
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            p = rng.random((n, dim))
+        elif kind == 1:
+            p = rng.integers(0, 7, (n, dim)).astype(np.float64)
+        else:
+            p = rng.normal(0, 1, (n, dim)) * 10.0 ** rng.integers(-2, 3)
+        pts.append(p)
+        ws.append(rng.integers(1, 50, n).astype(np.float64))
+        specs.append((n, int(rng.integers(0, 11)), float(rng.choice([0.0, 0.02, 0.05]))))
+    return specs, np.concatenate(pts).reshape(-1, dim), np.concatenate(ws)
+
+
+@pytest.mark.parametrize("seed,dim", [(0, 2), (1, 3), (2, 2), (3, 3)])
+def test_batch_matches_each_instance(oracle_mod, seed, dim):
+    torch = pytest.importorskip("torch")
+    specs, pts, w = _batch_case(seed, dim, 40)
+    say(f"batch seed {seed} dim {dim}: {len(specs)} instances, {pts.shape[0]} points")
+    out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
+    coupe_amd.partition_batch(out, torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(), specs)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().astype(np.uint64)
+    off = 0
+    for q, (n, iters, tol) in enumerate(specs):
+        exp, _ = oracle_rcb(oracle_mod, pts[off:off + n], w[off:off + n], iters, tol)
+        assert np.array_equal(got[off:off + n], exp), f"instance {q}: n={n} L={iters} tol={tol}"
+        off += n
+
+
+def test_batch_of_one_equals_plain_call():
+    torch = pytest.importorskip("torch")
+    pts, w, it, tol = workloads.standard("gauss3d", scale=6)
+    plain = gpu_rcb(pts, w, it, tol)
+    out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
+    coupe_amd.partition_batch(out, torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(),
+                              [(pts.shape[0], it, tol)])
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().astype(np.uint64), plain)
