@@ -28,7 +28,7 @@ from . import _native
 from ._native import strerror
 
 __all__ = ["Rcb", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
-           "partition_batch"]
+           "partition_batch", "Server"]
 
 
 class Error(Exception):
@@ -240,3 +240,67 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
     if err != _native.OK:
         raise CoupeError(err)
     return None
+
+
+class Server:
+    """Request server with continuous batching (coupe_hip_server_*, include/coupe_hip.h).
+
+    ``submit(part_ids, points, weights, iter_count, tolerance)`` takes numpy
+    arrays (host memory) or torch tensors on the GPU (a mesh kept resident in
+    HBM) and returns a ticket; ``wait(ticket)`` blocks until the ids are
+    written and returns the request's latency figures.  The arrays are kept
+    alive until the request is waited for.
+    """
+
+    def __init__(self, device: int = 0, slots: int = 4, memory_budget: int = 64 << 30,
+                 max_batch_points: int = 1 << 26, max_batch_requests: int = 256):
+        cfg = _native.ServerConfig(device, slots, memory_budget, max_batch_points, max_batch_requests)
+        self._s = _native.lib().coupe_hip_server_create(C.byref(cfg))
+        if not self._s:
+            raise RuntimeError(f"cannot start a server on device {device}")
+        self._keep = {}
+
+    def submit(self, part_ids, points, weights, iter_count: int, tolerance: float) -> int:
+        dev_in = _is_device_tensor(points)
+        dev_out = _is_device_tensor(part_ids)
+        if dev_in != _is_device_tensor(weights):
+            raise TypeError("points and weights must both be host arrays or both device tensors")
+        if not dev_in:
+            points = np.ascontiguousarray(points, dtype=np.float64)
+            weights = np.ascontiguousarray(weights)
+        n = points.shape[0]
+        if weights.shape[0] != n:
+            raise InputLenMismatch(n, weights.shape[0])
+        if part_ids.shape[0] != n:
+            raise InputLenMismatch(part_ids.shape[0], n)
+        ptr = (lambda a: a.data_ptr()) if dev_in else (lambda a: a.ctypes.data)
+        optr = part_ids.data_ptr() if dev_out else part_ids.ctypes.data
+        req = _native.Request(ptr(points), ptr(weights), _weight_type(weights.dtype), n,
+                              points.shape[1] if points.ndim == 2 else 0, int(iter_count), float(tolerance),
+                              optr, 1 if dev_in else 0, 1 if dev_out else 0)
+        t = C.c_uint64()
+        err = _native.lib().coupe_hip_server_submit(self._s, C.byref(req), C.byref(t))
+        if err != _native.OK:
+            raise CoupeError(err)
+        self._keep[t.value] = (part_ids, points, weights)
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        st = _native.RequestStats()
+        err = _native.lib().coupe_hip_server_wait(self._s, int(ticket), C.byref(st))
+        self._keep.pop(ticket, None)
+        if err != _native.OK:
+            raise CoupeError(err)
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self._s:
+            _native.lib().coupe_hip_server_destroy(self._s)
+            self._s = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -22,8 +22,26 @@ EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb",
     "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_batch",
+    "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
     "coupe_hip_last_stats", "coupe_hip_set_profiling", "coupe_hip_kernel_times",
 ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_uint32), ("memory_budget", C.c_uint64),
+                ("max_batch_points", C.c_uint64), ("max_batch_requests", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("weights", C.c_void_p), ("weight_type", C.c_int),
+                ("n", C.c_size_t), ("dimension", C.c_size_t), ("iter_count", C.c_size_t),
+                ("tolerance", C.c_double), ("partition", C.c_void_p), ("device_input", C.c_int),
+                ("device_output", C.c_int)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
+                ("batch_requests", C.c_uint32), ("batch_points", C.c_uint64)]
 
 
 class Instance(C.Structure):
@@ -94,6 +112,14 @@ def lib():
         L.coupe_hip_rcb_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                           C.c_void_p, C.POINTER(Instance), C.c_size_t]
         L.coupe_hip_rcb_batch.restype = C.c_int
+        L.coupe_hip_server_create.argtypes = [C.POINTER(ServerConfig)]
+        L.coupe_hip_server_create.restype = C.c_void_p
+        L.coupe_hip_server_destroy.argtypes = [C.c_void_p]
+        L.coupe_hip_server_destroy.restype = None
+        L.coupe_hip_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.coupe_hip_server_submit.restype = C.c_int
+        L.coupe_hip_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(RequestStats)]
+        L.coupe_hip_server_wait.restype = C.c_int
         L.coupe_hip_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.coupe_hip_last_stats.restype = C.c_int
         L.coupe_hip_set_profiling.argtypes = [C.c_void_p, C.c_int]

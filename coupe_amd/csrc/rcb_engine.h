@@ -12,6 +12,11 @@
 
 namespace coupe_hip {
 
+// A HIP failure inside the engine; entry points map it onto coupe's codes.
+struct HipFailure {
+    hipError_t err;
+};
+
 // A grow-only device allocation.
 struct DevBuf {
     void *ptr = nullptr;

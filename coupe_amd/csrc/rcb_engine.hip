@@ -19,10 +19,6 @@ namespace coupe_hip {
 // Errors: any HIP failure unwinds to the entry point, which maps it onto
 // coupe's codes (out of memory -> COUPE_ERR_ALLOC, anything else -> CRASH).
 // ---------------------------------------------------------------------------
-struct HipFailure {
-    hipError_t err;
-};
-
 static void check(hipError_t e) {
     if (e != hipSuccess) throw HipFailure{e};
 }

@@ -68,6 +68,57 @@ enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d
 		const void *d_weights, const struct coupe_hip_instance *instances, uintptr_t n_instances);
 
 /*
+ * Serving: a request server with continuous batching.
+ *
+ * `slots` workers each own a context and a stream.  A free worker takes the
+ * queued requests (same dimension and weight type, in arrival order) that
+ * fit its batch limits and the device-memory budget, packs them, runs them
+ * as one batch, and copies every request's ids to its output; requests that
+ * arrive while batches run are admitted as slots and memory free up, and
+ * batches of different workers overlap on the device.  A request's ids are
+ * exactly those of coupe_hip_rcb_device on the same input.
+ * Inputs may be host or device pointers (a mesh kept resident in HBM is
+ * submitted with device_input = 1); they must stay valid until the request
+ * is waited for.  No counterpart in coupe.
+ */
+typedef struct coupe_hip_server coupe_hip_server;
+struct coupe_hip_server_config {
+	int device;
+	uint32_t slots;               /* concurrent batches (streams), >= 1 */
+	uint64_t memory_budget;       /* device bytes the workers may hold together */
+	uint64_t max_batch_points;    /* points packed into one batch */
+	uint32_t max_batch_requests;  /* requests packed into one batch */
+};
+struct coupe_hip_request {
+	const double *points;         /* n * dimension doubles */
+	const void *weights;          /* n values of weight_type */
+	enum coupe_type weight_type;
+	uintptr_t n;
+	uintptr_t dimension;
+	uintptr_t iter_count;
+	double tolerance;
+	uint64_t *partition;          /* n ids */
+	int device_input;             /* points and weights are device pointers */
+	int device_output;            /* partition is a device pointer */
+};
+struct coupe_hip_request_stats {
+	double queue_ms;              /* submit -> batch start */
+	double run_ms;                /* batch start -> ids written */
+	double total_ms;              /* submit -> ids written */
+	uint32_t batch_requests;      /* requests in its batch */
+	uint64_t batch_points;        /* points in its batch */
+};
+coupe_hip_server *coupe_hip_server_create(const struct coupe_hip_server_config *config);
+/* Waits for every submitted request, then frees the server. */
+void coupe_hip_server_destroy(coupe_hip_server *server);
+/* COUPE_ERR_ALLOC when the request alone exceeds the budget or a batch limit. */
+enum coupe_err coupe_hip_server_submit(coupe_hip_server *server, const struct coupe_hip_request *request,
+		uint64_t *ticket);
+/* Blocks until the request is done; returns its status (each ticket once). */
+enum coupe_err coupe_hip_server_wait(coupe_hip_server *server, uint64_t ticket,
+		struct coupe_hip_request_stats *stats);
+
+/*
  * Counters of the last call on a context (for benchmarks and tests).
  */
 struct coupe_hip_stats {

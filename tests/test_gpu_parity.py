@@ -245,3 +245,52 @@ def test_batch_of_one_equals_plain_call():
                               [(pts.shape[0], it, tol)])
     torch.cuda.synchronize()
     assert np.array_equal(out.cpu().numpy().astype(np.uint64), plain)
+
+
+# ---------------------------------------------------------------------------
+# Serving (coupe_hip_server_*): each result equals the lone call, in the mix.
+# ---------------------------------------------------------------------------
+def test_server_results_equal_lone_calls():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21)
+    reqs = []
+    for q in range(60):
+        n = int(10 ** rng.uniform(2, 5.3))
+        dim = 3 if q % 4 else 2
+        pts = rng.random((n, dim))
+        w = rng.integers(1, 100, n).astype(np.float64)
+        reqs.append((pts, w, int(rng.integers(3, 11)), float(rng.choice([0.01, 0.05]))))
+    say(f"server: {len(reqs)} requests")
+    srv = coupe_amd.Server(slots=3, memory_budget=8 << 30, max_batch_requests=16)
+    try:
+        outs, tickets = [], []
+        for q, (pts, w, it, tol) in enumerate(reqs):
+            if q % 3 == 0:   # a mesh resident in HBM, ids to device memory
+                out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
+                t = srv.submit(out, torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(), it, tol)
+            else:            # host buffers
+                out = np.zeros(pts.shape[0], dtype=np.uint64)
+                t = srv.submit(out, pts, w, it, tol)
+            outs.append(out)
+            tickets.append(t)
+        stats = [srv.wait(t) for t in tickets]
+    finally:
+        srv.close()
+    assert max(s["batch_requests"] for s in stats) > 1   # requests were batched
+    for q, ((pts, w, it, tol), out) in enumerate(zip(reqs, outs)):
+        got = out.cpu().numpy().astype(np.uint64) if hasattr(out, "cpu") else out
+        assert np.array_equal(got, gpu_rcb(pts, w, it, tol)), f"request {q}"
+
+
+def test_server_rejects_oversized_and_bad_requests():
+    srv = coupe_amd.Server(slots=1, memory_budget=1 << 20)
+    try:
+        pts = np.zeros((100_000, 3))
+        with pytest.raises(coupe_amd.CoupeError) as e:
+            srv.submit(np.zeros(100_000, dtype=np.uint64), pts, np.ones(100_000), 4, 0.05)
+        assert e.value.code == _native.ALLOC
+        with pytest.raises(coupe_amd.CoupeError) as e:
+            srv.submit(np.zeros(10, dtype=np.uint64), np.zeros((10, 4)), np.ones(10), 4, 0.05)
+        assert e.value.code == _native.BAD_DIMENSION
+    finally:
+        srv.close()
