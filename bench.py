@@ -10,8 +10,22 @@ dominant kernel's achieved bandwidth against the 8 TB/s HBM peak (HIP events
 on the launch stream), and the CPU oracle (fork-join C restatement of coupe's
 Rcb) timed on the same input on this host, with an id-for-id parity check.
 
+Two more measurements ride in the same line (skip them with --plain-only):
+  "batched": 4096 independent instances (sizes log-uniform in [10^3, 10^6],
+             D in {2, 3}, integer weights in [1, 16], iter_count 4..8,
+             tolerance 0.02, seed 3) partitioned by coupe_hip_rcb_batch (one
+             call per dimension) versus one coupe_hip_rcb_device call per
+             instance on the same inputs, ids compared instance by instance.
+  "served":  a Poisson request stream (mean 200 req/s, sizes log-uniform in
+             [10^4, 10^7] 3D points, weights 1..100, iter_count 6..12, seed 4)
+             through the request server; 30% of requests re-partition a mesh
+             resident in HBM after every point moved by N(0, 0.01) and 5% of
+             its weights were resampled.  p50/p99 latency (submit -> ids in
+             the caller's buffer) and aggregate Mpts/s.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                       [--no-cpu] [--cpu-threads T]
+                       [--no-cpu] [--cpu-threads T] [--plain-only]
+                       [--serve-requests R]
 """
 import argparse
 import json
@@ -59,6 +73,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
+    ap.add_argument("--plain-only", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the partitioner runs on a "
@@ -232,7 +248,192 @@ def main():
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
+    if not args.plain_only:
+        del d_pts, d_w, d_out
+        torch.cuda.empty_cache()
+        out["batched"] = bench_batched(torch, coupe_amd, workloads)
+        torch.cuda.empty_cache()
+        out["served"] = bench_served(torch, coupe_amd, workloads, args.serve_requests)
     print(json.dumps(out), flush=True)
+
+
+def bench_batched(torch, coupe_amd, workloads, count=4096, seed=3):
+    """Batched RCB versus per-instance launches on the same inputs."""
+    t0 = time.time()
+    n, dim, lo, ext, iters = workloads.batch_header(count, seed=seed, lg_lo=3.0, lg_hi=6.0, l_lo=4, l_hi=8)
+    tol = 0.02
+    groups = {}
+    for d in (2, 3):
+        idx = [q for q in range(count) if dim[q] == d]
+        pts, ws = [], []
+        for q in idx:
+            p, w = workloads.box(int(n[q]), d, lo[q, :d], ext[q, :d], workloads.instance_seed(seed, q), 16)
+            pts.append(p)
+            ws.append(w)
+        groups[d] = (idx, np.concatenate(pts), np.concatenate(ws))
+    total = int(n.sum())
+    log(f"batched: {count} instances, {total} points generated in {time.time() - t0:.1f}s")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = coupe_amd.DeviceContext(0)
+    res = {}
+    dev_data = {}
+    for d, (idx, pts, ws) in groups.items():
+        dev_data[d] = (torch.from_numpy(pts).to(dev), torch.from_numpy(ws).to(dev),
+                       torch.zeros(pts.shape[0], dtype=torch.int64, device=dev),
+                       torch.zeros(pts.shape[0], dtype=torch.int64, device=dev))
+    torch.cuda.synchronize()
+
+    def run_batched():
+        for d, (idx, _, _) in groups.items():
+            dp, dw, out, _ = dev_data[d]
+            specs = [(int(n[q]), int(iters[q]), tol) for q in idx]
+            coupe_amd.partition_batch(out, dp, dw, specs, context=ctx, stream=stream.cuda_stream)
+
+    def run_plain():
+        for d, (idx, _, _) in groups.items():
+            dp, dw, _, out = dev_data[d]
+            off = 0
+            for q in idx:
+                k = int(n[q])
+                coupe_amd.Rcb(int(iters[q]), tol).partition(out[off:off + k], (dp[off:off + k], dw[off:off + k]),
+                                                            context=ctx, stream=stream.cuda_stream)
+                off += k
+
+    for fn, name in ((run_batched, "batched"), (run_plain, "plain")):
+        fn()   # warm-up (buffer growth)
+        torch.cuda.synchronize()
+        reps = 3 if name == "batched" else 1
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t1 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t1) * 1e3 / reps
+        res[name] = {"ms": round(max(e0.elapsed_time(e1) / reps, wall_ms), 3)}
+        res[name]["Mpts_s"] = round(total / (res[name]["ms"] * 1e-3) / 1e6, 1)
+        log(f"batched bench: {name} {res[name]['ms']:.1f} ms, {res[name]['Mpts_s']} Mpts/s")
+    same = all(torch.equal(dev_data[d][2], dev_data[d][3]) for d in groups)
+    ctx.close()
+    return {"workload": f"{count} instances, sizes log-uniform in [1e3, 1e6], D in {{2,3}}, weights 1..16, "
+                        f"iter_count 4..8, tolerance {tol}, seed {seed}",
+            "points": total, "value": res["batched"]["Mpts_s"], "unit": "Mpts/s",
+            "batched_ms": res["batched"]["ms"], "per_instance_ms": res["plain"]["ms"],
+            "per_instance_Mpts_s": res["plain"]["Mpts_s"],
+            "speedup_vs_per_instance": round(res["plain"]["ms"] / res["batched"]["ms"], 2),
+            "ids_equal_to_per_instance": bool(same)}
+
+
+def bench_served(torch, coupe_amd, workloads, n_requests, seed=4, rate=200.0, resident=8, pool=24):
+    """Poisson request stream through the request server."""
+    import threading
+
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    # Host meshes for new partition requests (pinned, reused across requests).
+    sizes = np.round(10 ** rng.uniform(4, 7, pool)).astype(np.int64)
+    host = []
+    for k, m in enumerate(sizes):
+        p, w = workloads.uniform(int(m), 3, seed * 1000 + k, 100)
+        hp = torch.from_numpy(p).pin_memory()
+        hw = torch.from_numpy(w).pin_memory()
+        host.append((hp, hw, torch.zeros(int(m), dtype=torch.int64).pin_memory()))
+    # Meshes resident in HBM for re-partition requests.
+    rsizes = np.round(10 ** rng.uniform(4, 7, resident)).astype(np.int64)
+    res = []
+    for k, m in enumerate(rsizes):
+        p, w = workloads.uniform(int(m), 3, seed * 2000 + k, 100)
+        res.append([torch.from_numpy(p).to(dev), torch.from_numpy(w).to(dev),
+                    torch.zeros(int(m), dtype=torch.int64, device=dev), None])
+    torch.cuda.synchronize()
+    log(f"served: pools ready in {time.time() - t0:.1f}s")
+    srv = coupe_amd.Server(device=0, slots=4, memory_budget=64 << 30, max_batch_points=1 << 26,
+                           max_batch_requests=64)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    lat, pts_done, tickets = [], [], []
+    lock = threading.Lock()
+    done_evt = threading.Event()
+
+    def waiter():
+        i = 0
+        while True:
+            with lock:
+                if i < len(tickets):
+                    t, npts = tickets[i]
+                else:
+                    t = None
+            if t is None:
+                if done_evt.is_set():
+                    with lock:
+                        if i >= len(tickets):
+                            return
+                    continue
+                time.sleep(0.0005)
+                continue
+            st = srv.wait(t)
+            lat.append(st)
+            pts_done.append(npts)
+            i += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    next_t = start
+    n_repart = 0
+    for r in range(n_requests):
+        next_t += rng.exponential(1.0 / rate)
+        iters = int(6 + np.floor(7 * rng.random()))
+        repart = rng.random() < 0.3
+        k = int(rng.integers(0, resident if repart else pool))
+        now = time.perf_counter()
+        if next_t > now:
+            time.sleep(next_t - now)
+        if repart:
+            mesh = res[k]
+            if mesh[3] is not None:        # the mesh moves only after its last request is done
+                while True:
+                    with lock:
+                        finished = len(lat) > mesh[3]
+                    if finished:
+                        break
+                    time.sleep(0.0002)
+            dp, dw = mesh[0], mesh[1]
+            dp.add_(torch.randn(dp.shape, generator=gen, device=dev, dtype=dp.dtype), alpha=0.01)
+            sel = torch.rand(dw.shape[0], generator=gen, device=dev) < 0.05
+            dw[sel] = torch.floor(torch.rand(int(sel.sum()), generator=gen, device=dev,
+                                             dtype=dw.dtype) * 100) + 1
+            torch.cuda.synchronize()
+            with lock:
+                t = srv.submit(mesh[2], dp, dw, iters, 0.01)
+                mesh[3] = len(tickets)
+                tickets.append((t, dp.shape[0]))
+            n_repart += 1
+        else:
+            hp, hw, ho = host[k]
+            with lock:
+                t = srv.submit(ho.numpy(), hp.numpy(), hw.numpy(), iters, 0.01)
+                tickets.append((t, hp.shape[0]))
+    done_evt.set()
+    th.join()
+    wall = time.perf_counter() - start
+    srv.close()
+    tot = np.array([s["total_ms"] for s in lat])
+    run = np.array([s["run_ms"] for s in lat])
+    bsz = np.array([s["batch_requests"] for s in lat])
+    return {"workload": f"Poisson {rate:.0f} req/s, sizes log-uniform in [1e4, 1e7] 3D points, weights 1..100, "
+                        f"iter_count 6..12, tolerance 0.01, 30% re-partitions of {resident} HBM-resident meshes "
+                        f"(N(0,0.01) drift, 5% weights resampled), seed {seed}",
+            "requests": n_requests, "repartitions": n_repart,
+            "p50_ms": round(float(np.percentile(tot, 50)), 3), "p99_ms": round(float(np.percentile(tot, 99)), 3),
+            "p50_run_ms": round(float(np.percentile(run, 50)), 3), "p99_run_ms": round(float(np.percentile(run, 99)), 3),
+            "mean_batch_requests": round(float(bsz.mean()), 2),
+            "Mpts_s": round(float(np.sum(pts_done)) / wall / 1e6, 1),
+            "offered_Mpts_s": round(float(np.sum([p for _, p in tickets])) / wall / 1e6, 1),
+            "wall_s": round(wall, 2)}
 
 
 if __name__ == "__main__":
