@@ -5,6 +5,7 @@
 // current HIP device, and the ids copied back into the caller's array.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -89,6 +90,7 @@ Context *default_context() {
         Context *c = new (std::nothrow) Context();
         if (!c) return nullptr;
         c->device = dev;
+        c->lookback = coupe_hip::lookback_from_env();
         g_default[dev] = c;
     }
     return g_default[dev];
@@ -142,6 +144,7 @@ extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
     Context *c = new (std::nothrow) Context();
     if (!c) return nullptr;
     c->device = device;
+    c->lookback = coupe_hip::lookback_from_env();
     return reinterpret_cast<coupe_hip_ctx *>(c);
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
 #include <mutex>
 #include <vector>
 
@@ -57,6 +58,8 @@ struct Context {
     DevBuf heap_split;               // split values by heap index (shallow trees)
     DevBuf instances;                // instance table of the current call
     DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
+    DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
+    bool lookback = false;           // partition with decoupled look-back instead
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
@@ -71,6 +74,12 @@ struct Context {
 };
 
 enum coupe_err map_hip_error(hipError_t e);
+
+// COUPE_HIP_PARTITION=lookback selects the look-back partition.
+inline bool lookback_from_env() {
+    const char *v = getenv("COUPE_HIP_PARTITION");
+    return v && v[0] == 'l';
+}
 
 // RCB of a batch of independent instances packed in input order (see
 // coupe_hip_rcb_batch).  counts / levels / tols: one entry per instance.

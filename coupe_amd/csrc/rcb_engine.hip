@@ -240,6 +240,13 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // read it, while the two ping-pong tables serve levels 1, 2, ...
     Tile *tiles0 = (Tile *)cx.tiles0.ptr;
 
+    // Partition without look-back (default): per-tile left counts and their
+    // scan.  COUPE_HIP_PARTITION=lookback selects the decoupled look-back.
+    const bool lookback = cx.lookback;
+    cx.tile_left.ensure(tiles_cap * sizeof(uint64_t));
+    const bool fresh_below = cx.below.bytes < tiles_cap * sizeof(uint32_t);
+    cx.below.ensure(tiles_cap * sizeof(uint32_t));
+    if (fresh_below || !cx.bins_clean) check(hipMemsetAsync(cx.below.ptr, 0, cx.below.bytes, st));
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
     if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(L, 1u << 20) - 2) {
@@ -307,7 +314,8 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin,
                                                            first_standalone ? kFirstBins : 2, wrap32,
                                                            &ctr->searching,
-                                                           first_standalone ? &ctr->swept : &ctr->swept_fused);
+                                                           first_standalone ? &ctr->swept : &ctr->swept_fused,
+                                                           (first_standalone || lookback) ? 0u : 1u);
         });
         uint32_t passes = 1;
         const int nb = choose_bins(parts_ub);
@@ -327,7 +335,7 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept);
+                                                               &ctr->searching, &ctr->swept, 0u);
             });
             passes++;
         }
@@ -381,18 +389,34 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         if (level + 1 == L) break;   // every part left is at its last level
 
         // Partition of this level, fused with the children's first pass.
-        const uint32_t epoch = ++cx.epoch;
-        unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
-        if (level > 0 && level % kMaxTickets == 0)
-            check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
         float *const *ci = cs[cslot(level)];
         float *const *co = cs[cslot(level + 1)];
-        const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
-        timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-            k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
-                parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0], ci[1],
-                ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
-        });
+        if (lookback) {
+            const uint32_t epoch = ++cx.epoch;
+            unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
+            if (level > 0 && level % kMaxTickets == 0)
+                check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
+            const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
+            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                    parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0],
+                    ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
+            });
+        } else {
+            const unsigned gt = persistent_grid(cx, k_tile_left, tiles_ub);
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_tile_left<<<gt, kBlock, 0, st>>>(parts, tiles, ntl, ci[0], ci[1], ci[2], (uint32_t *)cx.below.ptr,
+                                                  (uint64_t *)cx.tile_left.ptr);
+            });
+            scan_u64(cx, st, (uint64_t *)cx.tile_left.ptr, ntl, tiles_ub, nullptr);
+            const unsigned g = persistent_grid(cx, k_partition_tiles<D, WS, Acc>, tiles_ub);
+            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                k_partition_tiles<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                    parts, (const Part *)cx.parts[nxt].ptr, tiles, ntl, (const uint64_t *)cx.tile_left.ptr, ci[0],
+                    ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin,
+                    (uint32_t *)cx.below.ptr);
+            });
+        }
 
         cur = nxt;
         parts_ub = std::max<uint64_t>(B.parts_ub[level + 1], 1);
@@ -576,7 +600,7 @@ uint64_t Context::device_bytes() const {
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                           &instances, &tiles0, &below, &tile_left, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -585,7 +609,7 @@ void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                     &instances, &tiles0, &below, &tile_left, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

@@ -461,7 +461,7 @@ template <typename Acc>
 __global__ void __launch_bounds__(64)
 k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
          Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
-         uint32_t *n_searching, unsigned long long *points_swept) {
+         uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready) {
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
     __shared__ uint32_t sm[kMaxBins + 1];
@@ -558,6 +558,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
             if (cnt == prev || hi <= t + d || imb <= tol) {   // :552-563
                 P->status = kSplitAtT; P->split = t; P->wl = Ops::to_bits(wl);
                 P->cnt_left = cnt; P->kstar = q;
+                P->pad = counts_ready;   // per-tile left counts already known
                 return;
             }
             prev = cnt;                                   // :564
@@ -1104,6 +1105,221 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
                 const float xb = sel3(a2, x0[k], x1[k], x2[k]);
                 r.add(!f[k], !(xb >= t2), AccOps<Acc>::of(wv[k]), f32_key(xb));
             }
+        }
+    }
+    if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
+}
+
+// ---------------------------------------------------------------------------
+// Partition without look-back.  Every tile's left count is known before the
+// partition: k_tile_left gathers it (from the fused counts of the previous
+// partition when the part stopped at its first candidate, else by counting),
+// a scan turns the counts into offsets, and every tile is then independent.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_tile_left(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+            const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
+            uint32_t *__restrict__ below, uint64_t *__restrict__ tile_left) {
+    __shared__ uint32_t wcount[kBlock / 64];
+    const uint32_t nt = *n_tiles;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        uint64_t left;
+        if (P.status == kDone || P.depth + 1 >= P.levels) {
+            left = 0;                          // not partitioned
+        } else if (P.status == kAllLeft) {
+            left = T.count;
+        } else if (P.pad) {
+            left = below[t];                   // counted by the previous partition
+        } else {
+            const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
+            const float split = P.split;
+            uint32_t k = 0;
+            for (uint64_t i = T.begin + threadIdx.x; i < T.begin + T.count; i += kBlock)
+                k += !(c[i] >= split) ? 1u : 0u;
+            k = wave_sum(k);
+            if (lane_id() == 0) wcount[threadIdx.x >> 6] = k;
+            __syncthreads();
+            left = 0;
+            for (int v = 0; v < kBlock / 64; v++) left += wcount[v];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            tile_left[t] = left;
+            below[t] = 0;                      // ready for the next partition
+        }
+    }
+}
+
+template <int D, typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_partition_tiles(const Part *__restrict__ parts, const Part *__restrict__ next_parts,
+                  const Tile *__restrict__ tiles, const uint32_t *n_tiles, const uint64_t *__restrict__ tile_prefix,
+                  const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
+                  const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1,
+                  float *__restrict__ o2, WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt,
+                  Acc *__restrict__ bw, uint32_t *__restrict__ bmin, uint32_t *__restrict__ below) {
+    __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
+    __shared__ uint32_t wtot[kItems][kBlock / 64];
+    __shared__ uint32_t s_tile_left;
+    __shared__ ChildBinsShared<Acc> sh;
+    float *sf0 = reinterpret_cast<float *>(stage);
+    float *sf1 = sf0 + kTile;
+    WS *sw = reinterpret_cast<WS *>(stage);
+    WS *sw1 = reinterpret_cast<WS *>(sf1);
+
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nt = *n_tiles;
+    ChildBins<Acc> r;
+    r.reset();
+    uint32_t cur = 0xffffffffu;
+    int32_t ch0 = -1, ch1 = -1;
+    int a2 = 0;
+    float t2 = 0.f;
+    uint64_t cb0 = 0, cb1 = 0;        // first position of each child
+    uint32_t ct0 = 0, ct1 = 0;        // first tile of each child
+
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        if (P.status == kDone || P.depth + 1 >= P.levels) continue;   // leaves: not moved
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
+            cur = T.part;
+            ch0 = P.child[0];
+            ch1 = P.child[1];
+            a2 = (int)((P.axis + 1) % D);
+            const float blo = sel3(a2, P.lo[0], P.lo[1], P.lo[2]);
+            const float bhi = sel3(a2, P.hi[0], P.hi[1], P.hi[2]);
+            t2 = (blo + bhi) / 2.0f;                     // the children's first candidate (:472)
+            cb0 = P.begin;
+            cb1 = P.begin + P.cnt_left;
+            ct0 = ch0 >= 0 ? next_parts[ch0].tile_begin : 0;
+            ct1 = ch1 >= 0 ? next_parts[ch1].tile_begin : 0;
+        }
+        const int ax = (int)P.axis;
+        const bool all_left = P.status == kAllLeft;
+        const float split = P.split;
+        const uint64_t end = T.begin + T.count;
+
+        float x0[kItems], x1[kItems], x2[kItems];
+        WS wv[kItems];
+        bool f[kItems];
+        uint32_t pre[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
+            const bool valid = p < end;
+            x0[k] = valid ? i0[p] : 0.f;
+            x1[k] = valid ? i1[p] : 0.f;
+            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
+            wv[k] = valid ? iw[p] : (WS)0;
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
+            f[k] = p < end && (all_left || !(sel3(ax, x0[k], x1[k], x2[k]) >= split));
+            const uint64_t bal = __ballot(f[k]);
+            pre[k] = __popcll(bal & lt_mask);
+            if (lane == 0) wtot[k][wid] = __popcll(bal);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kItems; k++)
+                for (int v = 0; v < kBlock / 64; v++) { uint32_t c = wtot[k][v]; wtot[k][v] = run; run += c; }
+            s_tile_left = run;
+        }
+        __syncthreads();
+        const uint32_t tl = s_tile_left;
+        const uint64_t prefix = tile_prefix[t] - tile_prefix[P.tile_begin];
+        const uint64_t lbase = P.begin + prefix;
+        const uint64_t rbase = P.begin + P.cnt_left + ((T.begin - P.begin) - prefix);
+        const uint32_t cnt = T.count;
+        uint32_t cp[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t off = k * kBlock + tid;
+            const uint32_t rl = wtot[k][wid] + pre[k];
+            cp[k] = f[k] ? rl : tl + (off - rl);
+        }
+#define COUPE_STORE_RUN(EXPR)                                                  \
+    _Pragma("unroll") for (int k = 0; k < kItems; k++) {                       \
+        const uint32_t pos = k * kBlock + tid;                                 \
+        if (pos < cnt) {                                                       \
+            const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);    \
+            if (d < n_total) { EXPR; }                                         \
+        }                                                                      \
+    }
+#pragma unroll
+        for (int k = 0; k < kItems; k++)
+            if (k * kBlock + tid < (int)cnt) { sf0[cp[k]] = x0[k]; sf1[cp[k]] = x1[k]; }
+        __syncthreads();
+        COUPE_STORE_RUN(o0[d] = sf0[pos]; o1[d] = sf1[pos])
+        __syncthreads();
+        if constexpr (sizeof(WS) == 4) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) {
+                    if constexpr (D == 3) sf0[cp[k]] = x2[k];
+                    sw1[cp[k]] = wv[k];
+                }
+            __syncthreads();
+            if constexpr (D == 3) {
+                COUPE_STORE_RUN(o2[d] = sf0[pos]; ow[d] = sw1[pos])
+            } else {
+                COUPE_STORE_RUN(ow[d] = sw1[pos])
+            }
+            __syncthreads();
+        } else {
+            if constexpr (D == 3) {
+#pragma unroll
+                for (int k = 0; k < kItems; k++)
+                    if (k * kBlock + tid < (int)cnt) sf0[cp[k]] = x2[k];
+                __syncthreads();
+                COUPE_STORE_RUN(o2[d] = sf0[pos])
+                __syncthreads();
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) sw[cp[k]] = wv[k];
+            __syncthreads();
+            COUPE_STORE_RUN(ow[d] = sw[pos])
+            __syncthreads();
+        }
+#undef COUPE_STORE_RUN
+        // Children's first pass, and per child tile the points below their
+        // first candidate: this tile's left run spans at most two tiles of
+        // the left child, its right run two of the right child.
+        const uint32_t jl = ct0 + (uint32_t)((lbase - cb0) / kTile);
+        const uint32_t jr = ct1 + (uint32_t)((rbase - cb1) / kTile);
+        uint32_t nl0 = 0, nl1 = 0, nr0 = 0, nr1 = 0;
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            if (k * kBlock + tid < (int)cnt) {
+                const float xb = sel3(a2, x0[k], x1[k], x2[k]);
+                const bool bel = !(xb >= t2);
+                r.add(!f[k], bel, AccOps<Acc>::of(wv[k]), f32_key(xb));
+                const uint64_t d = f[k] ? lbase + cp[k] : rbase + (cp[k] - tl);
+                const uint32_t j = f[k] ? ct0 + (uint32_t)((d - cb0) / kTile) : ct1 + (uint32_t)((d - cb1) / kTile);
+                const bool first = j == (f[k] ? jl : jr);
+                nl0 += (bel && f[k] && first) ? 1u : 0u;
+                nl1 += (bel && f[k] && !first) ? 1u : 0u;
+                nr0 += (bel && !f[k] && first) ? 1u : 0u;
+                nr1 += (bel && !f[k] && !first) ? 1u : 0u;
+            }
+        }
+        nl0 = wave_sum(nl0);
+        nl1 = wave_sum(nl1);
+        nr0 = wave_sum(nr0);
+        nr1 = wave_sum(nr1);
+        if (lane == 0) {
+            if (nl0) atomicAdd(&below[jl], nl0);
+            if (nl1) atomicAdd(&below[jl + 1], nl1);
+            if (nr0) atomicAdd(&below[jr], nr0);
+            if (nr1) atomicAdd(&below[jr + 1], nr1);
         }
     }
     if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);

@@ -193,6 +193,7 @@ extern "C" coupe_hip_server *coupe_hip_server_create(const struct coupe_hip_serv
             return nullptr;
         }
         W->cx.device = config->device;
+        W->cx.lookback = coupe_hip::lookback_from_env();
         S->workers.push_back(W);
     }
     for (Worker *W : S->workers) W->th = std::thread(worker_main, S, W);
