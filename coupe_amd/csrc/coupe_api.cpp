@@ -212,3 +212,22 @@ extern "C" enum coupe_err coupe_hip_kernel_times(const coupe_hip_ctx *ctx, struc
     for (int k = 0; k < count && k < COUPE_HIP_K_COUNT; k++) out[k] = c->prof.out[k];
     return COUPE_ERR_OK;
 }
+
+extern "C" enum coupe_err coupe_hip_parts_load(coupe_hip_ctx *ctx, void *stream, const uint64_t *d_partition,
+                                               uintptr_t n, enum coupe_type weight_type, const void *d_weights,
+                                               uintptr_t num_parts, void *loads) {
+    if (!ctx || (num_parts && !loads)) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    return coupe_hip::parts_load(*c, (hipStream_t)stream, d_partition, n, weight_type, d_weights, num_parts, loads);
+}
+
+extern "C" enum coupe_err coupe_hip_imbalance(coupe_hip_ctx *ctx, void *stream, const uint64_t *d_partition,
+                                              uintptr_t n, enum coupe_type weight_type, const void *d_weights,
+                                              uintptr_t num_parts, double *imbalance) {
+    if (!ctx || !imbalance) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    return coupe_hip::imbalance(*c, (hipStream_t)stream, d_partition, n, weight_type, d_weights, num_parts,
+                                imbalance);
+}

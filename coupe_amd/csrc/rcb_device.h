@@ -1,0 +1,111 @@
+// rcb_device.h -- device helpers shared by the HIP kernels: accumulator
+// types, wavefront reductions, axis select and candidate bins.
+#pragma once
+
+#include <type_traits>
+
+#include "rcb_common.h"
+
+namespace coupe_hip {
+
+// ---------------------------------------------------------------------------
+// Accumulator helpers.  Acc = uint64_t: integer weights, two's complement sums
+// (exact; i32 semantics are recovered by truncation when wrap32 is set, as
+// Rust's wrapping i32 arithmetic is a ring morphism mod 2^32).  Acc = double:
+// non-integer f64 weights.
+// ---------------------------------------------------------------------------
+template <typename Acc> struct AccOps;
+
+template <> struct AccOps<uint64_t> {
+    __device__ static uint64_t from_bits(uint64_t b) { return b; }
+    __device__ static uint64_t to_bits(uint64_t v) { return v; }
+    template <typename W> __device__ static uint64_t of(W w) { return (uint64_t)(int64_t)w; }
+    // RcbWeight::to_f64 (num-traits): i32 / i64 -> f64, round to nearest.
+    __device__ static double f64(uint64_t v, int wrap32) {
+        return wrap32 ? (double)(int32_t)(uint32_t)v : (double)(int64_t)v;
+    }
+    __device__ static bool lt(uint64_t a, uint64_t b, int wrap32) {
+        return wrap32 ? (int32_t)(uint32_t)a < (int32_t)(uint32_t)b : (int64_t)a < (int64_t)b;
+    }
+    __device__ static uint64_t sub(uint64_t a, uint64_t b) { return a - b; }
+};
+
+template <> struct AccOps<double> {
+    __device__ static double from_bits(uint64_t b) { return __builtin_bit_cast(double, b); }
+    __device__ static uint64_t to_bits(double v) { return __builtin_bit_cast(uint64_t, v); }
+    template <typename W> __device__ static double of(W w) { return (double)w; }
+    __device__ static double f64(double v, int) { return v; }
+    __device__ static bool lt(double a, double b, int) { return a < b; }
+    __device__ static double sub(double a, double b) { return a - b; }
+};
+
+__device__ inline void atomic_add_acc(uint64_t *p, uint64_t v) {
+    atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+__device__ inline void atomic_add_acc(double *p, double v) { atomicAdd(p, v); }
+
+__device__ inline int lane_id() { return (int)__lane_id(); }
+
+template <typename T> __device__ inline T wave_incl_add(T v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T n = __shfl_up(v, o, 64);
+        if (lane >= o) v += n;
+    }
+    return v;
+}
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t x = __shfl_xor(v, o, 64);
+        v = x < v ? x : v;
+    }
+    return v;
+}
+
+// v[a] for a uniform runtime axis a, as bit masks the compiler cannot turn
+// into an indexed (scratch) access to a register array.
+__device__ __forceinline__ float sel3(int a, float v0, float v1, float v2) {
+    uint32_t m0 = a == 0 ? 0xffffffffu : 0u, m1 = a == 1 ? 0xffffffffu : 0u, m2 = a == 2 ? 0xffffffffu : 0u;
+    asm volatile("" : "+v"(m0), "+v"(m1), "+v"(m2));
+    return __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, v0) & m0) |
+                                         (__builtin_bit_cast(uint32_t, v1) & m1) |
+                                         (__builtin_bit_cast(uint32_t, v2) & m2));
+}
+
+// Candidate `p` (in-order index among nb-1) of the bisection tree rooted at
+// (lo, hi): the midpoint sequence par_rcb_split computes (:472), following
+// the path to node p.  Going left sets max = t, going right min = t.
+__device__ inline float candidate(float lo, float hi, int p, int nb) {
+    int s = nb >> 1;
+    int q = s - 1;
+    for (;;) {
+        float t = (lo + hi) / 2.0f;
+        if (p == q) return t;
+        s >>= 1;
+        if (p < q) { hi = t; q -= s; } else { lo = t; q += s; }
+    }
+}
+
+// Number of candidates <= x (sorted ascending): the bin of x.  Bin j holds
+// cand[j-1] <= c < cand[j], so count(c < cand[q]) = sum of bins 0..q.
+// A NaN coordinate lands in bin 0, like !(c >= t) in the partition.
+__device__ inline int bin_of(const float *cand, int nb, float x) {
+    int pos = 0;
+    for (int step = nb >> 1; step >= 1; step >>= 1)
+        if (x >= cand[pos + step - 1]) pos += step;
+    return pos;
+}
+// Same for the 3 register candidates of the first pass.
+__device__ inline int bin4(float c0, float c1, float c2, float x) {
+    int b = (x >= c1) ? 2 : 0;
+    b += (x >= (b ? c2 : c0)) ? 1 : 0;
+    return b;
+}
+}  // namespace coupe_hip

@@ -60,6 +60,7 @@ struct Context {
     DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
     DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
     bool lookback = false;           // partition with decoupled look-back instead
+    DevBuf loads;                    // per-part loads (imbalance)
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
@@ -86,6 +87,12 @@ inline bool lookback_from_env() {
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
                          const double *tols, size_t n_inst);
+
+// Per-part loads and imbalance of a partition (coupe/src/imbalance.rs).
+enum coupe_err parts_load(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,
+                          uint64_t num_parts, void *host_loads);
+enum coupe_err imbalance(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,
+                         uint64_t num_parts, double *out);
 
 // RCB on device-resident inputs (see coupe_hip_rcb_device).
 enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,

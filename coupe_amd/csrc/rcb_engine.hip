@@ -600,7 +600,7 @@ uint64_t Context::device_bytes() const {
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &below, &tile_left, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                           &instances, &tiles0, &below, &tile_left, &loads, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -609,7 +609,7 @@ void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &below, &tile_left, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                     &instances, &tiles0, &below, &tile_left, &loads, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

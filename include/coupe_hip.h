@@ -68,6 +68,26 @@ enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d
 		const void *d_weights, const struct coupe_hip_instance *instances, uintptr_t n_instances);
 
 /*
+ * Per-part loads of a partition held in device memory: loads[p] = sum of the
+ * weights of the points with id p, in the weight type (int sums wrap like
+ * Rust's release arithmetic; int64 exact; double in device order).
+ * `loads` is a host array of num_parts values of weight_type.  Ids >=
+ * num_parts return COUPE_ERR_CRASH (coupe would panic on the index).
+ * Replaces: coupe::imbalance::compute_parts_load, coupe/src/imbalance.rs:14-39.
+ */
+enum coupe_err coupe_hip_parts_load(coupe_hip_ctx *ctx, void *stream, const uint64_t *d_partition,
+		uintptr_t n, enum coupe_type weight_type, const void *d_weights, uintptr_t num_parts,
+		void *loads);
+/*
+ * Imbalance of a partition: the largest (load - ideal) / ideal over the parts,
+ * ideal = total / num_parts; 0 when num_parts or the ideal load is 0.
+ * Replaces: coupe::imbalance::imbalance, coupe/src/imbalance.rs:41-77.
+ */
+enum coupe_err coupe_hip_imbalance(coupe_hip_ctx *ctx, void *stream, const uint64_t *d_partition,
+		uintptr_t n, enum coupe_type weight_type, const void *d_weights, uintptr_t num_parts,
+		double *imbalance);
+
+/*
  * Serving: a request server with continuous batching.
  *
  * `slots` workers each own a context and a stream.  A free worker takes the

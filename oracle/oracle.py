@@ -18,6 +18,11 @@ The oracle restates coupe's ``Rcb`` (coupe/src/algorithms/recursive_bisection.rs
 * ``split_1d`` -- the invariants of ``test_par_rcb_split`` (:1041-1075).
 * ``reorder_1d`` -- the invariants of ``test_reorder_split_scalar``
   (:952-983).
+* ``parts_load`` / ``imbalance`` / ``imbalance_target`` / ``max_imbalance``
+  (numpy restatement of coupe/src/imbalance.rs:14-105) -- closed forms on
+  hand-made partitions (equal loads give 0, one part holding everything gives
+  num_parts - 1), i32 wrapping sums, the zero-part and zero-weight guards,
+  and ``np.bincount`` for the per-part sums.
 * Non-integer f64 weights: the reference's summation order is rayon's and
   unspecified, so ids for real weights are "parity unpinned" beyond the
   1e-12 relative per-part weight tolerance used by the GPU tests.
@@ -175,3 +180,67 @@ def reorder_1d(coords, pivot):
 
 def num_threads() -> int:
     return lib().oracle_num_threads()
+
+
+def parts_load(partition, num_parts, weights):
+    """coupe::imbalance::compute_parts_load (coupe/src/imbalance.rs:14-39):
+    per-part sums of the weights in the weight type (i32 sums wrap mod 2^32
+    as Rust's release arithmetic does).  Raises IndexError for an id >=
+    num_parts (coupe's ``acc[part]`` panics)."""
+    ids = np.asarray(partition, dtype=np.uint64)
+    w = np.asarray(weights)
+    if ids.size and int(ids.max()) >= num_parts:
+        raise IndexError("part id out of range")
+    if w.dtype == np.float64:
+        out = np.zeros(num_parts, dtype=np.float64)
+        for p, x in zip(ids.tolist(), w.tolist()):
+            out[p] += x
+        return out
+    acc = [0] * num_parts
+    for p, x in zip(ids.tolist(), w.tolist()):
+        acc[p] += int(x)
+    bits = 32 if w.dtype == np.int32 else 64
+    wrap = [((a + (1 << (bits - 1))) % (1 << bits)) - (1 << (bits - 1)) for a in acc]
+    return np.array(wrap, dtype=np.int32 if bits == 32 else np.int64)
+
+
+def _wsum(vals, dtype):
+    if dtype == np.float64:
+        s = 0.0
+        for x in vals.tolist():
+            s += x
+        return s
+    bits = 32 if dtype == np.int32 else 64
+    s = sum(int(x) for x in vals.tolist())
+    return ((s + (1 << (bits - 1))) % (1 << bits)) - (1 << (bits - 1))
+
+
+def imbalance(num_parts, partition, weights):
+    """coupe::imbalance::imbalance (coupe/src/imbalance.rs:41-77)."""
+    if num_parts == 0:
+        return 0.0
+    loads = parts_load(partition, num_parts, weights)
+    total = _wsum(loads, loads.dtype)
+    ideal = float(total) / float(num_parts)
+    if ideal == 0.0:
+        return 0.0
+    return max((float(x) - ideal) / ideal for x in loads.tolist())
+
+
+def imbalance_target(targets, partition, weights):
+    """coupe::imbalance::imbalance_target (coupe/src/imbalance.rs:79-92):
+    the largest load - target, in the weight type (0 without parts)."""
+    t = np.asarray(targets)
+    loads = parts_load(partition, t.size, weights)
+    if t.size == 0:
+        return loads.dtype.type(0)
+    return max(loads[i] - t[i] for i in range(t.size))
+
+
+def max_imbalance(num_parts, partition, weights):
+    """coupe::imbalance::max_imbalance (coupe/src/imbalance.rs:94-105):
+    largest minus smallest part load, in the weight type."""
+    loads = parts_load(partition, num_parts, weights)
+    if loads.size == 0:
+        return loads.dtype.type(0)
+    return loads.max() - loads.min()
