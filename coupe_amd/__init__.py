@@ -27,7 +27,7 @@ import numpy as np
 from . import _native
 from ._native import strerror
 
-__all__ = ["Rcb", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
+__all__ = ["Rcb", "Rib", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
            "partition_batch", "Server"]
 
 
@@ -92,6 +92,14 @@ class DeviceContext:
         _native.lib().coupe_hip_last_stats(self._ctx, C.byref(st))
         return st.as_dict()
 
+    def last_rib(self) -> dict:
+        """Rotation of this context's last Rib call (coupe_hip_last_rib)."""
+        info = _native.RibInfo()
+        err = _native.lib().coupe_hip_last_rib(self.handle, C.byref(info))
+        if err != _native.OK:
+            raise CoupeError(err)
+        return info.as_dict()
+
     def set_profiling(self, enable: bool):
         _native.lib().coupe_hip_set_profiling(self._ctx, 1 if enable else 0)
 
@@ -136,6 +144,9 @@ class Rcb(Partition):
     iter_count: int = 0
     tolerance: float = 0.0
 
+    _host_fn = "coupe_rcb"
+    _device_fn = "coupe_hip_rcb_device"
+
     def partition(self, part_ids, data: Tuple[Any, Any], *, stream=None, context: DeviceContext = None):
         points, weights = data
         if _is_device_tensor(points) or _is_device_tensor(part_ids):
@@ -164,7 +175,8 @@ class Rcb(Partition):
         dp = L.coupe_data_array(n_part, _native.COUPE_DOUBLE, pts.ctypes.data)
         dw = L.coupe_data_array(n_part, wtype, w.ctypes.data)
         try:
-            err = L.coupe_rcb(part_ids.ctypes.data, dim, dp, dw, int(self.iter_count), float(self.tolerance))
+            err = getattr(L, self._host_fn)(part_ids.ctypes.data, dim, dp, dw, int(self.iter_count),
+                                            float(self.tolerance))
         finally:
             L.coupe_data_free(dp)
             L.coupe_data_free(dw)
@@ -192,12 +204,32 @@ class Rcb(Partition):
         st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
         dim = points.shape[1] if points.dim() == 2 else 0
         wtype = _weight_type(weights.dtype)
-        err = _native.lib().coupe_hip_rcb_device(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), n_part, dim,
-                                                  points.data_ptr(), wtype, weights.data_ptr(),
-                                                  int(self.iter_count), float(self.tolerance))
+        err = getattr(_native.lib(), self._device_fn)(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), n_part, dim,
+                                                      points.data_ptr(), wtype, weights.data_ptr(),
+                                                      int(self.iter_count), float(self.tolerance))
         if err != _native.OK:
             raise CoupeError(err)
         return None
+
+
+@dataclass
+class Rib(Rcb):
+    """coupe::Rib { iter_count, tolerance } (recursive_bisection.rs:901-930):
+    RCB after a change of basis whose first axis is the points' inertia axis.
+
+    As coupe's ``rib`` (:829-851), an empty point set returns before any
+    length check; otherwise the checks and errors are those of Rcb.  The
+    rotation of the last device call is ``DeviceContext.last_rib()``.
+    """
+
+    _host_fn = "coupe_rib"
+    _device_fn = "coupe_hip_rib_device"
+
+    def partition(self, part_ids, data: Tuple[Any, Any], *, stream=None, context: DeviceContext = None):
+        points, _ = data
+        if len(points) == 0:   # OrientedBoundingBox::from_points is None
+            return None
+        return super().partition(part_ids, data, stream=stream, context=context)
 
 
 def partition_batch(part_ids, points, weights, instances, *, stream=None, context: DeviceContext = None):

@@ -20,8 +20,9 @@ COUPE_INT, COUPE_INT64, COUPE_DOUBLE = 0, 1, 2
 # Every symbol the headers declare, in declaration order.
 EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
-    "coupe_rcb",
+    "coupe_rcb", "coupe_rib",
     "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_batch",
+    "coupe_hip_rib_device", "coupe_hip_last_rib",
     "coupe_hip_parts_load", "coupe_hip_imbalance",
     "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
     "coupe_hip_last_stats", "coupe_hip_set_profiling", "coupe_hip_kernel_times",
@@ -45,10 +46,22 @@ class RequestStats(C.Structure):
                 ("batch_requests", C.c_uint32), ("batch_points", C.c_uint64)]
 
 
+class RibInfo(C.Structure):
+    _fields_ = [("dimension", C.c_size_t), ("centroid", C.c_double * 3), ("inertia", C.c_double * 9),
+                ("axis", C.c_double * 3), ("aabb_to_obb", C.c_double * 9), ("obb_to_aabb", C.c_double * 9)]
+
+    def as_dict(self):
+        d = self.dimension
+        mat = lambda a: [[a[i * d + j] for j in range(d)] for i in range(d)]
+        return {"dimension": d, "centroid": list(self.centroid[:d]), "inertia": mat(self.inertia),
+                "axis": list(self.axis[:d]), "aabb_to_obb": mat(self.aabb_to_obb),
+                "obb_to_aabb": mat(self.obb_to_aabb)}
+
+
 class Instance(C.Structure):
     _fields_ = [("n", C.c_size_t), ("iter_count", C.c_size_t), ("tolerance", C.c_double)]
 
-KERNELS = ["prologue", "hist", "decide", "partition", "ids", "other"]
+KERNELS = ["prologue", "hist", "decide", "partition", "ids", "other", "moments"]
 
 
 class KernelTime(C.Structure):
@@ -103,6 +116,8 @@ def lib():
         L.coupe_data_fn.restype = C.c_void_p
         L.coupe_rcb.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_size_t, C.c_double]
         L.coupe_rcb.restype = C.c_int
+        L.coupe_rib.argtypes = L.coupe_rcb.argtypes
+        L.coupe_rib.restype = C.c_int
         L.coupe_hip_ctx_create.argtypes = [C.c_int]
         L.coupe_hip_ctx_create.restype = C.c_void_p
         L.coupe_hip_ctx_destroy.argtypes = [C.c_void_p]
@@ -110,6 +125,10 @@ def lib():
         L.coupe_hip_rcb_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                            C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_double]
         L.coupe_hip_rcb_device.restype = C.c_int
+        L.coupe_hip_rib_device.argtypes = L.coupe_hip_rcb_device.argtypes
+        L.coupe_hip_rib_device.restype = C.c_int
+        L.coupe_hip_last_rib.argtypes = [C.c_void_p, C.POINTER(RibInfo)]
+        L.coupe_hip_last_rib.restype = C.c_int
         L.coupe_hip_rcb_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                           C.c_void_p, C.POINTER(Instance), C.c_size_t]
         L.coupe_hip_rcb_batch.restype = C.c_int

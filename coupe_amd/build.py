@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 LIB = os.path.join(HERE, "libcoupe_hip.so")
 WORKLOADS = os.path.join(HERE, "libcoupe_workloads.so")
 
-HIP_SOURCES = ["rcb_engine.hip", "imbalance.hip", "coupe_api.cpp", "serve.cpp"]
+HIP_SOURCES = ["rcb_engine.hip", "rib.hip", "imbalance.hip", "coupe_api.cpp", "serve.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
     "--offload-arch=gfx950",

@@ -98,8 +98,12 @@ Context *default_context() {
 
 }  // namespace
 
-extern "C" enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension, const coupe_data *points,
-                                    const coupe_data *weights, uintptr_t iter_count, double tolerance) {
+namespace {
+
+// Host-memory contract shared by coupe_rcb and coupe_rib
+// (coupe-ffi/src/lib.rs:272-304, :331-364).
+enum coupe_err host_partition(uintptr_t *partition, uintptr_t dimension, const coupe_data *points,
+                              const coupe_data *weights, uintptr_t iter_count, double tolerance, bool rib) {
     const uintptr_t n = points->len;
     if (n != weights->len) return COUPE_ERR_LEN_MISMATCH;   // lib.rs:285-288
     if (dimension != 2 && dimension != 3) return COUPE_ERR_BAD_DIMENSION;   // lib.rs:297-301
@@ -124,9 +128,10 @@ extern "C" enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension, c
             return coupe_hip::map_hip_error(e);
         if ((e = hipMemcpy(cx->in_weights.ptr, hw, wbytes, hipMemcpyHostToDevice)) != hipSuccess)
             return coupe_hip::map_hip_error(e);
-        enum coupe_err r = coupe_hip::rcb_device(*cx, st, (uint64_t *)cx->out_ids.ptr, n, dimension,
-                                                 (const double *)cx->in_points.ptr, weights->type,
-                                                 cx->in_weights.ptr, iter_count, tolerance);
+        auto run = rib ? coupe_hip::rib_device : coupe_hip::rcb_device;
+        enum coupe_err r = run(*cx, st, (uint64_t *)cx->out_ids.ptr, n, dimension,
+                               (const double *)cx->in_points.ptr, weights->type, cx->in_weights.ptr, iter_count,
+                               tolerance);
         if (r != COUPE_ERR_OK) return r;
         if ((e = hipMemcpy(partition, cx->out_ids.ptr, obytes, hipMemcpyDeviceToHost)) != hipSuccess)
             return coupe_hip::map_hip_error(e);
@@ -136,6 +141,18 @@ extern "C" enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension, c
         return COUPE_ERR_ALLOC;   // DevBuf::ensure failures
     }
     return COUPE_ERR_OK;
+}
+
+}  // namespace
+
+extern "C" enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension, const coupe_data *points,
+                                    const coupe_data *weights, uintptr_t iter_count, double tolerance) {
+    return host_partition(partition, dimension, points, weights, iter_count, tolerance, false);
+}
+
+extern "C" enum coupe_err coupe_rib(uintptr_t *partition, uintptr_t dimension, const coupe_data *points,
+                                    const coupe_data *weights, uintptr_t iter_count, double tolerance) {
+    return host_partition(partition, dimension, points, weights, iter_count, tolerance, true);
 }
 
 extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
@@ -165,6 +182,25 @@ extern "C" enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream,
     std::lock_guard<std::mutex> g(c->lock);
     return coupe_hip::rcb_device(*c, (hipStream_t)stream, d_partition, n, dimension, d_points, weight_type,
                                  d_weights, iter_count, tolerance);
+}
+
+extern "C" enum coupe_err coupe_hip_rib_device(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition, uintptr_t n,
+                                               uintptr_t dimension, const double *d_points,
+                                               enum coupe_type weight_type, const void *d_weights,
+                                               uintptr_t iter_count, double tolerance) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    return coupe_hip::rib_device(*c, (hipStream_t)stream, d_partition, n, dimension, d_points, weight_type,
+                                 d_weights, iter_count, tolerance);
+}
+
+extern "C" enum coupe_err coupe_hip_last_rib(coupe_hip_ctx *ctx, struct coupe_hip_rib_info *info) {
+    if (!ctx || !info) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    *info = c->rib;
+    return COUPE_ERR_OK;
 }
 
 extern "C" enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,

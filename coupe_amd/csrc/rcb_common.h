@@ -88,6 +88,12 @@ struct Node {
     int32_t child[2];      // node index of the left / right child (-1: empty or leaf level)
 };
 
+// Change of basis applied to every point in the prologue (Rib): row-major
+// D x D, p -> m p with the sum over columns taken left to right.
+struct Rotation {
+    double m[9];
+};
+
 // Results of the prologue reductions over the whole batch.
 struct Prologue {
     double sum_abs;             // sum of |w| (exactness guard)

@@ -13,6 +13,8 @@
 
 namespace coupe_hip {
 
+struct Rotation;   // rcb_common.h
+
 // A HIP failure inside the engine; entry points map it onto coupe's codes.
 struct HipFailure {
     hipError_t err;
@@ -61,6 +63,9 @@ struct Context {
     DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
+    DevBuf rib_moments;              // Rib: block partials, centroid, inertia
+    coupe_hip_rib_info rib{};        // Rib: rotation of the last call
+    const Rotation *rot = nullptr;   // change of basis for the running call
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
@@ -84,9 +89,15 @@ inline bool lookback_from_env() {
 
 // RCB of a batch of independent instances packed in input order (see
 // coupe_hip_rcb_batch).  counts / levels / tols: one entry per instance.
+// With `rot`, every point is first mapped by the rotation (Rib); the call
+// then adds to the profile the caller started instead of starting one.
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
-                         const double *tols, size_t n_inst);
+                         const double *tols, size_t n_inst, const Rotation *rot = nullptr);
+
+// coupe::Rib on device-resident inputs (see coupe_hip_rib_device).
+enum coupe_err rib_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
+                          const double *d_points, int wkind, const void *d_weights, size_t iter_count, double tol);
 
 // Per-part loads and imbalance of a partition (coupe/src/imbalance.rs).
 enum coupe_err parts_load(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,

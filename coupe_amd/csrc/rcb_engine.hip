@@ -475,15 +475,22 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
     });
     const uint32_t nt = (uint32_t)B.tiles0;
     if (nt > 0) {
-        const unsigned g = persistent_grid(cx, k_prologue<D, WIN>, nt);
         // reads AoS f64 points and the input weights; writes SoA f32 and int32 weights
         const double bytes = (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0);
-        timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
-            k_prologue<D, WIN><<<g, kBlock, 0, st>>>(
-                (const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr, (float *)cx.cx[0][1].ptr,
-                (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr), (int32_t *)cx.w[0].ptr,
-                (Instance *)cx.instances.ptr, (Prologue *)cx.prologue.ptr);
-        });
+        Rotation rot;
+        memset(&rot, 0, sizeof rot);
+        if (cx.rot) rot = *cx.rot;
+        auto launch = [&](auto kernel) {
+            const unsigned g = persistent_grid(cx, kernel, nt);
+            timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
+                kernel<<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr,
+                                             (float *)cx.cx[0][1].ptr, (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
+                                             (int32_t *)cx.w[0].ptr, (Instance *)cx.instances.ptr,
+                                             (Prologue *)cx.prologue.ptr, rot);
+            });
+        };
+        if (cx.rot) launch(k_prologue<D, WIN, true>);
+        else launch(k_prologue<D, WIN, false>);
     }
     check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
@@ -552,11 +559,16 @@ static void run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPla
 
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
-                         const double *tols, size_t n_inst) {
+                         const double *tols, size_t n_inst, const Rotation *rot) {
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
     memset(&cx.stats, 0, sizeof cx.stats);
-    cx.prof.reset();
+    if (!rot) cx.prof.reset();
+    struct RotScope {
+        Context &c;
+        RotScope(Context &c_, const Rotation *r) : c(c_) { c.rot = r; }
+        ~RotScope() { c.rot = nullptr; }
+    } rot_scope(cx, rot);
     cx.swept_bytes = 0.0;
     try {
         BatchPlan B = plan_batch(counts, levels, tols, n_inst);
@@ -600,7 +612,7 @@ uint64_t Context::device_bytes() const {
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &below, &tile_left, &loads, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -609,7 +621,7 @@ void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &below, &tile_left, &loads, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

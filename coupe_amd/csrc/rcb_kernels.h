@@ -89,11 +89,13 @@ __device__ __forceinline__ void flush_instance(Instance *I, double (&mn)[D], dou
 // Workgroups stride over the level-0 tiles (each inside one instance) and
 // read the AoS points of a tile through LDS so every global load is
 // contiguous.
-template <int D, typename WIN>
+// With ROT every point p is replaced by rot.m p (Rib's obb_to_aabb map,
+// recursive_bisection.rs:850) before its coordinates are used.
+template <int D, typename WIN, bool ROT>
 __global__ void __launch_bounds__(kBlock)
 k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__restrict__ pts,
            const WIN *__restrict__ win, float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
-           int32_t *__restrict__ w32, Instance *__restrict__ insts, Prologue *out) {
+           int32_t *__restrict__ w32, Instance *__restrict__ insts, Prologue *out, Rotation rot) {
     __shared__ double sp[kProPoints * D];
     __shared__ double s_d[kBlock / 64 * 8];
     __shared__ uint64_t s_u[kBlock / 64];
@@ -122,9 +124,23 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                 const uint32_t p = k * kBlock + tid;
                 if (p < cnt) {
                     const uint64_t i = base + p;
+                    double q[D];
+#pragma unroll
+                    for (int a = 0; a < D; a++) q[a] = sp[p * D + a];
+                    if constexpr (ROT) {
+                        double r[D];
+#pragma unroll
+                        for (int a = 0; a < D; a++) {
+                            r[a] = rot.m[a * D] * q[0];
+#pragma unroll
+                            for (int b = 1; b < D; b++) r[a] = r[a] + rot.m[a * D + b] * q[b];
+                        }
+#pragma unroll
+                        for (int a = 0; a < D; a++) q[a] = r[a];
+                    }
 #pragma unroll
                     for (int a = 0; a < D; a++) {
-                        const double v = sp[p * D + a];
+                        const double v = q[a];
                         if (v < mn[a]) mn[a] = v;
                         if (mx[a] < v) mx[a] = v;
                         float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);

@@ -113,6 +113,18 @@ enum coupe_err coupe_rcb(uintptr_t *partition, uintptr_t dimension,
 		const coupe_data *points, const coupe_data *weights,
 		uintptr_t iter_count, double tolerance);
 
+/*
+ * Recursive Inertial Bisection: coupe_rcb after a change of basis whose first
+ * axis is the inertia axis of the points.  Same arguments, data-set rules and
+ * errors as coupe_rcb.
+ *
+ * Replaces: coupe-ffi/include/coupe.h:294-296 (coupe_rib),
+ * coupe-ffi/src/lib.rs:331-364.
+ */
+enum coupe_err coupe_rib(uintptr_t *partition, uintptr_t dimension,
+		const coupe_data *points, const coupe_data *weights,
+		uintptr_t iter_count, double tolerance);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,35 @@ enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d
 		const void *d_weights, const struct coupe_hip_instance *instances, uintptr_t n_instances);
 
 /*
+ * coupe::Rib on device-resident data: same arguments and errors as
+ * coupe_hip_rcb_device.  The points are rotated so that the first axis is
+ * their inertia axis, then partitioned by RCB; the rotation is applied on the
+ * fly (the points are not modified and no rotated copy is stored).
+ * Replaces: coupe::Rib::partition, coupe/src/algorithms/recursive_bisection.rs:829-851, :911-930.
+ */
+enum coupe_err coupe_hip_rib_device(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition, uintptr_t n,
+		uintptr_t dimension, const double *d_points, enum coupe_type weight_type,
+		const void *d_weights, uintptr_t iter_count, double tolerance);
+
+/*
+ * The rotation of the context's last Rib call.  Matrices are row-major
+ * dimension x dimension, packed at the start of their arrays.
+ * centroid / inertia: geometry.rs:256-267; axis: geometry.rs:269-288 (unit
+ * eigenvector of the largest eigenvalue); aabb_to_obb: the Householder
+ * reflection, geometry.rs:290-305; obb_to_aabb: its inverse, the map applied
+ * to every point (geometry.rs:217-231).
+ */
+struct coupe_hip_rib_info {
+	uintptr_t dimension;
+	double centroid[3];
+	double inertia[9];
+	double axis[3];
+	double aabb_to_obb[9];
+	double obb_to_aabb[9];
+};
+enum coupe_err coupe_hip_last_rib(coupe_hip_ctx *ctx, struct coupe_hip_rib_info *info);
+
+/*
  * Per-part loads of a partition held in device memory: loads[p] = sum of the
  * weights of the points with id p, in the weight type (int sums wrap like
  * Rust's release arithmetic; int64 exact; double in device order).
@@ -167,6 +196,7 @@ enum coupe_hip_kernel {
 	COUPE_HIP_K_PARTITION,  /* stable partition of all parts of a level */
 	COUPE_HIP_K_IDS,        /* part ids through the original index */
 	COUPE_HIP_K_OTHER,      /* tables, scans, weight conversion */
+	COUPE_HIP_K_MOMENTS,    /* Rib: centroid and inertia-matrix passes */
 	COUPE_HIP_K_COUNT
 };
 struct coupe_hip_kernel_time {

@@ -23,6 +23,18 @@ The oracle restates coupe's ``Rcb`` (coupe/src/algorithms/recursive_bisection.rs
   hand-made partitions (equal loads give 0, one part holding everything gives
   num_parts - 1), i32 wrapping sums, the zero-part and zero-weight guards,
   and ``np.bincount`` for the per-part sums.
+* ``rib_frame`` (inertia matrix, axis, Householder reflection, inverse;
+  oracle/rib_oracle.c) -- the reference's geometry tests
+  (geometry.rs:363-393 inertia matrix [[18,-18],[-18,18]] and axis
+  parallel to (1,-1); :505-518 axis parallel to (1,-1,-1); :395-408 the box
+  centre (3,3); :460-487 reflection columns orthonormal, first column
+  parallel to the axis), numpy.linalg.eigh (LAPACK) on random symmetric
+  matrices, and H H^-1 = I.  ``rib`` -- the ``Rib`` doctest
+  (recursive_bisection.rs:866-894) and, given a rotation, ``rcb`` on the
+  mapped points.  nalgebra's iterative eigen-solver is not restated, so the
+  last bits of the rotation, and ids that depend on them, are "parity
+  unpinned" against the reference; the GPU tests compare ids given the
+  rotation the GPU used.
 * Non-integer f64 weights: the reference's summation order is rayon's and
   unspecified, so ids for real weights are "parity unpinned" beyond the
   1e-12 relative per-part weight tolerance used by the GPU tests.
@@ -85,6 +97,17 @@ def lib():
         L.oracle_rcb.restype = C.c_int
         L.oracle_rcb_par.argtypes = sig + [C.c_int]
         L.oracle_rcb_par.restype = C.c_int
+        L.oracle_rib.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+                                 C.c_size_t, C.c_int, C.c_size_t, C.c_double, C.c_void_p, C.c_void_p]
+        L.oracle_rib.restype = C.c_int
+        L.oracle_rib_frame.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t] + [C.c_void_p] * 5
+        L.oracle_rib_frame.restype = C.c_int
+        L.oracle_inertia_vector.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+        L.oracle_inertia_vector.restype = C.c_double
+        L.oracle_householder.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+        L.oracle_householder.restype = None
+        L.oracle_map_points.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
+        L.oracle_map_points.restype = None
         L.oracle_bbox.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
         L.oracle_bbox.restype = C.c_int
         L.oracle_split_1d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_float,
@@ -244,3 +267,57 @@ def max_imbalance(num_parts, partition, weights):
     if loads.size == 0:
         return loads.dtype.type(0)
     return loads.max() - loads.min()
+
+
+def rib_frame(points):
+    """Rotation of ``Rib`` for these points: dict with centroid, inertia,
+    axis, aabb_to_obb (Householder reflection) and obb_to_aabb (its inverse),
+    or None for no points."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    n, dim = pts.shape
+    c, m, a, h, inv = (np.zeros(dim), np.zeros(dim * dim), np.zeros(dim), np.zeros(dim * dim), np.zeros(dim * dim))
+    rc = lib().oracle_rib_frame(pts.ctypes.data, n, dim, c.ctypes.data, m.ctypes.data, a.ctypes.data,
+                                h.ctypes.data, inv.ctypes.data)
+    if rc == 0:
+        return None
+    return {"centroid": c, "inertia": m.reshape(dim, dim), "axis": a, "aabb_to_obb": h.reshape(dim, dim),
+            "obb_to_aabb": inv.reshape(dim, dim), "singular": rc < 0}
+
+
+def inertia_vector(matrix):
+    m = np.ascontiguousarray(matrix, dtype=np.float64)
+    dim = m.shape[0]
+    v = np.zeros(dim)
+    lam = lib().oracle_inertia_vector(m.ctypes.data, dim, v.ctypes.data)
+    return v, lam
+
+
+def householder(axis):
+    e = np.ascontiguousarray(axis, dtype=np.float64)
+    h = np.zeros(e.size * e.size)
+    lib().oracle_householder(e.ctypes.data, e.size, h.ctypes.data)
+    return h.reshape(e.size, e.size)
+
+
+def map_points(points, obb_to_aabb):
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    m = np.ascontiguousarray(obb_to_aabb, dtype=np.float64)
+    out = np.empty_like(pts)
+    lib().oracle_map_points(pts.ctypes.data, pts.shape[0], pts.shape[1], m.ctypes.data, out.ctypes.data)
+    return out
+
+
+def rib(points, weights, iter_count, tolerance, *, obb_to_aabb=None, n_part=None):
+    """``Rib`` on the CPU: (err, ids).  With ``obb_to_aabb`` the given
+    rotation replaces the oracle's own."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    w = np.ascontiguousarray(weights)
+    kind = _kind_of(w)
+    n = pts.shape[0]
+    n_part = n if n_part is None else n_part
+    ids = np.zeros(max(n_part, 1), dtype=np.uint64)
+    dim = pts.shape[1] if pts.ndim == 2 else 0
+    m = None if obb_to_aabb is None else np.ascontiguousarray(obb_to_aabb, dtype=np.float64)
+    err = lib().oracle_rib(ids.ctypes.data, n_part, dim, pts.ctypes.data, n, kind, w.ctypes.data, w.shape[0], 0,
+                           int(iter_count), float(tolerance), None, None if m is None else m.ctypes.data)
+    return err, ids[:n_part]
