@@ -244,9 +244,12 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // scan.  COUPE_HIP_PARTITION=lookback selects the decoupled look-back.
     const bool lookback = cx.lookback;
     cx.tile_left.ensure(tiles_cap * sizeof(uint64_t));
-    const bool fresh_below = cx.below.bytes < tiles_cap * sizeof(uint32_t);
     cx.below.ensure(tiles_cap * sizeof(uint32_t));
-    if (fresh_below || !cx.bins_clean) check(hipMemsetAsync(cx.below.ptr, 0, cx.below.bytes, st));
+    // The per-tile counts start at zero: each partition's fused pass adds to
+    // the entries of the next level's tiles and k_tile_left clears what it
+    // reads, but the last level's counts are never read (no partition), so
+    // a call cannot rely on the previous one leaving them clear.
+    if (!lookback) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
     if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(L, 1u << 20) - 2) {

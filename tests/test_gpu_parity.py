@@ -114,6 +114,27 @@ def test_random_medium_cases(oracle_mod, seed):
                   f"medium seed {seed}")
 
 
+def test_consecutive_calls_on_one_context(oracle_mod):
+    """A call must not see per-tile state left by the previous, deeper one."""
+    torch = pytest.importorskip("torch")
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(77)
+        runs = [(rng.random((400_000, 3)), 12, 0.01), (rng.random((221_239, 3)), 9, 0.05),
+                (rng.random((150_000, 2)), 11, 0.0), (rng.random((90_000, 3)), 5, 0.05)]
+        for q, (pts, iters, tol) in enumerate(runs):
+            w = rng.integers(1, 1000, pts.shape[0]).astype(np.int32)
+            say(f"consecutive call {q}: n={pts.shape[0]} L={iters}")
+            out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"call {q}"
+    finally:
+        ctx.close()
+
+
 def test_edge_cases(oracle_mod):
     rng = np.random.default_rng(7)
     one = rng.random((1, 3))
