@@ -59,6 +59,24 @@ def algo_bytes_per_point(dim: int, levels: int) -> int:
     return (12 * dim + 20) + 12 * levels + (levels - 1) * (8 * dim + 24) + 12
 
 
+TRAFFIC_PROFILES = {("jgrid512", "partition"): "profiles/rcb_jgrid512_hbm_traffic.md"}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by a 1 GiB copy), or
+    None when no such profile exists for this workload and kernel."""
+    path = TRAFFIC_PROFILES.get((workload, kernel))
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None
+    text = open(os.path.join(ROOT, path)).read()
+    block = text.split("```json", 1)[-1].split("```", 1)[0] if "```json" in text else ""
+    try:
+        return int(json.loads(block)["traffic_bytes"])
+    except (ValueError, KeyError):
+        return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -187,7 +205,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(args.workload, dominant),
         "avg_launch_ms": round(dk["avg_launch_ms"], 4),
         "algo_bytes_per_launch": dk["algo_bytes_per_launch"],
     }

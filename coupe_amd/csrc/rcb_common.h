@@ -51,7 +51,7 @@ struct Part {
     uint32_t inst;         // instance of a batch (0 for a single call)
     uint32_t depth;        // level of this part in its instance's tree
     uint32_t levels;       // iter_count of its instance
-    uint32_t pad;
+    uint32_t counts_ready;      // per-tile left counts of this part's split already gathered
     double tol;            // tolerance of its instance
 };
 

@@ -474,7 +474,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
             if (cnt == prev || hi <= t + d || imb <= tol) {   // :552-563
                 P->status = kSplitAtT; P->split = t; P->wl = Ops::to_bits(wl);
                 P->cnt_left = cnt; P->kstar = q;
-                P->pad = counts_ready;   // per-tile left counts already known
+                P->counts_ready = counts_ready;   // per-tile left counts already known
                 return;
             }
             prev = cnt;                                   // :564
@@ -647,7 +647,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             C.inst = P.inst;
             C.depth = P.depth + 1;
             C.levels = P.levels;
-            C.pad = 0;
+            C.counts_ready = 0;
             C.tol = P.tol;
             next[co] = C;
         }
@@ -1046,7 +1046,7 @@ k_tile_left(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
             left = 0;                          // not partitioned
         } else if (P.status == kAllLeft) {
             left = T.count;
-        } else if (P.pad) {
+        } else if (P.counts_ready) {
             left = below[t];                   // counted by the previous partition
         } else {
             const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
