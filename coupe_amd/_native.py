@@ -79,6 +79,7 @@ class Stats(C.Structure):
         ("weight_mode", C.c_uint32),
         ("weight_bytes", C.c_uint32),
         ("bins", C.c_uint32),
+        ("reruns", C.c_uint32),
         ("unit_points", C.c_uint64),
         ("device_bytes", C.c_uint64),
     ]

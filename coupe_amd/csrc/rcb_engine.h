@@ -61,6 +61,7 @@ struct Context {
     DevBuf instances;                // instance table of the current call
     DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
     DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
+    int search_passes = 3;           // histogram passes queued per level without a host check
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf rib_moments;              // Rib: block partials, centroid, inertia
@@ -85,6 +86,15 @@ enum coupe_err map_hip_error(hipError_t e);
 inline bool lookback_from_env() {
     const char *v = getenv("COUPE_HIP_PARTITION");
     return v && v[0] == 'l';
+}
+
+// COUPE_HIP_SEARCH_PASSES=k: histogram passes queued per level before the
+// host looks (default 3; 0 = the host checks after every pass).
+inline int search_passes_from_env() {
+    const char *v = getenv("COUPE_HIP_SEARCH_PASSES");
+    if (!v || !*v) return 3;
+    const int k = atoi(v);
+    return k < 0 ? 0 : (k > 16 ? 16 : k);
 }
 
 // RCB of a batch of independent instances packed in input order (see

@@ -96,7 +96,9 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 }
 
 // Small device counters shared by the kernels of one call.
+constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
 constexpr int kMaxTickets = 4096;
+constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
 struct Counters {
     uint32_t n_parts[2];
     uint32_t n_tiles[2];
@@ -107,6 +109,8 @@ struct Counters {
     uint64_t min_id;
     unsigned long long swept;                   // points swept by standalone passes
     unsigned long long swept_fused;             // points binned inside partitions
+    uint32_t unfinished;                        // a speculative search did not end in its passes
+    uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
     unsigned long long ticket[kMaxTickets];     // partition ticket counter, one per level
 };
 
@@ -203,8 +207,15 @@ static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const 
     return b;
 }
 
+// Runs the levels of a batch.  With spec_passes > 0 the split search of a
+// level is queued as its first decision plus spec_passes further
+// histogram+decision passes, each skipped on the device when no part is left
+// searching, so the host never waits inside the level loop; returns true when
+// some part needed more passes (the result is then invalid and the caller
+// runs the batch again with spec_passes = 0, which checks every pass).
 template <int D, typename WS, typename Acc>
-static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap32, uint64_t *d_out, WS *w_first) {
+static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap32, uint64_t *d_out, WS *w_first,
+                       int spec_passes) {
     Counters *ctr = (Counters *)cx.counters.ptr;
     uint32_t *h_search = (uint32_t *)cx.host_scratch;
     Instance *insts = (Instance *)cx.instances.ptr;
@@ -213,6 +224,7 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     const size_t L = B.max_levels;
     const uint64_t parts_cap = B.parts_cap, tiles_cap = B.tiles_cap;
     if (tiles_cap > 0xffffffffull || parts_cap > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
+    if ((uint64_t)L * (uint64_t)(spec_passes + 1) > (uint64_t)kSpecSlots) spec_passes = 0;
 
     for (int b = 0; b < 2; b++) {
         cx.parts[b].ensure(parts_cap * sizeof(Part));
@@ -312,17 +324,44 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
         uint32_t *ntl = &ctr->n_tiles[sl];
+        const int nb = choose_bins(parts_ub);
+        (void)n_parts_host;
+        if (spec_passes > 0) {
+            // Queued search: slot p counts the parts still searching after
+            // decision p; pass p + 1 runs only if it is non-zero.
+            uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
+            timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                    parts, np, bcnt, bw, bmin, first_standalone ? kFirstBins : 2, wrap32, &slot[0],
+                    first_standalone ? &ctr->swept : &ctr->swept_fused, (first_standalone || lookback) ? 0u : 1u,
+                    nullptr);
+            });
+            const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
+            for (int p = 1; p <= spec_passes; p++) {
+                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
+                    k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
+                                                         bmin, &slot[p - 1]);
+                });
+                timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+                    k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
+                                                                   &slot[p], &ctr->swept, 0u, &slot[p - 1]);
+                });
+            }
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_force_finish<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &slot[spec_passes],
+                                                                            &ctr->unfinished);
+            });
+            return;
+        }
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
             k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin,
                                                            first_standalone ? kFirstBins : 2, wrap32,
                                                            &ctr->searching,
                                                            first_standalone ? &ctr->swept : &ctr->swept_fused,
-                                                           (first_standalone || lookback) ? 0u : 1u);
+                                                           (first_standalone || lookback) ? 0u : 1u, nullptr);
         });
         uint32_t passes = 1;
-        const int nb = choose_bins(parts_ub);
-        (void)n_parts_host;
         for (;;) {
             check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             check(hipStreamSynchronize(st));
@@ -334,11 +373,11 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             cx.stats.bins = nb;
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
                 k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
-                                                     bmin);
+                                                     bmin, nullptr);
             });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept, 0u);
+                                                               &ctr->searching, &ctr->swept, 0u, nullptr);
             });
             passes++;
         }
@@ -452,14 +491,29 @@ static void run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     }
 
     // Standalone-pass bytes: swept points times (axis coordinate + stored weight).
-    unsigned long long *h_swept = (unsigned long long *)((char *)cx.host_scratch + 64);
-    uint32_t *h_error = (uint32_t *)((char *)cx.host_scratch + 128);
-    check(hipMemcpyAsync(h_swept, &ctr->swept, sizeof(*h_swept), hipMemcpyDeviceToHost, st));
-    check(hipMemcpyAsync(h_error, &ctr->error, sizeof(*h_error), hipMemcpyDeviceToHost, st));
+    // One readback for the whole call: the counters from `swept` to the end
+    // of the speculative slots.
+    const size_t rb_off = offsetof(Counters, swept);
+    const size_t rb_len = offsetof(Counters, ticket) - rb_off;
+    static_assert(offsetof(Counters, ticket) - offsetof(Counters, swept) <= kHostScratch - 256, "readback");
+    char *h_rb = (char *)cx.host_scratch + 256;
+    check(hipMemcpyAsync(h_rb, (char *)ctr + rb_off, rb_len, hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
-    if (*h_error) throw HipFailure{hipErrorUnknown};
-    cx.swept_bytes = (double)*h_swept * (4.0 + sizeof(WS));
+    const Counters *h = (const Counters *)(h_rb - rb_off);   // only the copied fields are read
+    if (h->error) throw HipFailure{hipErrorUnknown};
+    cx.swept_bytes = (double)h->swept * (4.0 + sizeof(WS));
     cx.bins_clean = true;
+    if (spec_passes > 0) {
+        // passes per level: the first decision plus every pass whose gate was open
+        for (size_t l = 0; l < L; l++) {
+            uint32_t passes = 1;
+            for (int p = 1; p <= spec_passes; p++)
+                if (h->spec[l * (spec_passes + 1) + p - 1]) passes++;
+            cx.stats.passes += passes;
+            cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+        }
+    }
+    return h->unfinished != 0;
 }
 
 template <int D, typename WIN>
@@ -500,8 +554,8 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
 }
 
 template <int D, typename WIN>
-static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts,
-                      const WIN *win, int wkind) {
+static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts,
+                      const WIN *win, int wkind, int spec_passes) {
     const uint64_t n = B.n;
     // Coordinate slots: 0 (prologue output, input order), 1 and 2 (levels).
     for (int b = 0; b < 3; b++)
@@ -527,7 +581,7 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         cx.stats.weight_mode = 0;
         cx.stats.weight_bytes = 4;
         cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
-        run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr);
+        return run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr, spec_passes);
     } else if (integral) {
         cx.stats.weight_mode = 0;
         cx.stats.weight_bytes = 8;
@@ -536,7 +590,7 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
         });
-        run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr);
+        return run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr, spec_passes);
     } else {
         cx.stats.weight_mode = 1;
         cx.stats.weight_bytes = 8;
@@ -545,18 +599,18 @@ static void run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_store_weights<WIN, double><<<grid, kBlock, 0, st>>>(win, (double *)cx.w[0].ptr, n);
         });
-        run_levels<D, double, double>(cx, st, B, 0, d_out, (double *)cx.w[0].ptr);
+        return run_levels<D, double, double>(cx, st, B, 0, d_out, (double *)cx.w[0].ptr, spec_passes);
     }
-    check(hipStreamSynchronize(st));
 }
 
+// Returns true when a speculative search did not end (see run_levels).
 template <int D>
-static void run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts, int wkind,
-                    const void *w) {
+static bool run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPlan &B, const double *pts, int wkind,
+                    const void *w, int spec_passes) {
     switch (wkind) {
-        case COUPE_INT: run_typed<D, int32_t>(cx, st, d_out, B, pts, (const int32_t *)w, wkind); break;
-        case COUPE_INT64: run_typed<D, int64_t>(cx, st, d_out, B, pts, (const int64_t *)w, wkind); break;
-        default: run_typed<D, double>(cx, st, d_out, B, pts, (const double *)w, wkind); break;
+        case COUPE_INT: return run_typed<D, int32_t>(cx, st, d_out, B, pts, (const int32_t *)w, wkind, spec_passes);
+        case COUPE_INT64: return run_typed<D, int64_t>(cx, st, d_out, B, pts, (const int64_t *)w, wkind, spec_passes);
+        default: return run_typed<D, double>(cx, st, d_out, B, pts, (const double *)w, wkind, spec_passes);
     }
 }
 
@@ -580,13 +634,23 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         if (B.n > 0xffffffffull) return COUPE_ERR_ALLOC;
         if (B.inst.size() > 1 && !B.dense) return COUPE_ERR_BAD_TYPE;
         check(hipSetDevice(cx.device));
-        if (!cx.host_scratch) check(hipHostMalloc(&cx.host_scratch, 4096, hipHostMallocDefault));
+        if (!cx.host_scratch) check(hipHostMalloc(&cx.host_scratch, kHostScratch, hipHostMallocDefault));
         if (cx.cu_count <= 0) {
             check(hipDeviceGetAttribute(&cx.cu_count, hipDeviceAttributeMultiprocessorCount, cx.device));
             if (cx.cu_count <= 0) cx.cu_count = 1;
         }
-        if (dim == 2) run_dim<2>(cx, st, d_out, B, d_points, wkind, d_weights);
-        else run_dim<3>(cx, st, d_out, B, d_points, wkind, d_weights);
+        auto run = [&](int spec) {
+            return dim == 2 ? run_dim<2>(cx, st, d_out, B, d_points, wkind, d_weights, spec)
+                            : run_dim<3>(cx, st, d_out, B, d_points, wkind, d_weights, spec);
+        };
+        if (run(cx.search_passes)) {
+            // a split search needed more passes than were queued: again,
+            // checking every pass on the host
+            cx.stats = coupe_hip_stats{};
+            cx.stats.points = B.n;
+            cx.stats.reruns = 1;
+            run(0);
+        }
         cx.prof.collect();
         cx.prof.out[COUPE_HIP_K_HIST].algo_bytes = cx.swept_bytes;
     } catch (const HipFailure &f) {

@@ -330,7 +330,8 @@ __global__ void __launch_bounds__(kBlock)
 k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-       uint32_t *__restrict__ bmin) {
+       uint32_t *__restrict__ bmin, const uint32_t *gate) {
+    if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
     __shared__ Acc sw[kMaxBins];
@@ -377,7 +378,9 @@ template <typename Acc>
 __global__ void __launch_bounds__(64)
 k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
          Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
-         uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready) {
+         uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
+         const uint32_t *gate) {
+    if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
     __shared__ uint32_t sm[kMaxBins + 1];
@@ -487,6 +490,25 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     }
     P->blo = lo; P->bhi = hi; P->prev = prev;             // next pass continues here
     atomicAdd(n_searching, 1u);
+}
+
+// End of a speculative split search: a part still searching after the last
+// queued pass is closed as "all left" (a consistent tree for the kernels that
+// follow) and the call is flagged, so the host runs it again with a search
+// that checks every pass.
+__global__ void __launch_bounds__(kBlock)
+k_force_finish(Part *__restrict__ parts, const uint32_t *n_parts, const uint32_t *gate, uint32_t *unfinished) {
+    if (*gate == 0) return;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= *n_parts) return;
+    Part *P = &parts[s];
+    if (P->status != kSearching) return;
+    P->status = kAllLeft;
+    P->split = P->bhi;
+    P->cnt_left = P->count;
+    P->wl = P->sum;
+    P->kstar = -1;
+    atomicOr(unfinished, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1142,11 +1164,13 @@ k_partition_tiles(const Part *__restrict__ parts, const Part *__restrict__ next_
             if (lane == 0) wtot[k][wid] = __popcll(bal);
         }
         __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < kItems; k++)
-                for (int v = 0; v < kBlock / 64; v++) { uint32_t c = wtot[k][v]; wtot[k][v] = run; run += c; }
-            s_tile_left = run;
+        if (tid < 64) {   // exclusive scan of the kItems x waves counts, item-major
+            static_assert(kItems * (kBlock / 64) <= 64, "one wavefront scans the counts");
+            uint32_t *flat = &wtot[0][0];
+            const uint32_t c = tid < kItems * (kBlock / 64) ? flat[tid] : 0u;
+            const uint32_t inc = wave_incl_add(c);
+            if (tid < kItems * (kBlock / 64)) flat[tid] = inc - c;
+            if (tid == kItems * (kBlock / 64) - 1) s_tile_left = inc;
         }
         __syncthreads();
         const uint32_t tl = s_tile_left;
@@ -1326,7 +1350,7 @@ k_descend(const Node *__restrict__ nodes, uint64_t n_nodes, uint32_t levels, uin
 // the children of h are 2h+1 and 2h+2, and the leaf's heap index minus the
 // offset is the part id.  The top kDenseCached levels sit in LDS.
 constexpr int kDenseLevels = 24;
-constexpr int kDenseCached = 13;   // 8191 splits, 32 KB of LDS
+constexpr int kDenseCached = 12;   // 4095 splits, 16 KB of LDS
 
 template <int D>
 __global__ void __launch_bounds__(kBlock)
@@ -1350,22 +1374,40 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
             for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = heap_split[hb + j];
             __syncthreads();
         }
-        const uint64_t end = T.begin + T.count;
-        for (uint64_t i = T.begin + threadIdx.x; i < end; i += kBlock) {
-            const float x0 = c0[i], x1 = c1[i], x2 = D == 3 ? c2[i] : 0.f;
-            uint32_t h = 0;
-            int a = 0;
-            for (uint32_t l = 0; l < cached_levels; l++) {
-                const float t = cache[h];
-                h = 2 * h + (sel3(a, x0, x1, x2) >= t ? 2u : 1u);
-                a = a + 1 == D ? 0 : a + 1;
+        // kItems points per thread walk the heap side by side: their loads
+        // are issued together and their lookup chains interleave.
+        float x0[kItems], x1[kItems], x2[kItems];
+        uint32_t h[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t p = k * kBlock + threadIdx.x;
+            const bool valid = p < T.count;
+            x0[k] = valid ? c0[T.begin + p] : 0.f;
+            x1[k] = valid ? c1[T.begin + p] : 0.f;
+            x2[k] = (D == 3 && valid) ? c2[T.begin + p] : 0.f;
+            h[k] = 0;
+        }
+        int a = 0;
+        for (uint32_t l = 0; l < cached_levels; l++) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const float t = cache[h[k]];
+                h[k] = 2 * h[k] + (sel3(a, x0[k], x1[k], x2[k]) >= t ? 2u : 1u);
             }
-            for (uint32_t l = cached_levels; l < levels; l++) {
-                const float t = heap_split[hb + h];
-                h = 2 * h + (sel3(a, x0, x1, x2) >= t ? 2u : 1u);
-                a = a + 1 == D ? 0 : a + 1;
+            a = a + 1 == D ? 0 : a + 1;
+        }
+        for (uint32_t l = cached_levels; l < levels; l++) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const float t = heap_split[hb + h[k]];
+                h[k] = 2 * h[k] + (sel3(a, x0[k], x1[k], x2[k]) >= t ? 2u : 1u);
             }
-            out[i] = (uint64_t)h - off;
+            a = a + 1 == D ? 0 : a + 1;
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t p = k * kBlock + threadIdx.x;
+            if (p < T.count) out[T.begin + p] = (uint64_t)h[k] - off;
         }
     }
 }

@@ -135,6 +135,42 @@ def test_consecutive_calls_on_one_context(oracle_mod):
         ctx.close()
 
 
+@pytest.mark.parametrize("queued", [0, 1, 3])
+def test_split_search_queue_depths(oracle_mod, monkeypatch, queued):
+    """Split searches queued without host checks (COUPE_HIP_SEARCH_PASSES):
+    a search that needs more passes than were queued makes the call run
+    again with a host check after every pass; ids never change."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_SEARCH_PASSES", str(queued))
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(500 + queued)
+        reruns = 0
+        for q in range(6):
+            n = int(rng.integers(20_000, 200_000))
+            dim = 2 + q % 2
+            pts = rng.normal(0, 1, (n, dim)) ** 3 if q % 2 else rng.random((n, dim))
+            w = rng.integers(1, 1000, n).astype(np.int64)
+            iters, tol = int(rng.integers(4, 12)), [0.0, 0.001][q % 2]
+            say(f"queued {queued}, case {q}: n={n} d={dim} L={iters} tol={tol}")
+            out = torch.zeros(n, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"queued {queued} case {q}"
+            st = ctx.stats()
+            reruns += st["reruns"]
+            if queued == 0:
+                assert st["reruns"] == 0
+            elif st["reruns"] == 0:
+                assert st["max_passes"] <= queued + 1
+        if queued == 1:
+            assert reruns > 0, "tol 0 searches should outrun one queued pass"
+    finally:
+        ctx.close()
+
+
 def test_edge_cases(oracle_mod):
     rng = np.random.default_rng(7)
     one = rng.random((1, 3))
