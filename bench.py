@@ -23,6 +23,14 @@ Two more measurements ride in the same line (skip them with --plain-only):
              its weights were resampled.  p50/p99 latency (submit -> ids in
              the caller's buffer) and aggregate Mpts/s.
 
+Also in the line: "stream" (a STREAM-style probe of the device with the
+partition's shape: 4 f32 arrays read and 4 written; the practical HBM
+ceiling next to the 8 TB/s data-sheet peak), "end_to_end_host" (coupe_rcb
+from host memory, PCIe copies included), "configs" (the 2^20-point 2D and
+2^24-point Gaussian 3D workloads, device-resident), and the CPU baseline as
+the median of 3 runs on all host threads plus a 1-thread run on a 2^24-point
+sample of the same generator.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                        [--no-cpu] [--cpu-threads T] [--plain-only]
                        [--serve-requests R]
@@ -92,7 +100,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched and served measurements")
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the partitioner runs on a "
@@ -104,6 +112,7 @@ def main():
     import coupe_amd
     from coupe_amd import workloads
 
+    stream_probe = probe_stream()
     t0 = time.time()
     pts, w, iters, tol = workloads.standard(args.workload)
     n, dim = pts.shape
@@ -198,6 +207,7 @@ def main():
     dominant = max((k for k in kernels if kernels[k]["algo_bytes_per_launch"] > 0),
                    key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
+    stream_copy = stream_probe["copy_GBps"] if stream_probe else None
     roofline = {
         "kernel": dominant,
         "bound": "hbm",
@@ -208,6 +218,8 @@ def main():
         "traffic": measured_traffic(args.workload, dominant),
         "avg_launch_ms": round(dk["avg_launch_ms"], 4),
         "algo_bytes_per_launch": dk["algo_bytes_per_launch"],
+        "stream_copy_GBps": stream_copy,
+        "frac_of_stream_copy": round(dk["GBps"] / stream_copy, 4) if stream_copy else None,
     }
     value = n / (ms_per_step * 1e-3) / 1e6
     bpp = algo_bytes_per_point(dim, iters)
@@ -225,14 +237,20 @@ def main():
         from oracle import oracle
 
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu oracle on {threads} threads ...")
-        t1 = time.perf_counter()
-        err, cpu_ids, cst = oracle.rcb(pts, w, iters, tol, parallel=True, nthreads=threads)
-        cpu_s = time.perf_counter() - t1
-        assert err == 0
+        runs = []
+        for r in range(3):
+            log(f"cpu oracle on {threads} threads, run {r + 1}/3 ...")
+            t1 = time.perf_counter()
+            err, cpu_ids, cst = oracle.rcb(pts, w, iters, tol, parallel=True, nthreads=threads)
+            runs.append(time.perf_counter() - t1)
+            assert err == 0
+        cpu_s = sorted(runs)[1]
+        one = cpu_one_thread(oracle, workloads, args.workload)
         cpu = {"value": round(n / cpu_s / 1e6, 2), "unit": "Mpts/s", "cores": threads, "kind": "port",
-               "sample": f"the full workload ({n} points), one run, fork-join C restatement of coupe::Rcb "
-                         f"(OpenMP tasks for rayon::join, chunked parallel split passes), {cpu_s:.2f} s"}
+               "sample": f"the full workload ({n} points), median of 3 runs "
+                         f"({', '.join(f'{x:.2f}' for x in runs)} s), fork-join C restatement of coupe::Rcb "
+                         f"(OpenMP tasks for rayon::join, chunked parallel split passes)",
+               "one_thread": one, "host": host_info()}
         same = bool(np.array_equal(gpu_ids, cpu_ids))
         parity = {"ids_equal_to_cpu_oracle": same,
                   "differing_ids": int(np.count_nonzero(gpu_ids != cpu_ids)),
@@ -261,18 +279,126 @@ def main():
         "parity": parity,
         "whole_job": whole_job,
         "copy_probe": copy_probe,
+        "stream": stream_probe,
         "step_ms": {"median": round(step_ms[len(step_ms) // 2], 4), "min": round(step_ms[0], 4),
                     "max": round(step_ms[-1], 4), "profiled_median": round(sorted(prof_ms)[len(prof_ms) // 2], 4)},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
+    out["end_to_end_host"] = end_to_end_host(coupe_amd, pts, w, iters, tol, gpu_ids)
     if not args.plain_only:
         del d_pts, d_w, d_out
+        torch.cuda.empty_cache()
+        out["configs"] = bench_configs(torch, coupe_amd, workloads)
         torch.cuda.empty_cache()
         out["batched"] = bench_batched(torch, coupe_amd, workloads)
         torch.cuda.empty_cache()
         out["served"] = bench_served(torch, coupe_amd, workloads, args.serve_requests)
     print(json.dumps(out), flush=True)
+
+
+def probe_stream():
+    """STREAM-style bandwidth of the device with the partition's shape
+    (libcoupe_probe.so): 4 x 2^27 f32 copied, read, written."""
+    import ctypes as C
+    path = os.path.join(ROOT, "coupe_amd", "libcoupe_probe.so")
+    if not os.path.exists(path):
+        log("stream probe library missing; build with `python -m coupe_amd.build probe`")
+        return None
+    import torch  # noqa: F401  (one HIP runtime in the process)
+    L = C.CDLL(path)
+    L.coupe_probe_stream.argtypes = [C.c_int, C.c_uint64, C.c_int] + [C.POINTER(C.c_double)] * 3
+    L.coupe_probe_stream.restype = C.c_int
+    cp, rd, wr = C.c_double(), C.c_double(), C.c_double()
+    err = L.coupe_probe_stream(0, 1 << 27, 10, C.byref(cp), C.byref(rd), C.byref(wr))
+    if err != 0:
+        log(f"stream probe failed: HIP error {err}")
+        return None
+    res = {"copy_GBps": round(cp.value, 1), "read_GBps": round(rd.value, 1), "write_GBps": round(wr.value, 1),
+           "shape": "4 arrays of 2^27 f32 read and 4 written (copy), read only, written only; "
+                    "best of 4 and 8 workgroups per CU, 10 launches each"}
+    log(f"stream probe: copy {res['copy_GBps']} GB/s, read {res['read_GBps']}, write {res['write_GBps']}")
+    return res
+
+
+def host_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_one_thread(oracle, workloads, workload):
+    """The sequential oracle on a 2^24-point sample of the same generator
+    (jgrid: the 256^3 grid; other workloads: the 2^24 first points)."""
+    if workload == "jgrid512":
+        p, w = workloads.jgrid(256, 2, True)
+        iters, tol = 12, 0.01
+    else:
+        p, w, iters, tol = workloads.standard(workload)
+        p, w = p[: 1 << 24], w[: 1 << 24]
+    t1 = time.perf_counter()
+    err, _, _ = oracle.rcb(p, w, iters, tol)
+    s = time.perf_counter() - t1
+    assert err == 0
+    log(f"cpu oracle, 1 thread, {p.shape[0]} points: {s:.2f} s")
+    return {"value": round(p.shape[0] / s / 1e6, 2), "unit": "Mpts/s", "cores": 1,
+            "sample": f"{p.shape[0]} points of the same generator, iter_count {iters}, one run, {s:.2f} s"}
+
+
+def end_to_end_host(coupe_amd, pts, w, iters, tol, gpu_ids):
+    """coupe_rcb from host memory: H2D of points and weights, the kernels, D2H of the ids."""
+    part = np.zeros(pts.shape[0], dtype=np.uint64)
+    rcb = coupe_amd.Rcb(iter_count=iters, tolerance=tol)
+    rcb.partition(part, (pts, w))   # warm-up (staging buffers)
+    times = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        rcb.partition(part, (pts, w))
+        times.append(time.perf_counter() - t1)
+    s = sorted(times)[1]
+    same = bool(np.array_equal(part, gpu_ids))
+    log(f"end to end from host memory: {s * 1e3:.1f} ms, ids equal {same}")
+    return {"ms": round(s * 1e3, 2), "Mpts_s": round(pts.shape[0] / s / 1e6, 1),
+            "note": "coupe_rcb with host (pageable) buffers: PCIe copies of points, weights and ids included; "
+                    "median of 3", "ids_equal_to_device_entry": same}
+
+
+def bench_configs(torch, coupe_amd, workloads, steps=10, warmup=3):
+    """Device-resident throughput of the 2^20-point 2D and 2^24-point Gaussian 3D workloads."""
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for name in ("uniform2d", "gauss3d"):
+        pts, w, iters, tol = workloads.standard(name)
+        d_pts, d_w = torch.from_numpy(pts).to(dev), torch.from_numpy(w).to(dev)
+        d_out = torch.zeros(pts.shape[0], dtype=torch.int64, device=dev)
+        ctx = coupe_amd.DeviceContext(0)
+        rcb = coupe_amd.Rcb(iter_count=iters, tolerance=tol)
+        for _ in range(warmup):
+            rcb.partition(d_out, (d_pts, d_w), context=ctx, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        ev = []
+        for _ in range(steps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            rcb.partition(d_out, (d_pts, d_w), context=ctx, stream=stream.cuda_stream)
+            e1.record(stream)
+            ev.append((e0, e1))
+        torch.cuda.synchronize()
+        ms = sorted(a.elapsed_time(b) for a, b in ev)[steps // 2]
+        st = ctx.stats()
+        ctx.close()
+        res[name] = {"workload": WORKLOADS[name], "points": int(pts.shape[0]), "median_ms": round(ms, 4),
+                     "Mpts_s": round(pts.shape[0] / (ms * 1e-3) / 1e6, 1), "passes": st["passes"]}
+        log(f"config {name}: {ms:.3f} ms, {res[name]['Mpts_s']} Mpts/s")
+        del d_pts, d_w, d_out
+    return res
 
 
 def bench_batched(torch, coupe_amd, workloads, count=4096, seed=3):

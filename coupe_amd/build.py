@@ -4,6 +4,8 @@
   the C ABI of include/coupe.h and include/coupe_hip.h (hipcc).
 * ``coupe_amd/libcoupe_workloads.so`` -- synthetic input generators used by
   benchmarks and tests (gcc).
+* ``coupe_amd/libcoupe_probe.so`` -- STREAM-style bandwidth probe reported by
+  the benchmark (hipcc).
 
 Run ``python -m coupe_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -20,6 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 LIB = os.path.join(HERE, "libcoupe_hip.so")
 WORKLOADS = os.path.join(HERE, "libcoupe_workloads.so")
+PROBE = os.path.join(HERE, "libcoupe_probe.so")
 
 HIP_SOURCES = ["rcb_engine.hip", "rib.hip", "imbalance.hip", "coupe_api.cpp", "serve.cpp"]
 HIP_FLAGS = [
@@ -47,6 +50,14 @@ def build_hip(verbose: bool = True) -> str:
     return LIB
 
 
+def build_probe() -> str:
+    """The bandwidth probe bench.py reports beside the roofline (measurement only)."""
+    tmp = PROBE + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "probe.hip")])
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 def build_workloads() -> str:
     src = os.path.join(HERE, "workloads.c")
     tmp = WORKLOADS + ".tmp"
@@ -57,6 +68,7 @@ def build_workloads() -> str:
 
 def build_all() -> None:
     build_hip()
+    build_probe()
     build_workloads()
 
 
@@ -64,5 +76,7 @@ if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     if "all" in what or "hip" in what:
         build_hip()
+    if "all" in what or "probe" in what:
+        build_probe()
     if "all" in what or "workloads" in what:
         build_workloads()

@@ -51,7 +51,7 @@ struct Part {
     uint32_t inst;         // instance of a batch (0 for a single call)
     uint32_t depth;        // level of this part in its instance's tree
     uint32_t levels;       // iter_count of its instance
-    uint32_t counts_ready;      // per-tile left counts of this part's split already gathered
+    uint32_t counts_ready;      // where the partition finds each tile's left count (kCounts*)
     double tol;            // tolerance of its instance
 };
 
@@ -93,6 +93,12 @@ struct Node {
 struct Rotation {
     double m[9];
 };
+
+// Part::counts_ready: each tile's count of points left of the split is
+// counted by reading the tile (kCountsRead), was counted by the partition
+// that made the part (kCountsBelow: split at the first candidate), or is the
+// sum of the tile's bins in the histogram pass that decided (kCountsBins).
+constexpr uint32_t kCountsRead = 0, kCountsBelow = 1, kCountsBins = 2;
 
 // Results of the prologue reductions over the whole batch.
 struct Prologue {

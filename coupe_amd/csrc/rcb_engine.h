@@ -64,6 +64,7 @@ struct Context {
     int search_passes = 3;           // histogram passes queued per level without a host check
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
+    DevBuf tile_bins;                // per-tile bin counts of the histogram passes
     DevBuf rib_moments;              // Rib: block partials, centroid, inertia
     coupe_hip_rib_info rib{};        // Rib: rotation of the last call
     const Rotation *rot = nullptr;   // change of basis for the running call

@@ -124,10 +124,14 @@ static void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t
                      uint64_t *total) {
     uint64_t nblocks = (n_ub + kScanTile - 1) / kScanTile;
     if (nblocks < 1) nblocks = 1;
-    cx.bsum.ensure(nblocks * sizeof(uint64_t));
+    cx.bsum.ensure((nblocks + 1) * sizeof(uint64_t));
     uint64_t *bsum = (uint64_t *)cx.bsum.ptr;
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev);
+        if (nblocks == 1) {
+            k_scan_local<<<1, kBlock, 0, st>>>(data, data, bsum, n_dev, total ? total : bsum + 1);
+            return;
+        }
+        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, nullptr);
         k_scan_top<<<1, kBlock, 0, st>>>(bsum, n_dev, total);
         k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev);
     });
@@ -257,6 +261,14 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     const bool lookback = cx.lookback;
     cx.tile_left.ensure(tiles_cap * sizeof(uint64_t));
     cx.below.ensure(tiles_cap * sizeof(uint32_t));
+    // Per-tile bin counts of the histogram passes (prefix partition): the
+    // partition's per-tile left counts of a part decided in such a pass.
+    uint16_t *tile_bins = nullptr;
+    if (!lookback) {
+        cx.tile_bins.ensure(tiles_cap * kMaxBins * sizeof(uint16_t));
+        tile_bins = (uint16_t *)cx.tile_bins.ptr;
+    }
+    const uint32_t hist_counts = lookback ? kCountsRead : kCountsBins;
     // The per-tile counts start at zero: each partition's fused pass adds to
     // the entries of the next level's tiles and k_tile_left clears what it
     // reads, but the last level's counts are never read (no partition), so
@@ -333,18 +345,18 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                     parts, np, bcnt, bw, bmin, first_standalone ? kFirstBins : 2, wrap32, &slot[0],
-                    first_standalone ? &ctr->swept : &ctr->swept_fused, (first_standalone || lookback) ? 0u : 1u,
+                    first_standalone ? &ctr->swept : &ctr->swept_fused, (first_standalone || lookback) ? kCountsRead : kCountsBelow,
                     nullptr);
             });
             const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
             for (int p = 1; p <= spec_passes; p++) {
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
                     k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
-                                                         bmin, &slot[p - 1]);
+                                                         bmin, &slot[p - 1], tile_bins);
                 });
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                     k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                                   &slot[p], &ctr->swept, 0u, &slot[p - 1]);
+                                                                   &slot[p], &ctr->swept, hist_counts, &slot[p - 1]);
                 });
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
@@ -359,7 +371,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                                                            first_standalone ? kFirstBins : 2, wrap32,
                                                            &ctr->searching,
                                                            first_standalone ? &ctr->swept : &ctr->swept_fused,
-                                                           (first_standalone || lookback) ? 0u : 1u, nullptr);
+                                                           (first_standalone || lookback) ? kCountsRead : kCountsBelow, nullptr);
         });
         uint32_t passes = 1;
         for (;;) {
@@ -373,11 +385,11 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             cx.stats.bins = nb;
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
                 k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
-                                                     bmin, nullptr);
+                                                     bmin, nullptr, tile_bins);
             });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept, 0u, nullptr);
+                                                               &ctr->searching, &ctr->swept, hist_counts, nullptr);
             });
             passes++;
         }
@@ -414,19 +426,29 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         uint32_t *ntl = &ctr->n_tiles[cur];
 
         // Children (next level's parts and tiles), split records, id offsets.
-        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-            k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
-                                                                          (uint64_t *)cx.ntiles.ptr);
-        });
-        scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
-        scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
+        const bool plan1 = parts_ub <= kPlanParts;
+        if (plan1) {
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_children_plan<<<1, kPlanThreads, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
+                                                           (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
+                                                           &ctr->tiles_total, &ctr->n_parts[nxt], &ctr->n_tiles[nxt]);
+            });
+        } else {
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                    parts, np, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
+            });
+            scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
+            scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
+        }
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                 parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                 (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
                 node_base[std::min(level + 1, L)], heap_split, insts);
-            k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
-                                          &ctr->n_tiles[nxt]);
+            if (!plan1)
+                k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
+                                              &ctr->n_tiles[nxt]);
         });
         if (level + 1 == L) break;   // every part left is at its last level
 
@@ -445,10 +467,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                     ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
             });
         } else {
-            const unsigned gt = persistent_grid(cx, k_tile_left, tiles_ub);
+            const unsigned gt = persistent_grid(cx, k_tile_left, (tiles_ub + kBlock / 64 - 1) / (kBlock / 64));
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_tile_left<<<gt, kBlock, 0, st>>>(parts, tiles, ntl, ci[0], ci[1], ci[2], (uint32_t *)cx.below.ptr,
-                                                  (uint64_t *)cx.tile_left.ptr);
+                                                  tile_bins, (uint64_t *)cx.tile_left.ptr);
             });
             scan_u64(cx, st, (uint64_t *)cx.tile_left.ptr, ntl, tiles_ub, nullptr);
             const unsigned g = persistent_grid(cx, k_partition_tiles<D, WS, Acc>, tiles_ub);
@@ -679,7 +701,7 @@ uint64_t Context::device_bytes() const {
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -688,7 +710,7 @@ void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

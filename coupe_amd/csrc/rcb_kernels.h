@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(kBlock)
 k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-       uint32_t *__restrict__ bmin, const uint32_t *gate) {
+       uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins) {
     if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
@@ -340,10 +340,12 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
     const uint32_t nt = *n_tiles;
     uint32_t cur = 0xffffffffu;
     bool active = false;
+    uint32_t before = 0;   // this thread's bin count of the part before the tile (tile_bins)
     const float *c = c0;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
+            before = 0;
             __syncthreads();
             if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin);
             __syncthreads();
@@ -365,6 +367,15 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
             atomicAdd(&scnt[b], 1u);
             atomic_add_acc(&sw[b], AccOps<Acc>::of(w[i]));
             atomicMin(&smin[b], f32_key(x));
+        }
+        if (tile_bins) {
+            // each tile's own bin counts (the growth of the part's counts
+            // over the tile) are kept for the partition's offsets
+            __syncthreads();
+            const uint32_t after = tid < nb ? scnt[tid] : 0u;
+            if (tid < nb) tile_bins[(uint64_t)t * kMaxBins + tid] = (uint16_t)(after - before);
+            before = after;
+            __syncthreads();
         }
     }
     __syncthreads();
@@ -534,14 +545,19 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t *wsum, uint64_t 
     return r;
 }
 
+// With `single` (a launch of one block covering every element) the block's
+// total is the scan's total and no further kernel is needed.
 __global__ void __launch_bounds__(kBlock)
 k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-             uint64_t *__restrict__ bsum, const uint32_t *n_dev) {
+             uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t *single) {
     __shared__ uint64_t wsum[kBlock / 64];
     __shared__ uint64_t tot;
     const uint64_t n = *n_dev;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    if ((uint64_t)blockIdx.x * kScanTile >= n) return;
+    if ((uint64_t)blockIdx.x * kScanTile >= n) {
+        if (single && threadIdx.x == 0) *single = 0;
+        return;
+    }
     uint64_t v[kScanItems];
     uint64_t s = 0;
 #pragma unroll
@@ -555,7 +571,10 @@ k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
         if (base + k < n) out[base + k] = e;
         e += v[k];
     }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = tot;
+        if (single) *single = tot;
+    }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -608,6 +627,64 @@ k_children_count(const Part *__restrict__ parts, const uint32_t *n_parts, uint64
     const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;
     nchild[s] = (nl > 0) + (nr > 0);
     ntiles[s] = tiles_for(nl) + tiles_for(nr);
+}
+
+// Children counts, their exclusive scans and the next level's table sizes in
+// one workgroup (k_children_count + two scans + k_set_sizes), for up to
+// kPlanParts parts.
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanItems = 8;
+constexpr uint32_t kPlanParts = kPlanThreads * kPlanItems;
+
+__global__ void __launch_bounds__(kPlanThreads)
+k_children_plan(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *__restrict__ child_off,
+                uint64_t *__restrict__ tile_off, uint64_t *parts_total, uint64_t *tiles_total,
+                uint32_t *next_n_parts, uint32_t *next_n_tiles) {
+    __shared__ uint64_t wc[kPlanThreads / 64], wt[kPlanThreads / 64];
+    const uint32_t n = *n_parts;
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    uint64_t c[kPlanItems], t[kPlanItems], sc = 0, st = 0;
+#pragma unroll
+    for (int k = 0; k < kPlanItems; k++) {
+        const uint32_t s = (uint32_t)tid * kPlanItems + k;
+        c[k] = 0;
+        t[k] = 0;
+        if (s < n) {
+            const Part &P = parts[s];
+            const bool inner = P.status != kDone && P.depth + 1 < P.levels;
+            const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;
+            c[k] = (nl > 0) + (nr > 0);
+            t[k] = tiles_for(nl) + tiles_for(nr);
+        }
+        sc += c[k];
+        st += t[k];
+    }
+    const uint64_t ic = wave_incl_add(sc), it = wave_incl_add(st);
+    if (lane == 63) { wc[wid] = ic; wt[wid] = it; }
+    __syncthreads();
+    if (wid == 0) {
+        const uint64_t vc = lane < kPlanThreads / 64 ? wc[lane] : 0, vt = lane < kPlanThreads / 64 ? wt[lane] : 0;
+        const uint64_t jc = wave_incl_add(vc), jt = wave_incl_add(vt);
+        if (lane < kPlanThreads / 64) { wc[lane] = jc - vc; wt[lane] = jt - vt; }
+        if (lane == kPlanThreads / 64 - 1) {
+            *parts_total = jc;
+            *tiles_total = jt;
+            *next_n_parts = (uint32_t)jc;
+            *next_n_tiles = (uint32_t)jt;
+        }
+    }
+    __syncthreads();
+    uint64_t ec = wc[wid] + ic - sc, et = wt[wid] + it - st;
+#pragma unroll
+    for (int k = 0; k < kPlanItems; k++) {
+        const uint32_t s = (uint32_t)tid * kPlanItems + k;
+        if (s < n) {
+            child_off[s] = ec;
+            tile_off[s] = et;
+        }
+        ec += c[k];
+        et += t[k];
+    }
 }
 
 template <typename Acc>
@@ -1057,10 +1134,13 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
 __global__ void __launch_bounds__(kBlock)
 k_tile_left(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
             const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
-            uint32_t *__restrict__ below, uint64_t *__restrict__ tile_left) {
-    __shared__ uint32_t wcount[kBlock / 64];
+            uint32_t *__restrict__ below, const uint16_t *__restrict__ tile_bins,
+            uint64_t *__restrict__ tile_left) {
+    // one wavefront per tile
     const uint32_t nt = *n_tiles;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const int lane = lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         uint64_t left;
@@ -1068,22 +1148,22 @@ k_tile_left(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
             left = 0;                          // not partitioned
         } else if (P.status == kAllLeft) {
             left = T.count;
-        } else if (P.counts_ready) {
+        } else if (P.counts_ready == kCountsBelow) {
             left = below[t];                   // counted by the previous partition
+        } else if (P.counts_ready == kCountsBins) {
+            // bins 0..kstar of this tile in the pass that decided the split
+            const uint16_t *tb = tile_bins + (uint64_t)t * kMaxBins;
+            uint32_t k = 0;
+            for (int j = lane; j <= P.kstar; j += 64) k += tb[j];
+            left = wave_sum(k);
         } else {
             const float *c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
             const float split = P.split;
             uint32_t k = 0;
-            for (uint64_t i = T.begin + threadIdx.x; i < T.begin + T.count; i += kBlock)
-                k += !(c[i] >= split) ? 1u : 0u;
-            k = wave_sum(k);
-            if (lane_id() == 0) wcount[threadIdx.x >> 6] = k;
-            __syncthreads();
-            left = 0;
-            for (int v = 0; v < kBlock / 64; v++) left += wcount[v];
-            __syncthreads();
+            for (uint64_t i = T.begin + lane; i < T.begin + T.count; i += 64) k += !(c[i] >= split) ? 1u : 0u;
+            left = wave_sum(k);
         }
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             tile_left[t] = left;
             below[t] = 0;                      // ready for the next partition
         }
