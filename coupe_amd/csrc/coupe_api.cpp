@@ -269,3 +269,14 @@ extern "C" enum coupe_err coupe_hip_imbalance(coupe_hip_ctx *ctx, void *stream, 
     return coupe_hip::imbalance(*c, (hipStream_t)stream, d_partition, n, weight_type, d_weights, num_parts,
                                 imbalance);
 }
+
+extern "C" enum coupe_err coupe_hip_barycentres(coupe_hip_ctx *ctx, void *stream, const double *d_coords,
+                                                uintptr_t n_nodes, uintptr_t dimension,
+                                                const int64_t *d_element_nodes, uintptr_t n_elements,
+                                                uintptr_t nodes_per_element, double *d_out) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    return coupe_hip::barycentres(*c, (hipStream_t)stream, d_coords, n_nodes, dimension, d_element_nodes,
+                                  n_elements, nodes_per_element, d_out);
+}

@@ -110,6 +110,10 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
 enum coupe_err rib_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,
                           const double *d_points, int wkind, const void *d_weights, size_t iter_count, double tol);
 
+// Element barycentres (see coupe_hip_barycentres).
+enum coupe_err barycentres(Context &cx, hipStream_t st, const double *coords, uint64_t n_nodes, size_t dim,
+                           const int64_t *nodes, uint64_t n_el, size_t k, double *out);
+
 // Per-part loads and imbalance of a partition (coupe/src/imbalance.rs).
 enum coupe_err parts_load(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,
                           uint64_t num_parts, void *host_loads);

@@ -97,6 +97,19 @@ struct coupe_hip_rib_info {
 enum coupe_err coupe_hip_last_rib(coupe_hip_ctx *ctx, struct coupe_hip_rib_info *info);
 
 /*
+ * Barycentres of mesh elements of one type, in device memory: element e has
+ * the nodes element_nodes[e*k .. e*k+k) (0-based indices into the node
+ * array, k = nodes_per_element); out[e*dimension + a] = (sum over its nodes
+ * of coords[node*dimension + a], in node order, from 0.0) / k.  A node index
+ * outside [0, n_nodes) returns COUPE_ERR_CRASH (coupe's indexing panics).
+ * Replaces: the per-element loop of barycentres(), tools/lib/lib.rs:511-539
+ * (the caller chooses the element blocks, as that function filters them).
+ */
+enum coupe_err coupe_hip_barycentres(coupe_hip_ctx *ctx, void *stream, const double *d_coords,
+		uintptr_t n_nodes, uintptr_t dimension, const int64_t *d_element_nodes, uintptr_t n_elements,
+		uintptr_t nodes_per_element, double *d_out);
+
+/*
  * Per-part loads of a partition held in device memory: loads[p] = sum of the
  * weights of the points with id p, in the weight type (int sums wrap like
  * Rust's release arithmetic; int64 exact; double in device order).

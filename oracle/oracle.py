@@ -35,6 +35,10 @@ The oracle restates coupe's ``Rcb`` (coupe/src/algorithms/recursive_bisection.rs
   last bits of the rotation, and ids that depend on them, are "parity
   unpinned" against the reference; the GPU tests compare ids given the
   rotation the GPU used.
+* ``barycentres`` (numpy restatement of tools/lib/lib.rs:511-539) -- the
+  reference's MEDIT test mesh (mesh-io/src/medit/parser.rs:578-595) with its
+  barycentres computed by hand, centres of unit squares / cubes, edges and
+  lower-dimension elements left out, out-of-range nodes rejected.
 * Non-integer f64 weights: the reference's summation order is rayon's and
   unspecified, so ids for real weights are "parity unpinned" beyond the
   1e-12 relative per-part weight tolerance used by the GPU tests.
@@ -321,3 +325,32 @@ def rib(points, weights, iter_count, tolerance, *, obb_to_aabb=None, n_part=None
     err = lib().oracle_rib(ids.ctypes.data, n_part, dim, pts.ctypes.data, n, kind, w.ctypes.data, w.shape[0], 0,
                            int(iter_count), float(tolerance), None, None if m is None else m.ctypes.data)
     return err, ids[:n_part]
+
+
+# element name -> (dimension, nodes per element), mesh-io/src/lib.rs:24-56
+_ELEMENTS = {"edges": (1, 2), "triangles": (2, 3), "quadrilaterals": (2, 4), "quadrangles": (2, 4),
+             "tetrahedra": (3, 4), "hexahedra": (3, 8)}
+
+
+def barycentres(dimension, coordinates, topology):
+    """barycentres() of tools/lib/lib.rs:511-539: for the elements of the
+    highest element dimension except edges, in topology order, the sum of
+    their nodes' coordinates taken in node order from 0.0, divided by the
+    node count.  ``topology`` is a list of (type name, 0-based node index
+    array (n x k)).  Raises IndexError for a node index out of range."""
+    coords = np.asarray(coordinates, dtype=np.float64).reshape(-1, dimension)
+    if not topology:
+        return np.zeros((0, dimension))
+    top = max(_ELEMENTS[name][0] for name, *_ in topology)
+    out = []
+    for name, nodes, *_ in topology:
+        if _ELEMENTS[name][0] != top or name == "edges":
+            continue
+        nodes = np.asarray(nodes, dtype=np.int64)
+        if nodes.size and (nodes.min() < 0 or nodes.max() >= coords.shape[0]):
+            raise IndexError("node index out of range")
+        bc = np.zeros((nodes.shape[0], dimension))
+        for j in range(nodes.shape[1]):
+            bc = bc + coords[nodes[:, j]]
+        out.append(bc / float(nodes.shape[1]))
+    return np.concatenate(out) if out else np.zeros((0, dimension))
