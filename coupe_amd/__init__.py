@@ -27,7 +27,7 @@ import numpy as np
 from . import _native
 from ._native import strerror
 
-__all__ = ["Rcb", "Rib", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
+__all__ = ["Rcb", "Rib", "Grid", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
            "partition_batch", "Server"]
 
 
@@ -336,3 +336,62 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Grid:
+    """coupe::Grid (coupe/src/cartesian/mod.rs:43-180): a 2D or 3D cartesian
+    grid partitioned by RCB on the GPU (coupe_hip_grid_rcb).  Weights and ids
+    are in memory order, x fastest."""
+
+    def __init__(self, size):
+        size = tuple(int(s) for s in size)
+        if len(size) not in (2, 3) or min(size) < 1:
+            raise ValueError("a grid has 2 or 3 axes of at least one cell")
+        self.size = size
+
+    @classmethod
+    def new_2d(cls, width: int, height: int) -> "Grid":
+        return cls((width, height))
+
+    @classmethod
+    def new_3d(cls, width: int, height: int, depth: int) -> "Grid":
+        return cls((width, height, depth))
+
+    def __len__(self):
+        n = 1
+        for s in self.size:
+            n *= s
+        return n
+
+    def rcb(self, partition, weights, iter_count: int, *, threads: int = 0, stream=None,
+            context: DeviceContext = None):
+        """Grid::rcb(&mut partition, &weights, iter_count).  ``partition`` and
+        ``weights`` are numpy arrays (host) or GPU tensors.  ``threads``: the
+        rayon pool size coupe's median chunking follows (0: RAYON_NUM_THREADS,
+        else the logical CPU count)."""
+        import torch
+
+        n = len(self)
+        if len(weights) != n or len(partition) != n:
+            raise ValueError(f"a {self.size} grid has {n} cells")
+        device_io = _is_device_tensor(weights)
+        dev = weights.device if device_io else torch.device("cuda", 0)
+        if device_io:
+            w = weights.contiguous()
+            wtype = _weight_type(w.dtype)
+        else:
+            wn = np.ascontiguousarray(weights)
+            wtype = _weight_type(wn.dtype)
+            w = torch.from_numpy(wn).to(dev)
+        out = partition if _is_device_tensor(partition) else torch.empty(n, dtype=torch.int64, device=dev)
+        ctx = context or _context_for(dev.index or 0)
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        sz = (C.c_size_t * 3)(*(list(self.size) + [1] * (3 - len(self.size))))
+        err = _native.lib().coupe_hip_grid_rcb(ctx.handle, C.c_void_p(st), len(self.size), sz, wtype, w.data_ptr(),
+                                                int(iter_count), int(threads), out.data_ptr())
+        if err != _native.OK:
+            raise CoupeError(err)
+        if not _is_device_tensor(partition):
+            torch.cuda.synchronize(dev)
+            partition[:] = out.cpu().numpy().astype(partition.dtype, copy=False)
+        return None

@@ -22,7 +22,7 @@ EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb", "coupe_rib",
     "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_batch",
-    "coupe_hip_rib_device", "coupe_hip_last_rib",
+    "coupe_hip_rib_device", "coupe_hip_last_rib", "coupe_hip_grid_rcb",
     "coupe_hip_barycentres", "coupe_hip_parts_load", "coupe_hip_imbalance",
     "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
     "coupe_hip_last_stats", "coupe_hip_set_profiling", "coupe_hip_kernel_times",
@@ -133,6 +133,9 @@ def lib():
         L.coupe_hip_rcb_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                           C.c_void_p, C.POINTER(Instance), C.c_size_t]
         L.coupe_hip_rcb_batch.restype = C.c_int
+        L.coupe_hip_grid_rcb.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int,
+                                         C.c_void_p, C.c_size_t, C.c_uint32, C.c_void_p]
+        L.coupe_hip_grid_rcb.restype = C.c_int
         L.coupe_hip_barycentres.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                             C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
         L.coupe_hip_barycentres.restype = C.c_int

@@ -280,3 +280,16 @@ extern "C" enum coupe_err coupe_hip_barycentres(coupe_hip_ctx *ctx, void *stream
     return coupe_hip::barycentres(*c, (hipStream_t)stream, d_coords, n_nodes, dimension, d_element_nodes,
                                   n_elements, nodes_per_element, d_out);
 }
+
+extern "C" enum coupe_err coupe_hip_grid_rcb(coupe_hip_ctx *ctx, void *stream, uintptr_t dimension,
+                                             const uintptr_t *size, enum coupe_type weight_type,
+                                             const void *d_weights, uintptr_t iter_count, uint32_t threads,
+                                             uint64_t *d_partition) {
+    if (!ctx || !size) return COUPE_ERR_CRASH;
+    if (dimension != 2 && dimension != 3) return COUPE_ERR_BAD_DIMENSION;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    const uint64_t sz[3] = {size[0], size[1], dimension == 3 ? size[2] : 1};
+    return coupe_hip::grid_rcb(*c, (hipStream_t)stream, dimension, sz, weight_type, d_weights, iter_count, threads,
+                               d_partition);
+}

@@ -65,6 +65,8 @@ struct Context {
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
+    DevBuf gparts[2], gaxis, gchunk, gcount, gctr;   // Grid::rcb: subgrids, axis sums, chunk offsets
+    bool grid_clean = false;         // Grid::rcb axis sums are all zero
     DevBuf rib_moments;              // Rib: block partials, centroid, inertia
     coupe_hip_rib_info rib{};        // Rib: rotation of the last call
     const Rotation *rot = nullptr;   // change of basis for the running call
@@ -105,6 +107,14 @@ inline int search_passes_from_env() {
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
                          const double *tols, size_t n_inst, const Rotation *rot = nullptr);
+
+// Exclusive scan of n_dev (read on the device, at most n_ub) uint64 values in
+// place; `total` (device, may be null) receives their sum.
+void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub, uint64_t *total);
+
+// Grid::rcb on a device-resident weight array (see coupe_hip_grid_rcb).
+enum coupe_err grid_rcb(Context &cx, hipStream_t st, size_t dim, const uint64_t *size, int wkind, const void *w,
+                        size_t iter_count, uint32_t threads, uint64_t *out);
 
 // coupe::Rib on device-resident inputs (see coupe_hip_rib_device).
 enum coupe_err rib_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,

@@ -120,7 +120,7 @@ static inline unsigned grid_for(uint64_t items, unsigned per_block) {
 }
 
 // Exclusive scan of `n_dev` (device) uint64 values, in place, upper bound n_ub.
-static void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub,
+void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub,
                      uint64_t *total) {
     uint64_t nblocks = (n_ub + kScanTile - 1) / kScanTile;
     if (nblocks < 1) nblocks = 1;
@@ -701,7 +701,7 @@ uint64_t Context::device_bytes() const {
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
 }
@@ -710,7 +710,7 @@ void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &prologue, &counters, &in_points, &in_weights, &out_ids};
+                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

@@ -97,6 +97,22 @@ struct coupe_hip_rib_info {
 enum coupe_err coupe_hip_last_rib(coupe_hip_ctx *ctx, struct coupe_hip_rib_info *info);
 
 /*
+ * RCB of a cartesian grid (coupe::Grid::rcb): `size` holds the cell counts of
+ * the dimension (2 or 3) axes; the weights and the ids are in memory order,
+ * x fastest (cell (x, y[, z]) at x + size[0] * (y + size[1] * z)).  Splits
+ * start on axis 1 and cycle; ids are the split paths (left 0, right 1),
+ * most significant first.  `threads` is the rayon pool size whose chunking
+ * coupe's weighted median follows (0: RAYON_NUM_THREADS, else the number of
+ * logical CPUs, as rayon's global pool).  Integer weights give coupe's ids
+ * bit for bit.
+ * Replaces: Grid::new_2d / new_3d + Grid::rcb, coupe/src/cartesian/mod.rs:127-180,
+ * coupe/src/cartesian/rcb.rs:52-281.
+ */
+enum coupe_err coupe_hip_grid_rcb(coupe_hip_ctx *ctx, void *stream, uintptr_t dimension, const uintptr_t *size,
+		enum coupe_type weight_type, const void *d_weights, uintptr_t iter_count, uint32_t threads,
+		uint64_t *d_partition);
+
+/*
  * Barycentres of mesh elements of one type, in device memory: element e has
  * the nodes element_nodes[e*k .. e*k+k) (0-based indices into the node
  * array, k = nodes_per_element); out[e*dimension + a] = (sum over its nodes

@@ -39,6 +39,14 @@ The oracle restates coupe's ``Rcb`` (coupe/src/algorithms/recursive_bisection.rs
   reference's MEDIT test mesh (mesh-io/src/medit/parser.rs:578-595) with its
   barycentres computed by hand, centres of unit squares / cubes, edges and
   lower-dimension elements left out, out-of-range nodes rejected.
+* ``grid_rcb`` / ``weighted_median`` (oracle/grid_oracle.c, coupe's
+  ``Grid::rcb``) -- the ``Grid`` doctest (mod.rs:21-41: a 2x2 grid, 2
+  iterations, ids {0,1,2,3}), ``test_3d`` (rcb.rs:316-361: a 4^3 grid, 3
+  iterations, 2x2x2 blocks), ``test_weighted_median`` (rcb.rs:300-314:
+  left weight = sum of the weights before the position) on seeded inputs,
+  and the closed form of power-of-two grids with unit weights (bit-halving
+  splits, first axis 1, ids = interleaved bits).  The result depends on
+  the rayon pool size (the median's chunk size); ``threads`` sets it.
 * Non-integer f64 weights: the reference's summation order is rayon's and
   unspecified, so ids for real weights are "parity unpinned" beyond the
   1e-12 relative per-part weight tolerance used by the GPU tests.
@@ -112,6 +120,11 @@ def lib():
         L.oracle_householder.restype = None
         L.oracle_map_points.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
         L.oracle_map_points.restype = None
+        L.oracle_grid_rcb.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
+                                      C.c_int, C.c_void_p]
+        L.oracle_grid_rcb.restype = C.c_int
+        L.oracle_weighted_median.argtypes = [C.c_int, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_weighted_median.restype = None
         L.oracle_bbox.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p]
         L.oracle_bbox.restype = C.c_int
         L.oracle_split_1d.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_double, C.c_float,
@@ -354,3 +367,30 @@ def barycentres(dimension, coordinates, topology):
             bc = bc + coords[nodes[:, j]]
         out.append(bc / float(nodes.shape[1]))
     return np.concatenate(out) if out else np.zeros((0, dimension))
+
+
+def grid_rcb(size, weights, iter_count, threads=16, splits=False):
+    """``Grid::rcb`` on the CPU: ids in memory order (x fastest).  With
+    ``splits`` also the heap of split positions (index 1 = root)."""
+    size = [int(x) for x in size]
+    dim = len(size)
+    w = np.ascontiguousarray(weights)
+    kind = _kind_of(w)
+    n = int(np.prod(size))
+    part = np.zeros(max(n, 1), dtype=np.uint64)
+    sz = (C.c_size_t * 3)(*(size + [1] * (3 - dim)))
+    sp = np.zeros(1 << iter_count, dtype=np.int64) if splits and iter_count < 30 else None
+    err = lib().oracle_grid_rcb(part.ctypes.data, dim, sz, kind, w.ctypes.data, int(iter_count), int(threads),
+                                None if sp is None else sp.ctypes.data)
+    if err:
+        raise RuntimeError(f"oracle_grid_rcb: error {err}")
+    return (part[:n], sp) if splits else part[:n]
+
+
+def weighted_median(weights, threads=16):
+    w = np.ascontiguousarray(weights)
+    kind = _kind_of(w)
+    pos = C.c_size_t()
+    left = np.zeros(1, dtype=np.float64 if kind == W_DOUBLE else np.int64)
+    lib().oracle_weighted_median(kind, w.ctypes.data, w.shape[0], int(threads), C.byref(pos), left.ctypes.data)
+    return pos.value, left[0]
