@@ -290,6 +290,7 @@ def main():
         del d_pts, d_w, d_out
         torch.cuda.empty_cache()
         out["configs"] = bench_configs(torch, coupe_amd, workloads)
+        out["grid"] = bench_grid(torch, coupe_amd, with_cpu=not args.no_cpu)
         torch.cuda.empty_cache()
         out["batched"] = bench_batched(torch, coupe_amd, workloads)
         torch.cuda.empty_cache()
@@ -398,6 +399,56 @@ def bench_configs(torch, coupe_amd, workloads, steps=10, warmup=3):
                      "Mpts_s": round(pts.shape[0] / (ms * 1e-3) / 1e6, 1), "passes": st["passes"]}
         log(f"config {name}: {ms:.3f} ms, {res[name]['Mpts_s']} Mpts/s")
         del d_pts, d_w, d_out
+    return res
+
+
+def bench_grid(torch, coupe_amd, with_cpu=True, side=10000, iters=12, threads=16, steps=10):
+    """coupe's own Grid benchmark (coupe/benches/rcb_cartesian.rs): a
+    10000 x 10000 grid, weight of cell i = i as f64, iter_count 12; the CPU
+    oracle (sequential restatement) on the 2000 x 2000 grid of the same shape
+    as the baseline and parity sample."""
+    dev = torch.device("cuda:0")
+    n = side * side
+    w = torch.arange(n, dtype=torch.float64, device=dev)
+    out = torch.zeros(n, dtype=torch.int64, device=dev)
+    ctx = coupe_amd.DeviceContext(0)
+    grid = coupe_amd.Grid.new_2d(side, side)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(3):
+        grid.rcb(out, w, iters, threads=threads, context=ctx, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    ev = []
+    for _ in range(steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        grid.rcb(out, w, iters, threads=threads, context=ctx, stream=stream.cuda_stream)
+        e1.record(stream)
+        ev.append((e0, e1))
+    torch.cuda.synchronize()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)[steps // 2]
+    # bytes: one read of the weights per level, one write of the ids
+    gbps = n * (8.0 * iters + 8.0) / (ms * 1e-3) / 1e9
+    res = {"workload": f"{side}x{side} grid, weight of cell i = i (f64), iter_count {iters}, rayon pool size "
+                       f"{threads} for the median's chunking (coupe/benches/rcb_cartesian.rs)",
+           "cells": n, "median_ms": round(ms, 4), "Mcells_s": round(n / (ms * 1e-3) / 1e6, 1),
+           "algo_GBps": round(gbps, 1)}
+    log(f"grid {side}^2: {ms:.3f} ms, {res['Mcells_s']} Mcells/s")
+    del w, out
+    if with_cpu:
+        from oracle import oracle
+        s = 2000
+        ws = np.arange(s * s, dtype=np.float64)
+        part = np.zeros(s * s, dtype=np.uint64)
+        grid2 = coupe_amd.Grid.new_2d(s, s)
+        grid2.rcb(part, ws, iters, threads=threads, context=ctx)
+        t1 = time.perf_counter()
+        exp = oracle.grid_rcb([s, s], ws, iters, threads=threads)
+        cpu_s = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": round(s * s / cpu_s / 1e6, 2), "unit": "Mcells/s", "cores": 1,
+                               "kind": "port", "sample": f"{s}x{s} grid of the same shape, sequential C "
+                                                         f"restatement of Grid::rcb, {cpu_s:.2f} s"}
+        res["ids_equal_to_cpu_oracle_on_sample"] = bool(np.array_equal(part, exp))
+    ctx.close()
     return res
 
 
