@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcoupe_hip.so")
+# COUPE_HIP_LIBRARY selects another build of the library (A/B measurements).
+LIB_PATH = os.environ.get("COUPE_HIP_LIBRARY") or os.path.join(HERE, "libcoupe_hip.so")
 WORKLOADS_PATH = os.path.join(HERE, "libcoupe_workloads.so")
 
 # enum coupe_err (include/coupe.h)
