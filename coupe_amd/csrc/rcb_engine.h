@@ -51,7 +51,7 @@ struct Context {
     std::mutex lock;                 // one call at a time per context
     DevBuf cx[3][3];                 // SoA f32 coordinates: input order, then two sides
     DevBuf w[2];                     // stored weights, two sides
-    DevBuf parts[2], tiles[2];       // part / tile tables, current and next level
+    DevBuf parts[3], tiles[3];       // part / tile tables (two levels per move uses three)
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
     DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
     DevBuf bsum;                     // scan block sums
@@ -61,7 +61,9 @@ struct Context {
     DevBuf instances;                // instance table of the current call
     DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
     DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
+    DevBuf below2, tile_bins2, tile4;   // two levels per move: per (tile, side) counts, 4 grandchild counts
     int search_passes = 3;           // histogram passes queued per level without a host check
+    bool pairs = true;               // move the points once per two levels
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
@@ -89,6 +91,12 @@ enum coupe_err map_hip_error(hipError_t e);
 inline bool lookback_from_env() {
     const char *v = getenv("COUPE_HIP_PARTITION");
     return v && v[0] == 'l';
+}
+
+// COUPE_HIP_PAIRS=0: move the points at every level instead of every other.
+inline bool pairs_from_env() {
+    const char *v = getenv("COUPE_HIP_PAIRS");
+    return !(v && v[0] == '0');
 }
 
 // COUPE_HIP_SEARCH_PASSES=k: histogram passes queued per level before the

@@ -12,6 +12,7 @@
 
 #include "rcb_engine.h"
 #include "rcb_kernels.h"
+#include "rcb_pairs.h"
 
 namespace coupe_hip {
 
@@ -100,8 +101,8 @@ constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
 constexpr int kMaxTickets = 4096;
 constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
 struct Counters {
-    uint32_t n_parts[2];
-    uint32_t n_tiles[2];
+    uint32_t n_parts[3];
+    uint32_t n_tiles[3];
     uint32_t searching;
     uint32_t error;
     uint64_t parts_total;
@@ -230,7 +231,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     if (tiles_cap > 0xffffffffull || parts_cap > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
     if ((uint64_t)L * (uint64_t)(spec_passes + 1) > (uint64_t)kSpecSlots) spec_passes = 0;
 
-    for (int b = 0; b < 2; b++) {
+    // Two levels per move (dense trees, prefix partition): three part / tile
+    // tables (a level, its children searched in place, its grandchildren).
+    const bool pairs = cx.pairs && B.dense && !cx.lookback;
+    for (int b = 0; b < (pairs ? 3 : 2); b++) {
         cx.parts[b].ensure(parts_cap * sizeof(Part));
         cx.tiles[b].ensure(tiles_cap * sizeof(Tile));
     }
@@ -326,37 +330,29 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     cx.stats.unit_points = kTile;
     cx.stats.bins = kFirstBins;
 
-    // Split search of the parts of slot `sl` after a first pass (k_sweep or
-    // the fused partition) filled their first bins (kFirstBins or 2).
-    auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, uint64_t n_parts_host,
-                      bool first_standalone) {
+    // Split search of the parts of table `sl` (level `level`) after a first
+    // pass filled their first bins (kFirstBins or 2).  `hist(gate)` launches
+    // one histogram pass over the parts' points.  With spec_passes > 0 the
+    // passes are queued without host checks (see run_levels' comment).
+    auto search_gen = [&](int sl, size_t level, uint64_t parts_ub, int first_nb, unsigned long long *swept_first,
+                          uint32_t counts_first, uint32_t counts_hist, auto &&hist) {
         Part *parts = (Part *)cx.parts[sl].ptr;
-        float *const *cl = cs[cslot(level)];
-        WS *wl = ws[wslot(level)];
-        Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
-        uint32_t *ntl = &ctr->n_tiles[sl];
         const int nb = choose_bins(parts_ub);
-        (void)n_parts_host;
+        cx.stats.bins = std::max<uint32_t>(cx.stats.bins, (uint32_t)nb);
         if (spec_passes > 0) {
-            // Queued search: slot p counts the parts still searching after
-            // decision p; pass p + 1 runs only if it is non-zero.
+            // slot p counts the parts still searching after decision p; pass
+            // p + 1 runs only if it is non-zero
             uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
-                    parts, np, bcnt, bw, bmin, first_standalone ? kFirstBins : 2, wrap32, &slot[0],
-                    first_standalone ? &ctr->swept : &ctr->swept_fused, (first_standalone || lookback) ? kCountsRead : kCountsBelow,
-                    nullptr);
+                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
+                                                               &slot[0], swept_first, counts_first, nullptr);
             });
-            const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
             for (int p = 1; p <= spec_passes; p++) {
-                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
-                    k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
-                                                         bmin, &slot[p - 1], tile_bins);
-                });
+                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                     k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                                   &slot[p], &ctr->swept, hist_counts, &slot[p - 1]);
+                                                                   &slot[p], &ctr->swept, counts_hist, &slot[p - 1]);
                 });
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
@@ -367,11 +363,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         }
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin,
-                                                           first_standalone ? kFirstBins : 2, wrap32,
-                                                           &ctr->searching,
-                                                           first_standalone ? &ctr->swept : &ctr->swept_fused,
-                                                           (first_standalone || lookback) ? kCountsRead : kCountsBelow, nullptr);
+            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
+                                                           &ctr->searching, swept_first, counts_first, nullptr);
         });
         uint32_t passes = 1;
         for (;;) {
@@ -381,20 +374,37 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // f32 bisection ends within a few hundred candidates; each pass takes 8
             if (passes >= 4096) throw HipFailure{hipErrorUnknown};
             check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
-            const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
-            cx.stats.bins = nb;
-            timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
-                k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt, bw,
-                                                     bmin, nullptr, tile_bins);
-            });
+            timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, nullptr); });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept, hist_counts, nullptr);
+                                                               &ctr->searching, &ctr->swept, counts_hist, nullptr);
             });
             passes++;
         }
         cx.stats.passes += passes;
         cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+    };
+
+    // Search of parts whose points are contiguous in `tiles` (coordinates cl,
+    // weights wl).
+    auto search_at = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, bool first_standalone,
+                         Tile *tiles, float *const *cl, WS *wl, uint32_t counts_first) {
+        Part *parts = (Part *)cx.parts[sl].ptr;
+        uint32_t *ntl = &ctr->n_tiles[sl];
+        const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
+        search_gen(sl, level, parts_ub, first_standalone ? kFirstBins : 2,
+                   first_standalone ? &ctr->swept : &ctr->swept_fused, counts_first, hist_counts,
+                   [&](int nb, const uint32_t *gate) {
+                       k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt,
+                                                             bw, bmin, gate, tile_bins);
+                   });
+    };
+    auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, uint64_t n_parts_host,
+                      bool first_standalone) {
+        (void)n_parts_host;
+        search_at(sl, level, parts_ub, tiles_ub, first_standalone, level == 0 ? tiles0 : (Tile *)cx.tiles[sl].ptr,
+                  cs[cslot(level)], ws[wslot(level)],
+                  (first_standalone || lookback) ? kCountsRead : kCountsBelow);
     };
 
     // Level 0: the root parts' tables come from k_init_roots (in prologue());
@@ -417,75 +427,180 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     }
 
     int cur = 0;
-    for (size_t level = 0; level < L; level++) {
-        cx.stats.levels++;
-        const int nxt = cur ^ 1;
-        Part *parts = (Part *)cx.parts[cur].ptr;
-        Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[cur].ptr;
-        uint32_t *np = &ctr->n_parts[cur];
-        uint32_t *ntl = &ctr->n_tiles[cur];
-
-        // Children (next level's parts and tiles), split records, id offsets.
-        const bool plan1 = parts_ub <= kPlanParts;
-        if (plan1) {
+    if (pairs) {
+        // Two levels per move.  Real levels l = 0, 2, 4, ...: parts with
+        // contiguous points (table ia, tiles tl, coordinates cl, weights wl).
+        // Their children (table ib) are searched in place over the parent's
+        // tiles; the grandchildren (table ic) receive the points.
+        cx.below2.ensure(tiles_cap * 2 * sizeof(uint32_t));
+        cx.tile_bins2.ensure(tiles_cap * 2 * kMaxBins * sizeof(uint16_t));
+        cx.tile4.ensure(tiles_cap * 4 * sizeof(uint64_t));
+        uint32_t *below2 = (uint32_t *)cx.below2.ptr;
+        uint16_t *tile_bins2 = (uint16_t *)cx.tile_bins2.ptr;
+        uint64_t *q4[4];
+        for (int j = 0; j < 4; j++) q4[j] = (uint64_t *)cx.tile4.ptr + (uint64_t)j * tiles_cap;
+        int ia = 0, csl = 0, wsl = 0;
+        Tile *tl = tiles0;
+        uint64_t tl_ub = std::max<uint64_t>(B.tiles0, 1);
+        auto children_of = [&](int from, int to, size_t level, Tile *next_tiles) {
+            Part *pf = (Part *)cx.parts[from].ptr;
+            const uint64_t pub = std::max<uint64_t>(B.parts_ub[level], 1);
+            if (pub <= kPlanParts) {
+                timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                    k_children_plan<<<1, kPlanThreads, 0, st>>>(pf, &ctr->n_parts[from], (uint64_t *)cx.nchild.ptr,
+                                                               (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
+                                                               &ctr->tiles_total, &ctr->n_parts[to], &ctr->n_tiles[to]);
+                });
+            } else {
+                timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                    k_children_count<<<grid_for(pub, kBlock), kBlock, 0, st>>>(
+                        pf, &ctr->n_parts[from], (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
+                });
+                scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, &ctr->n_parts[from], pub, &ctr->parts_total);
+                scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, &ctr->n_parts[from], pub, &ctr->tiles_total);
+            }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_plan<<<1, kPlanThreads, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
-                                                           (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
-                                                           &ctr->tiles_total, &ctr->n_parts[nxt], &ctr->n_tiles[nxt]);
+                k_children_write<Acc><<<(unsigned)pub, 64, 0, st>>>(
+                    pf, &ctr->n_parts[from], D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
+                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts);
+                if (pub > kPlanParts)
+                    k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[to],
+                                                  &ctr->n_tiles[to]);
             });
-        } else {
+        };
+        for (size_t level = 0; level < L; level += 2) {
+            const int ib = (ia + 1) % 3, ic = (ia + 2) % 3;
+            Part *pa = (Part *)cx.parts[ia].ptr;
+            Part *pb = (Part *)cx.parts[ib].ptr;
+            uint32_t *ntl = &ctr->n_tiles[ia];
+            float *const *cl = cs[csl];
+            WS *wl = ws[wsl];
+            cx.stats.levels++;
+            children_of(ia, ib, level, nullptr);   // in place: no tiles
+            if (level + 1 == L) break;
+            cx.stats.levels++;
+            // the children's first pass over the parent's tiles
+            {
+                const unsigned g = persistent_grid(cx, k_child_first<D, WS, Acc>, tl_ub);
+                timed(cx, st, COUPE_HIP_K_HIST, 12.0 * (double)n, [&] {
+                    k_child_first<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, cl[0], cl[1], cl[2], wl, bcnt, bw,
+                                                                   bmin, below2, (uint64_t *)cx.tile_left.ptr);
+                });
+            }
+            const uint64_t pub1 = std::max<uint64_t>(B.parts_ub[level + 1], 1);
+            {
+                const unsigned g = persistent_grid(cx, k_hist_virtual<D, WS, Acc>, tl_ub);
+                search_gen(ib, level + 1, pub1, 2, &ctr->swept_fused, kCountsBelow, kCountsBins,
+                           [&](int nb, const uint32_t *gate) {
+                               k_hist_virtual<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
+                                                                               wl, nb, bcnt, bw, bmin, gate,
+                                                                               tile_bins2);
+                           });
+            }
+            // grandchildren, with tiles
+            Tile *tn = (Tile *)cx.tiles[ic].ptr;
+            children_of(ib, ic, level + 1, tn);
+            if (level + 2 == L) break;
+            // the move
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
-                    parts, np, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
+                const unsigned g = persistent_grid(cx, k_tile_count4<D>, (tl_ub + kBlock / 64 - 1) / (kBlock / 64));
+                k_tile_count4<D><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
+                                                      (const uint64_t *)cx.tile_left.ptr, below2, tile_bins2, q4[0],
+                                                      q4[1], q4[2], q4[3]);
             });
-            scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
-            scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
+            for (int j = 0; j < 4; j++) scan_u64(cx, st, q4[j], ntl, tl_ub, nullptr);
+            const int csn = csl == 1 ? 2 : 1, wsn = wsl ^ 1;
+            float *const *co = cs[csn];
+            {
+                const unsigned g = persistent_grid(cx, k_partition4<D, WS, Acc>, tl_ub);
+                timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                    k_partition4<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                        pa, pb, (const Part *)cx.parts[ic].ptr, tl, ntl, q4[0], q4[1], q4[2], q4[3], cl[0], cl[1],
+                        cl[2], wl, co[0], co[1], co[2], ws[wsn], n, bcnt, bw, bmin);
+                });
+            }
+            // the grandchildren's search over their own tiles
+            const uint64_t pub2 = std::max<uint64_t>(B.parts_ub[level + 2], 1);
+            const uint64_t tub2 = std::min<uint64_t>(tiles_cap, pub2 + (n + kTile - 1) / kTile);
+            search_at(ic, level + 2, pub2, tub2, false, tn, co, ws[wsn], kCountsRead);
+            ia = ic;
+            csl = csn;
+            wsl = wsn;
+            tl = tn;
+            tl_ub = tub2;
         }
-        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-            k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
-                parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
-                node_base[std::min(level + 1, L)], heap_split, insts);
-            if (!plan1)
-                k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
-                                              &ctr->n_tiles[nxt]);
-        });
-        if (level + 1 == L) break;   // every part left is at its last level
+        parts_ub = std::max<uint64_t>(B.parts_ub[L - 1], 1);
+    } else {
+        for (size_t level = 0; level < L; level++) {
+            cx.stats.levels++;
+            const int nxt = cur ^ 1;
+            Part *parts = (Part *)cx.parts[cur].ptr;
+            Tile *tiles = level == 0 ? tiles0 : (Tile *)cx.tiles[cur].ptr;
+            uint32_t *np = &ctr->n_parts[cur];
+            uint32_t *ntl = &ctr->n_tiles[cur];
 
-        // Partition of this level, fused with the children's first pass.
-        float *const *ci = cs[cslot(level)];
-        float *const *co = cs[cslot(level + 1)];
-        if (lookback) {
-            const uint32_t epoch = ++cx.epoch;
-            unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
-            if (level > 0 && level % kMaxTickets == 0)
-                check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
-            const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
-            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-                k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
-                    parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0],
-                    ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
-            });
-        } else {
-            const unsigned gt = persistent_grid(cx, k_tile_left, (tiles_ub + kBlock / 64 - 1) / (kBlock / 64));
+            // Children (next level's parts and tiles), split records, id offsets.
+            const bool plan1 = parts_ub <= kPlanParts;
+            if (plan1) {
+                timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                    k_children_plan<<<1, kPlanThreads, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
+                                                               (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
+                                                               &ctr->tiles_total, &ctr->n_parts[nxt], &ctr->n_tiles[nxt]);
+                });
+            } else {
+                timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                    k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                        parts, np, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
+                });
+                scan_u64(cx, st, (uint64_t *)cx.nchild.ptr, np, parts_ub, &ctr->parts_total);
+                scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
+            }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_tile_left<<<gt, kBlock, 0, st>>>(parts, tiles, ntl, ci[0], ci[1], ci[2], (uint32_t *)cx.below.ptr,
-                                                  tile_bins, (uint64_t *)cx.tile_left.ptr);
+                k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                    parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
+                    (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
+                    node_base[std::min(level + 1, L)], heap_split, insts);
+                if (!plan1)
+                    k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
+                                                  &ctr->n_tiles[nxt]);
             });
-            scan_u64(cx, st, (uint64_t *)cx.tile_left.ptr, ntl, tiles_ub, nullptr);
-            const unsigned g = persistent_grid(cx, k_partition_tiles<D, WS, Acc>, tiles_ub);
-            timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-                k_partition_tiles<D, WS, Acc><<<g, kBlock, 0, st>>>(
-                    parts, (const Part *)cx.parts[nxt].ptr, tiles, ntl, (const uint64_t *)cx.tile_left.ptr, ci[0],
-                    ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin,
-                    (uint32_t *)cx.below.ptr);
-            });
-        }
+            if (level + 1 == L) break;   // every part left is at its last level
 
-        cur = nxt;
-        parts_ub = std::max<uint64_t>(B.parts_ub[level + 1], 1);
-        tiles_ub = std::min<uint64_t>(tiles_cap, parts_ub + (n + kTile - 1) / kTile);
-        search(cur, level + 1, parts_ub, tiles_ub, 0, false);
+            // Partition of this level, fused with the children's first pass.
+            float *const *ci = cs[cslot(level)];
+            float *const *co = cs[cslot(level + 1)];
+            if (lookback) {
+                const uint32_t epoch = ++cx.epoch;
+                unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
+                if (level > 0 && level % kMaxTickets == 0)
+                    check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
+                const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
+                timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                    k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                        parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0],
+                        ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
+                });
+            } else {
+                const unsigned gt = persistent_grid(cx, k_tile_left, (tiles_ub + kBlock / 64 - 1) / (kBlock / 64));
+                timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                    k_tile_left<<<gt, kBlock, 0, st>>>(parts, tiles, ntl, ci[0], ci[1], ci[2], (uint32_t *)cx.below.ptr,
+                                                      tile_bins, (uint64_t *)cx.tile_left.ptr);
+                });
+                scan_u64(cx, st, (uint64_t *)cx.tile_left.ptr, ntl, tiles_ub, nullptr);
+                const unsigned g = persistent_grid(cx, k_partition_tiles<D, WS, Acc>, tiles_ub);
+                timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                    k_partition_tiles<D, WS, Acc><<<g, kBlock, 0, st>>>(
+                        parts, (const Part *)cx.parts[nxt].ptr, tiles, ntl, (const uint64_t *)cx.tile_left.ptr, ci[0],
+                        ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin,
+                        (uint32_t *)cx.below.ptr);
+                });
+            }
+
+            cur = nxt;
+            parts_ub = std::max<uint64_t>(B.parts_ub[level + 1], 1);
+            tiles_ub = std::min<uint64_t>(tiles_cap, parts_ub + (n + kTile - 1) / kTile);
+            search(cur, level + 1, parts_ub, tiles_ub, 0, false);
+        }
     }
 
     // Part ids in input order (:589-601, :698-702).
@@ -699,7 +814,7 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
+                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &below2, &tile_bins2, &tile4,
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                            &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
@@ -708,7 +823,7 @@ uint64_t Context::device_bytes() const {
 
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &tiles[0], &tiles[1],
+                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &below2, &tile_bins2, &tile4,
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                      &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();

@@ -739,8 +739,8 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             C.kstar = -1;
             C.wl = 0;
             C.cnt_left = 0;
-            C.tile_begin = (uint32_t)to;
-            C.tile_count = nt;
+            C.tile_begin = next_tiles ? (uint32_t)to : 0u;   // children searched in place own no tiles
+            C.tile_count = next_tiles ? nt : 0u;
             C.child[0] = -1;
             C.child[1] = -1;
             C.inst = P.inst;
@@ -750,7 +750,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             C.tol = P.tol;
             next[co] = C;
         }
-        for (uint32_t k = threadIdx.x; k < nt; k += 64) {
+        for (uint32_t k = threadIdx.x; next_tiles && k < nt; k += 64) {
             Tile U;
             U.begin = b + (uint64_t)k * kTile;
             uint64_t rem = cnt - (uint64_t)k * kTile;

@@ -1,0 +1,493 @@
+// rcb_pairs.h -- two levels per data move.
+//
+// The points of a part are moved once for two bisection levels: after level
+// l decides a part p, its two children are left in place ("virtual": their
+// points stay interleaved in p's range) and searched there; after level l+1
+// decides them, every point moves straight to its grandchild (4 runs per
+// tile).  Per pair of levels this reads 12 B/point for the children's first
+// pass and moves the payload once, instead of moving it twice.
+//
+//   k_child_first   children's first pass over the parent's tiles: side of
+//                   the parent's split, bins at the children's first
+//                   candidate, per-tile left counts of the parent and per
+//                   (tile, side) counts below that candidate
+//   k_hist_virtual  later passes of the children over the parent's tiles,
+//                   with per-(tile, side) bin counts
+//   k_tile_count4   per tile, the points of each grandchild
+//   k_partition4    the move parent -> grandchildren, LDS-staged runs, and
+//                   the grandchildren's first pass
+#pragma once
+
+#include "rcb_kernels.h"
+
+namespace coupe_hip {
+
+// A part of level l whose children are searched in place and whose points
+// move to grandchildren: neither done nor at its instance's last two levels.
+__device__ __forceinline__ bool pair_active(const Part &P) {
+    return P.status != kDone && P.depth + 2 < P.levels;
+}
+// A part of level l whose children are searched (in place) at all.
+__device__ __forceinline__ bool children_searched(const Part &P) {
+    return P.status != kDone && P.depth + 1 < P.levels;
+}
+
+// Block-wide sum of one value per thread (all threads get it).
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *sh4) {
+    v = wave_sum(v);
+    if (lane_id() == 0) sh4[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) s += sh4[i];
+    __syncthreads();
+    return s;
+}
+
+template <int D, typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+              const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
+              const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
+              uint32_t *__restrict__ bmin, uint32_t *__restrict__ below2, uint64_t *__restrict__ tile_left) {
+    __shared__ ChildBinsShared<Acc> sh;
+    __shared__ uint32_t s4[kBlock / 64];
+    const int tid = threadIdx.x;
+    const uint32_t nt = *n_tiles;
+    ChildBins<Acc> r;
+    r.reset();
+    uint32_t cur = 0xffffffffu;
+    int32_t ch0 = -1, ch1 = -1;
+    const float *ca = c0, *cb = c1;
+    float t2 = 0.f, split = 0.f;
+    bool all_left = false;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        if (!children_searched(P)) continue;
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
+            cur = T.part;
+            ch0 = P.child[0];
+            ch1 = P.child[1];
+            const int ax = (int)P.axis, a2 = (int)((P.axis + 1) % D);
+            ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+            cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+            t2 = (sel3(a2, P.lo[0], P.lo[1], P.lo[2]) + sel3(a2, P.hi[0], P.hi[1], P.hi[2])) / 2.0f;
+            split = P.split;
+            all_left = P.status == kAllLeft;
+        }
+        uint32_t nleft = 0, b0 = 0, b1 = 0;
+        for (uint64_t i = T.begin + tid; i < T.begin + T.count; i += kBlock) {
+            const float xa = ca[i];
+            const float xb = cb[i];
+            const bool right = !all_left && (xa >= split);
+            const bool bel = !(xb >= t2);
+            r.add(right, bel, AccOps<Acc>::of(w[i]), f32_key(xb));
+            nleft += right ? 0u : 1u;
+            b0 += (!right && bel) ? 1u : 0u;
+            b1 += (right && bel) ? 1u : 0u;
+        }
+        nleft = block_sum_u32(nleft, s4);
+        b0 = block_sum_u32(b0, s4);
+        b1 = block_sum_u32(b1, s4);
+        if (tid == 0) {
+            tile_left[t] = nleft;
+            below2[2 * (uint64_t)t] = b0;
+            below2[2 * (uint64_t)t + 1] = b1;
+        }
+    }
+    if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
+}
+
+// Later passes of in-place children: LDS bins of both children of the tile's
+// parent, per-(tile, side) bin counts for the move's offsets.
+template <int D, typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+               const Part *__restrict__ children, const float *__restrict__ c0, const float *__restrict__ c1,
+               const float *__restrict__ c2, const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt,
+               Acc *__restrict__ bw, uint32_t *__restrict__ bmin, const uint32_t *gate,
+               uint16_t *__restrict__ tile_bins2) {
+    if (gate && *gate == 0) return;
+    __shared__ float cand[2][kMaxBins];
+    __shared__ uint32_t scnt[2][kMaxBins];
+    __shared__ Acc sw[2][kMaxBins];
+    __shared__ uint32_t smin[2][kMaxBins];
+    const int tid = threadIdx.x;
+    const uint32_t nt = *n_tiles;
+    uint32_t cur = 0xffffffffu;
+    bool act[2] = {false, false};
+    int32_t ch[2] = {-1, -1};
+    const float *ca = c0, *cb = c1;
+    float split = 0.f;
+    bool all_left = false;
+    uint32_t before0 = 0, before1 = 0;
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        if (!children_searched(P)) continue;
+        if (T.part != cur) {
+            before0 = before1 = 0;
+            __syncthreads();
+            for (int s = 0; s < 2; s++)
+                if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin);
+            __syncthreads();
+            cur = T.part;
+            {
+                const int ax = (int)P.axis, a2 = (int)((P.axis + 1) % D);
+                ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+                cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+            }
+            split = P.split;
+            all_left = P.status == kAllLeft;
+            for (int s = 0; s < 2; s++) {
+                ch[s] = P.child[s];
+                act[s] = ch[s] >= 0 && children[ch[s]].status == kSearching;
+                if (act[s]) {
+                    const Part &C = children[ch[s]];
+                    for (int p = tid; p < nb - 1; p += kBlock) cand[s][p] = candidate(C.blo, C.bhi, p, nb);
+                    for (int j = tid; j < nb; j += kBlock) { scnt[s][j] = 0; sw[s][j] = (Acc)0; smin[s][j] = 0xffffffffu; }
+                }
+            }
+            __syncthreads();
+        }
+        if (!act[0] && !act[1]) continue;
+        for (uint64_t i = T.begin + tid; i < T.begin + T.count; i += kBlock) {
+            const int s = (!all_left && (ca[i] >= split)) ? 1 : 0;
+            if (!act[s]) continue;
+            const float x = cb[i];
+            const int b = bin_of(cand[s], nb, x);
+            atomicAdd(&scnt[s][b], 1u);
+            atomic_add_acc(&sw[s][b], AccOps<Acc>::of(w[i]));
+            atomicMin(&smin[s][b], f32_key(x));
+        }
+        __syncthreads();
+        if (tid < nb) {
+            const uint32_t a0 = scnt[0][tid], a1 = scnt[1][tid];
+            uint16_t *tb = tile_bins2 + (uint64_t)t * 2 * kMaxBins;
+            if (act[0]) tb[tid] = (uint16_t)(a0 - before0);
+            if (act[1]) tb[kMaxBins + tid] = (uint16_t)(a1 - before1);
+            before0 = a0;
+            before1 = a1;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int s = 0; s < 2; s++)
+        if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin);
+}
+
+// Per tile of a parent, the points of each grandchild (q = 2 side + side of
+// the child's split), from the counts the passes kept when they can.
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+              const Part *__restrict__ children, const float *__restrict__ c0, const float *__restrict__ c1,
+              const float *__restrict__ c2, const uint64_t *__restrict__ tile_left,
+              const uint32_t *__restrict__ below2, const uint16_t *__restrict__ tile_bins2, uint64_t *__restrict__ q0,
+              uint64_t *__restrict__ q1, uint64_t *__restrict__ q2, uint64_t *__restrict__ q3) {
+    const uint32_t nt = *n_tiles;
+    const int lane = lane_id();
+    const uint32_t waves = gridDim.x * (kBlock / 64);
+    for (uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        uint64_t out[4] = {0, 0, 0, 0};
+        if (pair_active(P)) {
+            const uint64_t cnt[2] = {tile_left[t], T.count - tile_left[t]};
+            for (int s = 0; s < 2; s++) {
+                if (cnt[s] == 0 || P.child[s] < 0) continue;
+                const Part &C = children[P.child[s]];
+                uint64_t left;
+                if (C.status == kAllLeft) {
+                    left = cnt[s];
+                } else if (C.counts_ready == kCountsBelow) {
+                    left = below2[2 * (uint64_t)t + s];
+                } else if (C.counts_ready == kCountsBins) {
+                    const uint16_t *tb = tile_bins2 + ((uint64_t)t * 2 + s) * kMaxBins;
+                    uint32_t k = 0;
+                    for (int j = lane; j <= C.kstar; j += 64) k += tb[j];
+                    left = wave_sum(k);
+                } else {
+                    const int ax = (int)P.axis, a2 = (int)C.axis;
+                    const float *ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+                    const float *cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+                    const bool all_left = P.status == kAllLeft;
+                    uint32_t k = 0;
+                    for (uint64_t i = T.begin + lane; i < T.begin + T.count; i += 64) {
+                        const int si = (!all_left && (ca[i] >= P.split)) ? 1 : 0;
+                        k += (si == s && !(cb[i] >= C.split)) ? 1u : 0u;
+                    }
+                    left = wave_sum(k);
+                }
+                out[2 * s] = left;
+                out[2 * s + 1] = cnt[s] - left;
+            }
+        }
+        if (lane == 0) {
+            q0[t] = out[0];
+            q1[t] = out[1];
+            q2[t] = out[2];
+            q3[t] = out[3];
+        }
+    }
+}
+
+// Register bins of the grandchildren's first pass (4 parts x {count and
+// weight below the candidate, smallest key at or above it}).
+template <typename Acc>
+struct GrandBins {
+    uint32_t cnt[4], mn[4];
+    Acc w[4];
+    __device__ __forceinline__ void reset() {
+#pragma unroll
+        for (int q = 0; q < 4; q++) { cnt[q] = 0; mn[q] = 0xffffffffu; w[q] = (Acc)0; }
+    }
+    __device__ __forceinline__ void add(int q, bool below, Acc wv, uint32_t key) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool mine = q == j;
+            cnt[j] += (mine && below) ? 1u : 0u;
+            w[j] += (mine && below) ? wv : (Acc)0;
+            const uint32_t k = (mine && !below) ? key : 0xffffffffu;
+            mn[j] = k < mn[j] ? k : mn[j];
+        }
+    }
+};
+
+template <typename Acc>
+struct GrandBinsShared {
+    uint32_t cnt[kBlock / 64][4], mn[kBlock / 64][4];
+    Acc w[kBlock / 64][4];
+};
+
+template <typename Acc>
+__device__ __forceinline__ void flush_grand_bins(GrandBins<Acc> &r, GrandBinsShared<Acc> &sh, const int32_t (&g)[4],
+                                                 uint32_t *__restrict__ gcnt, Acc *__restrict__ gw,
+                                                 uint32_t *__restrict__ gmn) {
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint32_t c = wave_sum(r.cnt[q]);
+        const Acc x = wave_sum(r.w[q]);
+        const uint32_t m = wave_min(r.mn[q]);
+        if (lane == 0) { sh.cnt[wid][q] = c; sh.w[wid][q] = x; sh.mn[wid][q] = m; }
+    }
+    __syncthreads();
+    const int q = threadIdx.x;
+    if (q < 4 && g[q] >= 0) {
+        uint32_t c = 0, m = 0xffffffffu;
+        Acc x = (Acc)0;
+#pragma unroll
+        for (int v = 0; v < kBlock / 64; v++) {
+            c += sh.cnt[v][q];
+            x += sh.w[v][q];
+            m = sh.mn[v][q] < m ? sh.mn[v][q] : m;
+        }
+        const uint64_t b = (uint64_t)g[q] * 2;
+        if (c) {
+            atomicAdd(&gcnt[b], c);
+            atomic_add_acc(&gw[b], x);
+        }
+        if (m != 0xffffffffu) atomicMin(&gmn[b + 1], m);
+    }
+    __syncthreads();
+}
+
+// The move of a pair of levels: every point of an active parent goes to its
+// grandchild; stable within each grandchild.  prefix4[q] are the exclusive
+// scans of the per-tile counts of grandchild slot q.
+template <int D, typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
+             const Tile *__restrict__ tiles, const uint32_t *n_tiles, const uint64_t *__restrict__ p0,
+             const uint64_t *__restrict__ p1, const uint64_t *__restrict__ p2, const uint64_t *__restrict__ p3,
+             const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
+             const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
+             WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
+             uint32_t *__restrict__ bmin) {
+    __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
+    __shared__ uint32_t wtot[4][kItems * (kBlock / 64)];
+    __shared__ uint32_t s_run[5];
+    __shared__ GrandBinsShared<Acc> sh;
+    float *sf0 = reinterpret_cast<float *>(stage);
+    float *sf1 = sf0 + kTile;
+    WS *sw = reinterpret_cast<WS *>(stage);
+    WS *sw1 = reinterpret_cast<WS *>(sf1);
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nt = *n_tiles;
+    GrandBins<Acc> r;
+    r.reset();
+    uint32_t cur = 0xffffffffu;
+    int32_t g[4] = {-1, -1, -1, -1};
+    float gsplit[2] = {0.f, 0.f};
+    bool gall[2] = {true, true};
+    float t3[4] = {0.f, 0.f, 0.f, 0.f};
+    uint64_t gbeg[4] = {0, 0, 0, 0};
+    int ax = 0, a2 = 0, a3 = 0;
+    float split = 0.f;
+    bool all_left = false;
+    uint32_t ptile = 0;
+
+    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Part &P = parts[T.part];
+        if (!pair_active(P)) continue;
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) { flush_grand_bins(r, sh, g, bcnt, bw, bmin); r.reset(); }
+            cur = T.part;
+            ax = (int)P.axis;
+            a2 = (int)((P.axis + 1) % D);
+            a3 = (int)((P.axis + 2) % D);
+            split = P.split;
+            all_left = P.status == kAllLeft;
+            ptile = P.tile_begin;
+            for (int s = 0; s < 2; s++) {
+                const int32_t ci = P.child[s];
+                gsplit[s] = 0.f;
+                gall[s] = true;
+                g[2 * s] = g[2 * s + 1] = -1;
+                if (ci >= 0) {
+                    const Part &C = children[ci];
+                    gsplit[s] = C.split;
+                    gall[s] = C.status == kAllLeft;
+                    g[2 * s] = C.child[0];
+                    g[2 * s + 1] = C.child[1];
+                }
+            }
+            for (int q = 0; q < 4; q++) {
+                t3[q] = 0.f;
+                gbeg[q] = 0;
+                if (g[q] >= 0) {
+                    const Part &G = grand[g[q]];
+                    t3[q] = (G.blo + G.bhi) / 2.0f;   // the grandchild's first candidate (:472)
+                    gbeg[q] = G.begin;
+                }
+            }
+        }
+        const uint64_t end = T.begin + T.count;
+        float x0[kItems], x1[kItems], x2[kItems];
+        WS wv[kItems];
+        uint32_t qq[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
+            const bool valid = p < end;
+            x0[k] = valid ? i0[p] : 0.f;
+            x1[k] = valid ? i1[p] : 0.f;
+            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
+            wv[k] = valid ? iw[p] : (WS)0;
+        }
+        uint32_t pre[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
+            const int s1 = (!all_left && (sel3(ax, x0[k], x1[k], x2[k]) >= split)) ? 1 : 0;
+            const float xb = sel3(a2, x0[k], x1[k], x2[k]);
+            const bool go_right = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
+            const uint32_t q = (uint32_t)(2 * s1 + (go_right ? 1 : 0));
+            qq[k] = p < end ? q : 4u;
+            uint32_t rank = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint64_t bal = __ballot(qq[k] == (uint32_t)j);
+                if (lane == 0) wtot[j][k * (kBlock / 64) + wid] = __popcll(bal);
+                rank = qq[k] == (uint32_t)j ? __popcll(bal & lt_mask) : rank;
+            }
+            pre[k] = rank;
+        }
+        __syncthreads();
+        if (tid < 64) {   // exclusive scans of the counts of each grandchild slot, item-major
+            constexpr int kE = kItems * (kBlock / 64);
+            static_assert(kE <= 64, "one wavefront scans the counts");
+            uint32_t run = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t c = tid < kE ? wtot[j][tid] : 0u;
+                const uint32_t inc = wave_incl_add(c);
+                if (tid < kE) wtot[j][tid] = run + inc - c;
+                run += __shfl(inc, kE - 1, 64);
+                if (tid == 0) s_run[j + 1] = run;
+            }
+            if (tid == 0) s_run[0] = 0;
+        }
+        __syncthreads();
+        const uint32_t r1 = s_run[1], r2 = s_run[2], r3 = s_run[3];
+        uint64_t dbase[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t *pj = j == 0 ? p0 : (j == 1 ? p1 : (j == 2 ? p2 : p3));
+            dbase[j] = gbeg[j] + (pj[t] - pj[ptile]);
+        }
+        const uint32_t cnt = T.count;
+        uint32_t cp[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t q = qq[k] < 4 ? qq[k] : 0u;
+            cp[k] = wtot[q][k * (kBlock / 64) + wid] + pre[k];   // already includes the run start of q
+        }
+#define COUPE_STORE4(EXPR)                                                               \
+    _Pragma("unroll") for (int k = 0; k < kItems; k++) {                                 \
+        const uint32_t pos = k * kBlock + tid;                                           \
+        if (pos < cnt) {                                                                 \
+            const uint32_t q = pos >= r3 ? 3u : (pos >= r2 ? 2u : (pos >= r1 ? 1u : 0u)); \
+            const uint32_t rs = q == 3 ? r3 : (q == 2 ? r2 : (q == 1 ? r1 : 0u));        \
+            const uint64_t d = (q == 3 ? dbase[3] : (q == 2 ? dbase[2] : (q == 1 ? dbase[1] : dbase[0]))) + (pos - rs); \
+            if (d < n_total) { EXPR; }                                                   \
+        }                                                                                \
+    }
+#pragma unroll
+        for (int k = 0; k < kItems; k++)
+            if (k * kBlock + tid < (int)cnt) { sf0[cp[k]] = x0[k]; sf1[cp[k]] = x1[k]; }
+        __syncthreads();
+        COUPE_STORE4(o0[d] = sf0[pos]; o1[d] = sf1[pos])
+        __syncthreads();
+        if constexpr (sizeof(WS) == 4) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) {
+                    if constexpr (D == 3) sf0[cp[k]] = x2[k];
+                    sw1[cp[k]] = wv[k];
+                }
+            __syncthreads();
+            if constexpr (D == 3) {
+                COUPE_STORE4(o2[d] = sf0[pos]; ow[d] = sw1[pos])
+            } else {
+                COUPE_STORE4(ow[d] = sw1[pos])
+            }
+            __syncthreads();
+        } else {
+            if constexpr (D == 3) {
+#pragma unroll
+                for (int k = 0; k < kItems; k++)
+                    if (k * kBlock + tid < (int)cnt) sf0[cp[k]] = x2[k];
+                __syncthreads();
+                COUPE_STORE4(o2[d] = sf0[pos])
+                __syncthreads();
+            }
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) sw[cp[k]] = wv[k];
+            __syncthreads();
+            COUPE_STORE4(ow[d] = sw[pos])
+            __syncthreads();
+        }
+#undef COUPE_STORE4
+        // Grandchildren's first pass (their split counts are read again by
+        // the next pair's first pass, so no per-tile counts are kept here).
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            if (qq[k] < 4) {
+                const uint32_t q = qq[k];
+                const float xc = sel3(a3, x0[k], x1[k], x2[k]);
+                const float tq = q == 3 ? t3[3] : (q == 2 ? t3[2] : (q == 1 ? t3[1] : t3[0]));
+                r.add((int)q, !(xc >= tq), AccOps<Acc>::of(wv[k]), f32_key(xc));
+            }
+        }
+    }
+    if (cur != 0xffffffffu) flush_grand_bins(r, sh, g, bcnt, bw, bmin);
+}
+
+}  // namespace coupe_hip

@@ -171,6 +171,33 @@ def test_split_search_queue_depths(oracle_mod, monkeypatch, queued):
         ctx.close()
 
 
+@pytest.mark.parametrize("pairs", ["0", "1"])
+def test_move_every_level_or_every_other(oracle_mod, monkeypatch, pairs):
+    """COUPE_HIP_PAIRS=0 moves the points at every level, 1 (the default)
+    once per two levels with the children searched in place; same ids."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_PAIRS", pairs)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(900)
+        for q in range(8):
+            n = int(rng.integers(1_000, 150_000))
+            dim = 2 + q % 2
+            pts = [rng.random((n, dim)), rng.normal(0, 1, (n, dim)) ** 3,
+                   np.round(rng.random((n, dim)) * 20) / 20][q % 3]
+            w = rng.integers(1, 1000, n).astype([np.int32, np.int64, np.float64][q % 3])
+            iters, tol = int(rng.integers(1, 14)), float(rng.choice([0.0, 0.01, 0.05]))
+            say(f"pairs {pairs}, case {q}: n={n} d={dim} L={iters} tol={tol}")
+            out = torch.zeros(n, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"pairs {pairs} case {q}"
+    finally:
+        ctx.close()
+
+
 def test_edge_cases(oracle_mod):
     rng = np.random.default_rng(7)
     one = rng.random((1, 3))
