@@ -1001,12 +1001,15 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
         uint32_t pre[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
+            // clamped, unconditional loads: all of them issue back to back
+            // (the values past the tile's end are never used)
             const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const bool valid = p < end;
-            x0[k] = valid ? i0[p] : 0.f;
-            x1[k] = valid ? i1[p] : 0.f;
-            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
-            wv[k] = valid ? iw[p] : (WS)0;
+            const uint64_t c = p < end ? p : end - 1;
+            x0[k] = i0[c];
+            x1[k] = i1[c];
+            if constexpr (D == 3) x2[k] = i2[c];
+            else x2[k] = 0.f;
+            wv[k] = iw[c];
         }
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
@@ -1228,12 +1231,15 @@ k_partition_tiles(const Part *__restrict__ parts, const Part *__restrict__ next_
         uint32_t pre[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
+            // clamped, unconditional loads: all of them issue back to back
+            // (the values past the tile's end are never used)
             const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const bool valid = p < end;
-            x0[k] = valid ? i0[p] : 0.f;
-            x1[k] = valid ? i1[p] : 0.f;
-            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
-            wv[k] = valid ? iw[p] : (WS)0;
+            const uint64_t c = p < end ? p : end - 1;
+            x0[k] = i0[c];
+            x1[k] = i1[c];
+            if constexpr (D == 3) x2[k] = i2[c];
+            else x2[k] = 0.f;
+            wv[k] = iw[c];
         }
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
@@ -1461,10 +1467,10 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const uint32_t p = k * kBlock + threadIdx.x;
-            const bool valid = p < T.count;
-            x0[k] = valid ? c0[T.begin + p] : 0.f;
-            x1[k] = valid ? c1[T.begin + p] : 0.f;
-            x2[k] = (D == 3 && valid) ? c2[T.begin + p] : 0.f;
+            const uint64_t c = T.begin + (p < T.count ? p : T.count - 1);   // clamped, unconditional loads
+            x0[k] = c0[c];
+            x1[k] = c1[c];
+            x2[k] = D == 3 ? c2[c] : 0.f;
             h[k] = 0;
         }
         int a = 0;

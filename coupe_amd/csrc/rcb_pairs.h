@@ -51,7 +51,7 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
               const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
               uint32_t *__restrict__ bmin, uint32_t *__restrict__ below2, uint64_t *__restrict__ tile_left) {
     __shared__ ChildBinsShared<Acc> sh;
-    __shared__ uint32_t s4[kBlock / 64];
+    __shared__ uint32_t s4[3 * (kBlock / 64)];
     const int tid = threadIdx.x;
     const uint32_t nt = *n_tiles;
     ChildBins<Acc> r;
@@ -77,25 +77,45 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
             split = P.split;
             all_left = P.status == kAllLeft;
         }
-        uint32_t nleft = 0, b0 = 0, b1 = 0;
-        for (uint64_t i = T.begin + tid; i < T.begin + T.count; i += kBlock) {
-            const float xa = ca[i];
-            const float xb = cb[i];
-            const bool right = !all_left && (xa >= split);
-            const bool bel = !(xb >= t2);
-            r.add(right, bel, AccOps<Acc>::of(w[i]), f32_key(xb));
-            nleft += right ? 0u : 1u;
-            b0 += (!right && bel) ? 1u : 0u;
-            b1 += (right && bel) ? 1u : 0u;
+        // all loads of the tile first, then the points
+        float xa[kItems], xb[kItems];
+        WS wk[kItems];
+        const uint64_t end = T.begin + T.count;
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint64_t i = T.begin + (uint64_t)k * kBlock + tid;
+            const uint64_t c = i < end ? i : end - 1;   // clamped: loads issue back to back
+            xa[k] = ca[c];
+            xb[k] = cb[c];
+            wk[k] = w[c];
         }
-        nleft = block_sum_u32(nleft, s4);
-        b0 = block_sum_u32(b0, s4);
-        b1 = block_sum_u32(b1, s4);
+        uint32_t nleft = 0, b0 = 0, b1 = 0;   // wave-uniform counts from ballots
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const bool valid = T.begin + (uint64_t)k * kBlock + tid < end;
+            const bool right = !all_left && (xa[k] >= split);
+            const bool bel = !(xb[k] >= t2);
+            if (valid) r.add(right, bel, AccOps<Acc>::of(wk[k]), f32_key(xb[k]));
+            const uint64_t bv = __ballot(valid), br = __ballot(right), bb = __ballot(bel);
+            nleft += __popcll(bv & ~br);
+            b0 += __popcll(bv & ~br & bb);
+            b1 += __popcll(bv & br & bb);
+        }
+        if (lane_id() == 0) {
+            s4[(tid >> 6) * 3] = nleft;
+            s4[(tid >> 6) * 3 + 1] = b0;
+            s4[(tid >> 6) * 3 + 2] = b1;
+        }
+        __syncthreads();
         if (tid == 0) {
-            tile_left[t] = nleft;
-            below2[2 * (uint64_t)t] = b0;
-            below2[2 * (uint64_t)t + 1] = b1;
+            uint32_t a = 0, c = 0, e = 0;
+#pragma unroll
+            for (int v = 0; v < kBlock / 64; v++) { a += s4[3 * v]; c += s4[3 * v + 1]; e += s4[3 * v + 2]; }
+            tile_left[t] = a;
+            below2[2 * (uint64_t)t] = c;
+            below2[2 * (uint64_t)t + 1] = e;
         }
+        __syncthreads();
     }
     if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
 }
@@ -234,8 +254,9 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     }
 }
 
-// Register bins of the grandchildren's first pass (4 parts x {count and
-// weight below the candidate, smallest key at or above it}).
+// Bins of the grandchildren's first pass: the count below the candidate is
+// accumulated from ballots (wave-uniform), the weight below and the smallest
+// key at or above it per lane.
 template <typename Acc>
 struct GrandBins {
     uint32_t cnt[4], mn[4];
@@ -243,16 +264,6 @@ struct GrandBins {
     __device__ __forceinline__ void reset() {
 #pragma unroll
         for (int q = 0; q < 4; q++) { cnt[q] = 0; mn[q] = 0xffffffffu; w[q] = (Acc)0; }
-    }
-    __device__ __forceinline__ void add(int q, bool below, Acc wv, uint32_t key) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const bool mine = q == j;
-            cnt[j] += (mine && below) ? 1u : 0u;
-            w[j] += (mine && below) ? wv : (Acc)0;
-            const uint32_t k = (mine && !below) ? key : 0xffffffffu;
-            mn[j] = k < mn[j] ? k : mn[j];
-        }
     }
 };
 
@@ -269,10 +280,9 @@ __device__ __forceinline__ void flush_grand_bins(GrandBins<Acc> &r, GrandBinsSha
     const int lane = lane_id(), wid = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const uint32_t c = wave_sum(r.cnt[q]);
         const Acc x = wave_sum(r.w[q]);
         const uint32_t m = wave_min(r.mn[q]);
-        if (lane == 0) { sh.cnt[wid][q] = c; sh.w[wid][q] = x; sh.mn[wid][q] = m; }
+        if (lane == 0) { sh.cnt[wid][q] = r.cnt[q]; sh.w[wid][q] = x; sh.mn[wid][q] = m; }
     }
     __syncthreads();
     const int q = threadIdx.x;
@@ -296,8 +306,8 @@ __device__ __forceinline__ void flush_grand_bins(GrandBins<Acc> &r, GrandBinsSha
 }
 
 // The move of a pair of levels: every point of an active parent goes to its
-// grandchild; stable within each grandchild.  prefix4[q] are the exclusive
-// scans of the per-tile counts of grandchild slot q.
+// grandchild; stable within each grandchild.  p0..p3 are the exclusive scans
+// of the per-tile counts of grandchild slots q = 0..3.
 template <int D, typename WS, typename Acc>
 __global__ void __launch_bounds__(kBlock)
 k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
@@ -310,6 +320,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
     __shared__ uint32_t wtot[4][kItems * (kBlock / 64)];
     __shared__ uint32_t s_run[5];
+    __shared__ uint64_t s_dst[4];
     __shared__ GrandBinsShared<Acc> sh;
     float *sf0 = reinterpret_cast<float *>(stage);
     float *sf1 = sf0 + kTile;
@@ -324,8 +335,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     int32_t g[4] = {-1, -1, -1, -1};
     float gsplit[2] = {0.f, 0.f};
     bool gall[2] = {true, true};
-    float t3[4] = {0.f, 0.f, 0.f, 0.f};
-    uint64_t gbeg[4] = {0, 0, 0, 0};
+    float t3[2] = {0.f, 0.f};   // grandchildren's first candidate, per side of the parent's split
     int ax = 0, a2 = 0, a3 = 0;
     float split = 0.f;
     bool all_left = false;
@@ -356,47 +366,71 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     g[2 * s] = C.child[0];
                     g[2 * s + 1] = C.child[1];
                 }
+                // the grandchildren of one side share their box along a3
+                // (in 3D the same for both sides): the first candidate (:472)
+                const int32_t gq = g[2 * s] >= 0 ? g[2 * s] : g[2 * s + 1];
+                t3[s] = gq >= 0 ? (grand[gq].blo + grand[gq].bhi) / 2.0f : 0.f;
             }
-            for (int q = 0; q < 4; q++) {
-                t3[q] = 0.f;
-                gbeg[q] = 0;
-                if (g[q] >= 0) {
-                    const Part &G = grand[g[q]];
-                    t3[q] = (G.blo + G.bhi) / 2.0f;   // the grandchild's first candidate (:472)
-                    gbeg[q] = G.begin;
-                }
-            }
+        }
+        if (tid < 4) {
+            const int32_t gq = g[tid];
+            const uint64_t *pj = tid == 0 ? p0 : (tid == 1 ? p1 : (tid == 2 ? p2 : p3));
+            s_dst[tid] = gq >= 0 ? grand[gq].begin + (pj[t] - pj[ptile]) : 0;
         }
         const uint64_t end = T.begin + T.count;
         float x0[kItems], x1[kItems], x2[kItems];
         WS wv[kItems];
-        uint32_t qq[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
+            // clamped, unconditional loads: all of them issue back to back
+            // (the values past the tile's end are never used)
             const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const bool valid = p < end;
-            x0[k] = valid ? i0[p] : 0.f;
-            x1[k] = valid ? i1[p] : 0.f;
-            x2[k] = (D == 3 && valid) ? i2[p] : 0.f;
-            wv[k] = valid ? iw[p] : (WS)0;
+            const uint64_t c = p < end ? p : end - 1;
+            x0[k] = i0[c];
+            x1[k] = i1[c];
+            if constexpr (D == 3) x2[k] = i2[c];
+            else x2[k] = 0.f;
+            wv[k] = iw[c];
         }
         uint32_t pre[kItems];
+        uint8_t qq[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
-            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const int s1 = (!all_left && (sel3(ax, x0[k], x1[k], x2[k]) >= split)) ? 1 : 0;
+            const bool valid = T.begin + (uint64_t)k * kBlock + tid < end;
+            const bool s1 = !all_left && (sel3(ax, x0[k], x1[k], x2[k]) >= split);
             const float xb = sel3(a2, x0[k], x1[k], x2[k]);
-            const bool go_right = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
-            const uint32_t q = (uint32_t)(2 * s1 + (go_right ? 1 : 0));
-            qq[k] = p < end ? q : 4u;
-            uint32_t rank = 0;
+            const bool s2 = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
+            const uint32_t q = (s1 ? 2u : 0u) + (s2 ? 1u : 0u);
+            qq[k] = valid ? (uint8_t)q : (uint8_t)4;
+            // slot masks from three ballots (wave-uniform)
+            const uint64_t bv = __ballot(valid), b1 = __ballot(s1), b2 = __ballot(s2);
+            const uint64_t m0 = bv & ~b1 & ~b2, m1 = bv & ~b1 & b2, m2 = bv & b1 & ~b2, m3 = bv & b1 & b2;
+            if (lane == 0) {
+                const int e = k * (kBlock / 64) + wid;
+                wtot[0][e] = __popcll(m0);
+                wtot[1][e] = __popcll(m1);
+                wtot[2][e] = __popcll(m2);
+                wtot[3][e] = __popcll(m3);
+            }
+            const uint64_t mine = s1 ? (s2 ? m3 : m2) : (s2 ? m1 : m0);
+            pre[k] = __popcll(mine & lt_mask);
+            // grandchildren's first pass
+            const float xc = sel3(a3, x0[k], x1[k], x2[k]);
+            const bool bel = !(xc >= (s1 ? t3[1] : t3[0]));
+            const uint64_t bb = __ballot(bel);
+            r.cnt[0] += __popcll(m0 & bb);
+            r.cnt[1] += __popcll(m1 & bb);
+            r.cnt[2] += __popcll(m2 & bb);
+            r.cnt[3] += __popcll(m3 & bb);
+            const Acc wa = AccOps<Acc>::of(wv[k]);
+            const uint32_t key = f32_key(xc);
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const uint64_t bal = __ballot(qq[k] == (uint32_t)j);
-                if (lane == 0) wtot[j][k * (kBlock / 64) + wid] = __popcll(bal);
-                rank = qq[k] == (uint32_t)j ? __popcll(bal & lt_mask) : rank;
+                const bool mj = valid && q == (uint32_t)j;
+                r.w[j] += (mj && bel) ? wa : (Acc)0;
+                const uint32_t kj = (mj && !bel) ? key : 0xffffffffu;
+                r.mn[j] = kj < r.mn[j] ? kj : r.mn[j];
             }
-            pre[k] = rank;
         }
         __syncthreads();
         if (tid < 64) {   // exclusive scans of the counts of each grandchild slot, item-major
@@ -414,29 +448,23 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             if (tid == 0) s_run[0] = 0;
         }
         __syncthreads();
-        const uint32_t r1 = s_run[1], r2 = s_run[2], r3 = s_run[3];
-        uint64_t dbase[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const uint64_t *pj = j == 0 ? p0 : (j == 1 ? p1 : (j == 2 ? p2 : p3));
-            dbase[j] = gbeg[j] + (pj[t] - pj[ptile]);
-        }
         const uint32_t cnt = T.count;
         uint32_t cp[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const uint32_t q = qq[k] < 4 ? qq[k] : 0u;
-            cp[k] = wtot[q][k * (kBlock / 64) + wid] + pre[k];   // already includes the run start of q
+            cp[k] = wtot[q][k * (kBlock / 64) + wid] + pre[k];   // includes the run start of q
         }
-#define COUPE_STORE4(EXPR)                                                               \
-    _Pragma("unroll") for (int k = 0; k < kItems; k++) {                                 \
-        const uint32_t pos = k * kBlock + tid;                                           \
-        if (pos < cnt) {                                                                 \
-            const uint32_t q = pos >= r3 ? 3u : (pos >= r2 ? 2u : (pos >= r1 ? 1u : 0u)); \
-            const uint32_t rs = q == 3 ? r3 : (q == 2 ? r2 : (q == 1 ? r1 : 0u));        \
-            const uint64_t d = (q == 3 ? dbase[3] : (q == 2 ? dbase[2] : (q == 1 ? dbase[1] : dbase[0]))) + (pos - rs); \
-            if (d < n_total) { EXPR; }                                                   \
-        }                                                                                \
+        // Stores run by run: slot q holds positions [s_run[q], s_run[q+1]).
+#define COUPE_STORE4(EXPR)                                                   \
+    _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
+        const uint32_t rb = s_run[j], len = s_run[j + 1] - rb;               \
+        const uint64_t db = s_dst[j];                                        \
+        for (uint32_t i = tid; i < len; i += kBlock) {                       \
+            const uint32_t pos = rb + i;                                     \
+            const uint64_t d = db + i;                                       \
+            if (d < n_total) { EXPR; }                                       \
+        }                                                                    \
     }
 #pragma unroll
         for (int k = 0; k < kItems; k++)
@@ -475,17 +503,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             __syncthreads();
         }
 #undef COUPE_STORE4
-        // Grandchildren's first pass (their split counts are read again by
-        // the next pair's first pass, so no per-tile counts are kept here).
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            if (qq[k] < 4) {
-                const uint32_t q = qq[k];
-                const float xc = sel3(a3, x0[k], x1[k], x2[k]);
-                const float tq = q == 3 ? t3[3] : (q == 2 ? t3[2] : (q == 1 ? t3[1] : t3[0]));
-                r.add((int)q, !(xc >= tq), AccOps<Acc>::of(wv[k]), f32_key(xc));
-            }
-        }
     }
     if (cur != 0xffffffffu) flush_grand_bins(r, sh, g, bcnt, bw, bmin);
 }
