@@ -108,4 +108,16 @@ __device__ inline int bin4(float c0, float c1, float c2, float x) {
     b += (x >= (b ? c2 : c0)) ? 1 : 0;
     return b;
 }
+// The tiles [t0, t1) of this workgroup: a contiguous share of the nt tiles,
+// so per-part state (destination bases, slot masks) changes seldom within a
+// workgroup.  Used by the move kernels, where every tile costs the same; the
+// search kernels keep the strided order because tiles of finished parts are
+// skipped there, and contiguous shares of those would leave workgroups idle.
+__device__ __forceinline__ void tile_range(uint32_t nt, uint32_t &t0, uint32_t &t1) {
+    const uint64_t per = ((uint64_t)nt + gridDim.x - 1) / gridDim.x;
+    const uint64_t a = (uint64_t)blockIdx.x * per;
+    t0 = (uint32_t)(a < nt ? a : nt);
+    t1 = (uint32_t)(a + per < nt ? a + per : nt);
+}
+
 }  // namespace coupe_hip

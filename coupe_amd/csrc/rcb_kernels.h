@@ -1202,7 +1202,9 @@ k_partition_tiles(const Part *__restrict__ parts, const Part *__restrict__ next_
     uint64_t cb0 = 0, cb1 = 0;        // first position of each child
     uint32_t ct0 = 0, ct1 = 0;        // first tile of each child
 
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    uint32_t t_first, t_last;
+    tile_range(nt, t_first, t_last);
+    for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (P.status == kDone || P.depth + 1 >= P.levels) continue;   // leaves: not moved

@@ -341,7 +341,9 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     bool all_left = false;
     uint32_t ptile = 0;
 
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    uint32_t t_first, t_last;
+    tile_range(nt, t_first, t_last);
+    for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (!pair_active(P)) continue;
