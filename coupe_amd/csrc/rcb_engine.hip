@@ -418,7 +418,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     uint64_t tiles_ub = std::max<uint64_t>(B.tiles0, 1);
     if (L > 0) {
         const unsigned g = persistent_grid(cx, k_sweep<WS, Acc>, tiles_ub);
-        timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
+        // the roots' first pass: split-axis coordinate and weight of every point
+        timed(cx, st, COUPE_HIP_K_HIST, (4.0 + sizeof(WS)) * (double)n, [&] {
             k_sweep<WS, Acc><<<g, kBlock, 0, st>>>((const Part *)cx.parts[0].ptr, tiles0,
                                                   &ctr->n_tiles[0], cs[0][0], cs[0][1], cs[0][2], ws[0], bcnt, bw,
                                                   bmin);
@@ -482,7 +483,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // the children's first pass over the parent's tiles
             {
                 const unsigned g = persistent_grid(cx, k_child_first<D, WS, Acc>, tl_ub);
-                timed(cx, st, COUPE_HIP_K_HIST, 12.0 * (double)n, [&] {
+                // parent-axis and child-axis coordinates and the weight
+                timed(cx, st, COUPE_HIP_K_HIST, (8.0 + sizeof(WS)) * (double)n, [&] {
                     k_child_first<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, cl[0], cl[1], cl[2], wl, bcnt, bw,
                                                                    bmin, below2, (uint64_t *)cx.tile_left.ptr);
                 });
@@ -789,7 +791,7 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
             run(0);
         }
         cx.prof.collect();
-        cx.prof.out[COUPE_HIP_K_HIST].algo_bytes = cx.swept_bytes;
+        cx.prof.out[COUPE_HIP_K_HIST].algo_bytes += cx.swept_bytes;   // first passes + standalone passes
     } catch (const HipFailure &f) {
         cx.bins_clean = false;
         (void)hipGetLastError();
