@@ -109,9 +109,10 @@ __device__ inline int bin4(float c0, float c1, float c2, float x) {
     return b;
 }
 // The tiles [t0, t1) of this workgroup: a contiguous share of the nt tiles,
-// so per-part state (destination bases, slot masks) changes seldom within a
-// workgroup.  Used by the move kernels, where every tile costs the same; the
-// search kernels keep the strided order because tiles of finished parts are
+// so per-part state (destination bases, register bins flushed at a part
+// change) changes seldom within a workgroup.  Used by the kernels where every
+// tile costs the same (the moves, the children's first pass); the later
+// search passes keep the strided order because tiles of finished parts are
 // skipped there, and contiguous shares of those would leave workgroups idle.
 __device__ __forceinline__ void tile_range(uint32_t nt, uint32_t &t0, uint32_t &t1) {
     const uint64_t per = ((uint64_t)nt + gridDim.x - 1) / gridDim.x;

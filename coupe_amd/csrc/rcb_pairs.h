@@ -61,7 +61,9 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     const float *ca = c0, *cb = c1;
     float t2 = 0.f, split = 0.f;
     bool all_left = false;
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    uint32_t t_first, t_last;
+    tile_range(nt, t_first, t_last);
+    for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (!children_searched(P)) continue;
@@ -339,7 +341,12 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     int ax = 0, a2 = 0, a3 = 0;
     float split = 0.f;
     bool all_left = false;
-    uint32_t ptile = 0;
+    // lane j of every wavefront serves grandchild slot j & 3: its scan array
+    // and, per part, its destination base minus the scan at the part's first tile
+    const int js = lane & 3;
+    const uint64_t *pj = js == 0 ? p0 : (js == 1 ? p1 : (js == 2 ? p2 : p3));
+    uint64_t gbase = 0;
+    bool gvalid = false;
 
     uint32_t t_first, t_last;
     tile_range(nt, t_first, t_last);
@@ -355,7 +362,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             a3 = (int)((P.axis + 2) % D);
             split = P.split;
             all_left = P.status == kAllLeft;
-            ptile = P.tile_begin;
             for (int s = 0; s < 2; s++) {
                 const int32_t ci = P.child[s];
                 gsplit[s] = 0.f;
@@ -373,32 +379,38 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 const int32_t gq = g[2 * s] >= 0 ? g[2 * s] : g[2 * s + 1];
                 t3[s] = gq >= 0 ? (grand[gq].blo + grand[gq].bhi) / 2.0f : 0.f;
             }
+            const int32_t gj = js == 0 ? g[0] : (js == 1 ? g[1] : (js == 2 ? g[2] : g[3]));
+            gvalid = gj >= 0;
+            gbase = gvalid ? grand[gj].begin - pj[P.tile_begin] : 0;
         }
-        if (tid < 4) {
-            const int32_t gq = g[tid];
-            const uint64_t *pj = tid == 0 ? p0 : (tid == 1 ? p1 : (tid == 2 ? p2 : p3));
-            s_dst[tid] = gq >= 0 ? grand[gq].begin + (pj[t] - pj[ptile]) : 0;
-        }
-        const uint64_t end = T.begin + T.count;
+        // this tile's scan values: loaded by every lane without a branch, so
+        // the load overlaps the points' and is consumed after the classification
+        const uint64_t pv = pj[t];
+        const uint32_t cnt = T.count;
         float x0[kItems], x1[kItems], x2[kItems];
         WS wv[kItems];
+        {
+            // clamped, unconditional loads from the tile's base with 32-bit
+            // offsets: all of them issue back to back (the values past the
+            // tile's end are never used)
+            const float *b0 = i0 + T.begin, *b1 = i1 + T.begin, *b2 = D == 3 ? i2 + T.begin : i2;
+            const WS *bwt = iw + T.begin;
 #pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            // clamped, unconditional loads: all of them issue back to back
-            // (the values past the tile's end are never used)
-            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const uint64_t c = p < end ? p : end - 1;
-            x0[k] = i0[c];
-            x1[k] = i1[c];
-            if constexpr (D == 3) x2[k] = i2[c];
-            else x2[k] = 0.f;
-            wv[k] = iw[c];
+            for (int k = 0; k < kItems; k++) {
+                const uint32_t p = (uint32_t)(k * kBlock + tid);
+                const uint32_t c = p < cnt ? p : cnt - 1;
+                x0[k] = b0[c];
+                x1[k] = b1[c];
+                if constexpr (D == 3) x2[k] = b2[c];
+                else x2[k] = 0.f;
+                wv[k] = bwt[c];
+            }
         }
         uint32_t pre[kItems];
         uint8_t qq[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
-            const bool valid = T.begin + (uint64_t)k * kBlock + tid < end;
+            const bool valid = (uint32_t)(k * kBlock + tid) < cnt;
             const bool s1 = !all_left && (sel3(ax, x0[k], x1[k], x2[k]) >= split);
             const float xb = sel3(a2, x0[k], x1[k], x2[k]);
             const bool s2 = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
@@ -434,6 +446,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 r.mn[j] = kj < r.mn[j] ? kj : r.mn[j];
             }
         }
+        if (tid < 4) s_dst[tid] = gvalid ? gbase + pv : 0;
         __syncthreads();
         if (tid < 64) {   // exclusive scans of the counts of each grandchild slot, item-major
             constexpr int kE = kItems * (kBlock / 64);
@@ -450,7 +463,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             if (tid == 0) s_run[0] = 0;
         }
         __syncthreads();
-        const uint32_t cnt = T.count;
         uint32_t cp[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
