@@ -116,8 +116,9 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
                          const double *tols, size_t n_inst, const Rotation *rot = nullptr);
 
-// Exclusive scan of n_dev (read on the device, at most n_ub) uint64 values in
-// place; `total` (device, may be null) receives their sum.
+// Exclusive scan of n_dev (read on the device, at most n_ub; n_ub itself when
+// n_dev is null) uint64 values in place; `total` (device, may be null)
+// receives their sum.
 void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub, uint64_t *total);
 
 // Grid::rcb on a device-resident weight array (see coupe_hip_grid_rcb).

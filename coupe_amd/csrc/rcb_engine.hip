@@ -129,12 +129,12 @@ void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev
     uint64_t *bsum = (uint64_t *)cx.bsum.ptr;
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
         if (nblocks == 1) {
-            k_scan_local<<<1, kBlock, 0, st>>>(data, data, bsum, n_dev, total ? total : bsum + 1);
+            k_scan_local<<<1, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, total ? total : bsum + 1);
             return;
         }
-        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, nullptr);
-        k_scan_top<<<1, kBlock, 0, st>>>(bsum, n_dev, total);
-        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev);
+        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, nullptr);
+        k_scan_top<<<1, kBlock, 0, st>>>(bsum, n_dev, n_ub, total);
+        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub);
     });
 }
 
@@ -439,7 +439,6 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         uint32_t *below2 = (uint32_t *)cx.below2.ptr;
         uint16_t *tile_bins2 = (uint16_t *)cx.tile_bins2.ptr;
         uint64_t *q4[4];
-        for (int j = 0; j < 4; j++) q4[j] = (uint64_t *)cx.tile4.ptr + (uint64_t)j * tiles_cap;
         int ia = 0, csl = 0, wsl = 0;
         Tile *tl = tiles0;
         uint64_t tl_ub = std::max<uint64_t>(B.tiles0, 1);
@@ -503,14 +502,19 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             Tile *tn = (Tile *)cx.tiles[ic].ptr;
             children_of(ib, ic, level + 1, tn);
             if (level + 2 == L) break;
-            // the move
+            // the move.  The per-tile counts of the 4 grandchild slots lie end
+            // to end (stride tl_ub) and are scanned as one array: the move only
+            // takes differences within one slot's tiles of one part, which the
+            // sums of the preceding slots (and the unused entries past the
+            // tile count) do not change.
+            for (int j = 0; j < 4; j++) q4[j] = (uint64_t *)cx.tile4.ptr + (uint64_t)j * tl_ub;
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 const unsigned g = persistent_grid(cx, k_tile_count4<D>, (tl_ub + kBlock / 64 - 1) / (kBlock / 64));
                 k_tile_count4<D><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
                                                       (const uint64_t *)cx.tile_left.ptr, below2, tile_bins2, q4[0],
                                                       q4[1], q4[2], q4[3]);
             });
-            for (int j = 0; j < 4; j++) scan_u64(cx, st, q4[j], ntl, tl_ub, nullptr);
+            scan_u64(cx, st, q4[0], nullptr, 4 * tl_ub, nullptr);
             const int csn = csl == 1 ? 2 : 1, wsn = wsl ^ 1;
             float *const *co = cs[csn];
             {

@@ -546,13 +546,14 @@ __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t *wsum, uint64_t 
 }
 
 // With `single` (a launch of one block covering every element) the block's
-// total is the scan's total and no further kernel is needed.
+// total is the scan's total and no further kernel is needed.  The element
+// count is *n_dev, or n_host when n_dev is null.
 __global__ void __launch_bounds__(kBlock)
 k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-             uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t *single) {
+             uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t n_host, uint64_t *single) {
     __shared__ uint64_t wsum[kBlock / 64];
     __shared__ uint64_t tot;
-    const uint64_t n = *n_dev;
+    const uint64_t n = n_dev ? *n_dev : n_host;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     if ((uint64_t)blockIdx.x * kScanTile >= n) {
         if (single && threadIdx.x == 0) *single = 0;
@@ -578,10 +579,10 @@ k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_scan_top(uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t *total) {
+k_scan_top(uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t n_host, uint64_t *total) {
     __shared__ uint64_t wsum[kBlock / 64];
     __shared__ uint64_t tot;
-    const uint64_t nb = (*n_dev + kScanTile - 1) / kScanTile;
+    const uint64_t nb = ((n_dev ? *n_dev : n_host) + kScanTile - 1) / kScanTile;
     uint64_t carry = 0;
     for (uint64_t b0 = 0; b0 < nb; b0 += kBlock) {
         uint64_t i = b0 + threadIdx.x;
@@ -595,8 +596,9 @@ k_scan_top(uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t *total) 
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const uint32_t *n_dev) {
-    const uint64_t n = *n_dev;
+k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const uint32_t *n_dev,
+           uint64_t n_host) {
+    const uint64_t n = n_dev ? *n_dev : n_host;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     if (base >= n) return;
     const uint64_t add = bsum[blockIdx.x];
