@@ -93,6 +93,7 @@ Context *default_context() {
         c->lookback = coupe_hip::lookback_from_env();
         c->search_passes = coupe_hip::search_passes_from_env();
         c->pairs = coupe_hip::pairs_from_env();
+        c->forward = coupe_hip::forward_from_env();
         g_default[dev] = c;
     }
     return g_default[dev];
@@ -166,6 +167,7 @@ extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
     c->lookback = coupe_hip::lookback_from_env();
     c->search_passes = coupe_hip::search_passes_from_env();
     c->pairs = coupe_hip::pairs_from_env();
+    c->forward = coupe_hip::forward_from_env();
     return reinterpret_cast<coupe_hip_ctx *>(c);
 }
 

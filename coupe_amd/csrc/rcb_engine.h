@@ -61,9 +61,12 @@ struct Context {
     DevBuf instances;                // instance table of the current call
     DevBuf tiles0;                   // level-0 tiles (prologue, first level, descent)
     DevBuf below, tile_left;         // per-tile left counts (fused / counted) and their scan
-    DevBuf below2, tile_bins2, tile4;   // two levels per move: per (tile, side) counts, 4 grandchild counts
+    DevBuf tile_bins2, tile4;        // two levels per move: per-(tile, side) bins, 4 grandchild counts
+    DevBuf pleft[2], pbelow2[2];     // per tile: left count, per side the count below the children's candidate
+    DevBuf fwd_cnt, fwd_w, fwd_mn;   // children's first pass forwarded by the move, per (part, side)
     int search_passes = 3;           // histogram passes queued per level without a host check
     bool pairs = true;               // move the points once per two levels
+    bool forward = true;             // the move forwards the next children's first pass
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
@@ -96,6 +99,13 @@ inline bool lookback_from_env() {
 // COUPE_HIP_PAIRS=0: move the points at every level instead of every other.
 inline bool pairs_from_env() {
     const char *v = getenv("COUPE_HIP_PAIRS");
+    return !(v && v[0] == '0');
+}
+
+// COUPE_HIP_FORWARD=0: the move does not forward the children's first pass
+// of the level it fills (every such pass reads the points).
+inline bool forward_from_env() {
+    const char *v = getenv("COUPE_HIP_FORWARD");
     return !(v && v[0] == '0');
 }
 

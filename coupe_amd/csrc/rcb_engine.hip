@@ -433,16 +433,33 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         // contiguous points (table ia, tiles tl, coordinates cl, weights wl).
         // Their children (table ib) are searched in place over the parent's
         // tiles; the grandchildren (table ic) receive the points.
-        cx.below2.ensure(tiles_cap * 2 * sizeof(uint32_t));
+        for (int b = 0; b < 2; b++) {
+            cx.pleft[b].ensure(tiles_cap * sizeof(uint64_t));
+            cx.pbelow2[b].ensure(tiles_cap * 2 * sizeof(uint32_t));
+        }
+        cx.fwd_cnt.ensure(parts_cap * 2 * sizeof(uint32_t));
+        cx.fwd_w.ensure(parts_cap * 2 * sizeof(Acc));
+        cx.fwd_mn.ensure(parts_cap * 2 * sizeof(uint32_t));
         cx.tile_bins2.ensure(tiles_cap * 2 * kMaxBins * sizeof(uint16_t));
         cx.tile4.ensure(tiles_cap * 4 * sizeof(uint64_t));
-        uint32_t *below2 = (uint32_t *)cx.below2.ptr;
         uint16_t *tile_bins2 = (uint16_t *)cx.tile_bins2.ptr;
+        // Forwarding: the move to level l + 2 also sums, for each new part,
+        // its children's first pass (per side of its first candidate) and
+        // each of its tiles' counts; a part whose split ends at its first
+        // candidate then needs no first pass of its children.  Per-tile
+        // counts alternate between two buffers (par: the current level's).
+        const Forward<Acc> none{nullptr, nullptr, nullptr, nullptr, nullptr};
+        auto fwd_at = [&](int b) {
+            return Forward<Acc>{(uint32_t *)cx.fwd_cnt.ptr, (Acc *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr,
+                                (uint64_t *)cx.pleft[b].ptr, (uint32_t *)cx.pbelow2[b].ptr};
+        };
+        int par = 0;
         uint64_t *q4[4];
         int ia = 0, csl = 0, wsl = 0;
         Tile *tl = tiles0;
         uint64_t tl_ub = std::max<uint64_t>(B.tiles0, 1);
-        auto children_of = [&](int from, int to, size_t level, Tile *next_tiles) {
+        auto children_of = [&](int from, int to, size_t level, Tile *next_tiles, const Forward<Acc> &fsrc,
+                               const Forward<Acc> &fclr) {
             Part *pf = (Part *)cx.parts[from].ptr;
             const uint64_t pub = std::max<uint64_t>(B.parts_ub[level], 1);
             if (pub <= kPlanParts) {
@@ -462,7 +479,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_children_write<Acc><<<(unsigned)pub, 64, 0, st>>>(
                     pf, &ctr->n_parts[from], D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts);
+                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr);
                 if (pub > kPlanParts)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[to],
                                                   &ctr->n_tiles[to]);
@@ -476,7 +493,11 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             float *const *cl = cs[csl];
             WS *wl = ws[wsl];
             cx.stats.levels++;
-            children_of(ia, ib, level, nullptr);   // in place: no tiles
+            // the move that made this level forwarded its children's first pass
+            const bool fwd_in = cx.forward && level >= 2 && level + 1 < L;
+            uint64_t *pl = (uint64_t *)cx.pleft[par].ptr;
+            uint32_t *pb2 = (uint32_t *)cx.pbelow2[par].ptr;
+            children_of(ia, ib, level, nullptr, fwd_in ? fwd_at(par) : none, none);   // in place: no tiles
             if (level + 1 == L) break;
             cx.stats.levels++;
             // the children's first pass over the parent's tiles
@@ -485,7 +506,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 // parent-axis and child-axis coordinates and the weight
                 timed(cx, st, COUPE_HIP_K_HIST, (8.0 + sizeof(WS)) * (double)n, [&] {
                     k_child_first<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, cl[0], cl[1], cl[2], wl, bcnt, bw,
-                                                                   bmin, below2, (uint64_t *)cx.tile_left.ptr);
+                                                                   bmin, pb2, pl, fwd_in ? 1u : 0u);
                 });
             }
             const uint64_t pub1 = std::max<uint64_t>(B.parts_ub[level + 1], 1);
@@ -500,7 +521,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             }
             // grandchildren, with tiles
             Tile *tn = (Tile *)cx.tiles[ic].ptr;
-            children_of(ib, ic, level + 1, tn);
+            // the next move forwards when the grandchildren's children are searched
+            const bool fwd_out = cx.forward && level + 3 < L;
+            children_of(ib, ic, level + 1, tn, none, fwd_out ? fwd_at(par ^ 1) : none);
             if (level + 2 == L) break;
             // the move.  The per-tile counts of the 4 grandchild slots lie end
             // to end (stride tl_ub) and are scanned as one array: the move only
@@ -511,7 +534,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 const unsigned g = persistent_grid(cx, k_tile_count4<D>, (tl_ub + kBlock / 64 - 1) / (kBlock / 64));
                 k_tile_count4<D><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
-                                                      (const uint64_t *)cx.tile_left.ptr, below2, tile_bins2, q4[0],
+                                                      pl, pb2, tile_bins2, q4[0],
                                                       q4[1], q4[2], q4[3]);
             });
             scan_u64(cx, st, q4[0], nullptr, 4 * tl_ub, nullptr);
@@ -522,7 +545,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
                     k_partition4<D, WS, Acc><<<g, kBlock, 0, st>>>(
                         pa, pb, (const Part *)cx.parts[ic].ptr, tl, ntl, q4[0], q4[1], q4[2], q4[3], cl[0], cl[1],
-                        cl[2], wl, co[0], co[1], co[2], ws[wsn], n, bcnt, bw, bmin);
+                        cl[2], wl, co[0], co[1], co[2], ws[wsn], n, bcnt, bw, bmin, fwd_out ? fwd_at(par ^ 1) : none);
                 });
             }
             // the grandchildren's search over their own tiles
@@ -534,6 +557,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             wsl = wsn;
             tl = tn;
             tl_ub = tub2;
+            par ^= 1;
         }
         parts_ub = std::max<uint64_t>(B.parts_ub[L - 1], 1);
     } else {
@@ -565,7 +589,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                     parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                     (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
-                    node_base[std::min(level + 1, L)], heap_split, insts);
+                    node_base[std::min(level + 1, L)], heap_split, insts, Forward<Acc>{}, bcnt, bw, bmin,
+                    Forward<Acc>{});
                 if (!plan1)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                                   &ctr->n_tiles[nxt]);
@@ -820,7 +845,7 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &below2, &tile_bins2, &tile4,
+                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                            &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
@@ -829,7 +854,7 @@ uint64_t Context::device_bytes() const {
 
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &below2, &tile_bins2, &tile4,
+                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                      &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();

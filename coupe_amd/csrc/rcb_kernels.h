@@ -611,6 +611,33 @@ k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const 
 // and right (2i+2, box lo = split, sum S - wl), kept when non-empty
 // (:586-588, :603-641).  Their split search starts on the next axis.
 // ---------------------------------------------------------------------------
+// Forwarding, filled by the move for each grandchild g (a part of the next
+// real level) and each side of g's first candidate t3: the first pass of g's
+// children, valid when g's split ends at t3.  Per (g, side): the count and
+// weight of the points below the children's first candidate t4 (the midpoint
+// of g's box along the children's axis, which g's split does not change) and
+// the smallest key at or above it.  Per tile of g: the count below t3 (the
+// tile's left count) and, per side, the count below t4 (the children's
+// per-tile counts).
+template <typename Acc>
+struct Forward {
+    uint32_t *cnt;      // [2 * part + side]
+    Acc *w;
+    uint32_t *mn;
+    uint64_t *left;     // [tile]
+    uint32_t *below2;   // [2 * tile + side]
+};
+
+// A part whose children's first pass was forwarded by the move that filled
+// it: its split is its first candidate, so the forwarded sums below that
+// candidate are the children's.
+__device__ __forceinline__ bool forwarded(const Part &P) {
+    if (P.status != kSplitAtT) return false;
+    const int a = (int)P.axis;
+    const float t = (sel3(a, P.lo[0], P.lo[1], P.lo[2]) + sel3(a, P.hi[0], P.hi[1], P.hi[2])) / 2.0f;
+    return P.split == t;
+}
+
 // Split value of a node in the dense heap of the descent: NaN when every
 // point went left, since !(c >= NaN) holds for every c.
 __device__ inline float dense_split(const Part &P) {
@@ -695,11 +722,16 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
                  uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split,
-                 Instance *__restrict__ insts) {
+                 Instance *__restrict__ insts, Forward<Acc> fsrc, uint32_t *__restrict__ bcnt,
+                 Acc *__restrict__ bw, uint32_t *__restrict__ bmin, Forward<Acc> fclr) {
+    // fsrc (non-null): a forwarded part's children get their first-pass bins
+    // from it.  fclr (non-null): the new parts' forwarding sums and their
+    // tiles' counts are cleared for the move that will fill them.
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     const Part P = parts[s];
     if (P.status == kDone) return;
+    const bool fwd_in = fsrc.cnt != nullptr && forwarded(P);
     // A part at its instance's last level has leaves, not parts (:589-601).
     const bool inner = P.depth + 1 < P.levels;
     const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;
@@ -751,6 +783,20 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             C.counts_ready = 0;
             C.tol = P.tol;
             next[co] = C;
+            if (fwd_in) {   // bins of the child's first candidate (2 bins per part)
+                const uint64_t f = 2 * (uint64_t)s + side;
+                bcnt[2 * co] = fsrc.cnt[f];
+                bw[2 * co] = fsrc.w[f];
+                bmin[2 * co + 1] = fsrc.mn[f];
+            }
+            if (fclr.cnt) {
+                fclr.cnt[2 * co] = 0;
+                fclr.cnt[2 * co + 1] = 0;
+                fclr.w[2 * co] = (Acc)0;
+                fclr.w[2 * co + 1] = (Acc)0;
+                fclr.mn[2 * co] = 0xffffffffu;
+                fclr.mn[2 * co + 1] = 0xffffffffu;
+            }
         }
         for (uint32_t k = threadIdx.x; next_tiles && k < nt; k += 64) {
             Tile U;
@@ -759,6 +805,11 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
             U.count = (uint32_t)(rem < (uint64_t)kTile ? rem : (uint64_t)kTile);
             U.part = (uint32_t)co;
             next_tiles[to + k] = U;
+            if (fclr.left) {
+                fclr.left[to + k] = 0;
+                fclr.below2[2 * (to + k)] = 0;
+                fclr.below2[2 * (to + k) + 1] = 0;
+            }
         }
         co++;
         to += nt;

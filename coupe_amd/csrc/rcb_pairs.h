@@ -49,7 +49,8 @@ __global__ void __launch_bounds__(kBlock)
 k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
               const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
               const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-              uint32_t *__restrict__ bmin, uint32_t *__restrict__ below2, uint64_t *__restrict__ tile_left) {
+              uint32_t *__restrict__ bmin, uint32_t *__restrict__ below2, uint64_t *__restrict__ tile_left,
+              uint32_t fwd_ok) {
     __shared__ ChildBinsShared<Acc> sh;
     __shared__ uint32_t s4[3 * (kBlock / 64)];
     const int tid = threadIdx.x;
@@ -66,7 +67,7 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
-        if (!children_searched(P)) continue;
+        if (!children_searched(P) || (fwd_ok && forwarded(P))) continue;   // forwarded: nothing to read
         if (T.part != cur) {
             if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
             cur = T.part;
@@ -256,60 +257,21 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     }
 }
 
-// Bins of the grandchildren's first pass: the count below the candidate is
-// accumulated from ballots (wave-uniform), the weight below and the smallest
-// key at or above it per lane.
-template <typename Acc>
-struct GrandBins {
-    uint32_t cnt[4], mn[4];
-    Acc w[4];
-    __device__ __forceinline__ void reset() {
-#pragma unroll
-        for (int q = 0; q < 4; q++) { cnt[q] = 0; mn[q] = 0xffffffffu; w[q] = (Acc)0; }
-    }
-};
-
-template <typename Acc>
-struct GrandBinsShared {
-    uint32_t cnt[kBlock / 64][4], mn[kBlock / 64][4];
-    Acc w[kBlock / 64][4];
-};
-
-template <typename Acc>
-__device__ __forceinline__ void flush_grand_bins(GrandBins<Acc> &r, GrandBinsShared<Acc> &sh, const int32_t (&g)[4],
-                                                 uint32_t *__restrict__ gcnt, Acc *__restrict__ gw,
-                                                 uint32_t *__restrict__ gmn) {
-    const int lane = lane_id(), wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const Acc x = wave_sum(r.w[q]);
-        const uint32_t m = wave_min(r.mn[q]);
-        if (lane == 0) { sh.cnt[wid][q] = r.cnt[q]; sh.w[wid][q] = x; sh.mn[wid][q] = m; }
-    }
-    __syncthreads();
-    const int q = threadIdx.x;
-    if (q < 4 && g[q] >= 0) {
-        uint32_t c = 0, m = 0xffffffffu;
-        Acc x = (Acc)0;
-#pragma unroll
-        for (int v = 0; v < kBlock / 64; v++) {
-            c += sh.cnt[v][q];
-            x += sh.w[v][q];
-            m = sh.mn[v][q] < m ? sh.mn[v][q] : m;
-        }
-        const uint64_t b = (uint64_t)g[q] * 2;
-        if (c) {
-            atomicAdd(&gcnt[b], c);
-            atomic_add_acc(&gw[b], x);
-        }
-        if (m != 0xffffffffu) atomicMin(&gmn[b + 1], m);
-    }
-    __syncthreads();
-}
-
 // The move of a pair of levels: every point of an active parent goes to its
 // grandchild; stable within each grandchild.  p0..p3 are the exclusive scans
 // of the per-tile counts of grandchild slots q = 0..3.
+//
+// Per tile: the points are classified (three ballots give each point's slot
+// and rank), one wavefront scans the slot counts, and the points are stored
+// through LDS in two phases: first the coordinate of the children's axis (3D
+// only), all threads, run by run; then, one wavefront per grandchild slot, the
+// coordinates of the grandchildren's axis (a3) and of their children's axis
+// (a4) with the weight.  The second phase also sums, per wavefront and across
+// the tiles of one parent, the grandchild's first pass (count, weight and
+// smallest key at its first candidate t3) and, with fw.cnt non-null, the
+// forwarding of its children's first pass (per side of t3: count, weight and
+// smallest key at their first candidate t4; per destination tile: the count
+// below t3 and per side the count below t4).
 template <int D, typename WS, typename Acc>
 __global__ void __launch_bounds__(kBlock)
 k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
@@ -318,35 +280,63 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
              const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
              const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
              WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-             uint32_t *__restrict__ bmin) {
-    __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
+             uint32_t *__restrict__ bmin, Forward<Acc> fw) {
+    static_assert(kBlock / 64 == 4, "one wavefront per grandchild slot");
+    __shared__ float sfa[kTile], sfb[kTile];
+    __shared__ WS sws[kTile];
     __shared__ uint32_t wtot[4][kItems * (kBlock / 64)];
     __shared__ uint32_t s_run[5];
     __shared__ uint64_t s_dst[4];
-    __shared__ GrandBinsShared<Acc> sh;
-    float *sf0 = reinterpret_cast<float *>(stage);
-    float *sf1 = sf0 + kTile;
-    WS *sw = reinterpret_cast<WS *>(stage);
-    WS *sw1 = reinterpret_cast<WS *>(sf1);
+    __shared__ uint32_t s_off[4], s_tile[4];
+    __shared__ uint64_t s_gbeg[4], s_gpj0[4];
+    __shared__ uint32_t s_gtb[4];
     const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nt = *n_tiles;
-    GrandBins<Acc> r;
-    r.reset();
+    const bool fwd = fw.cnt != nullptr;
     uint32_t cur = 0xffffffffu;
-    int32_t g[4] = {-1, -1, -1, -1};
     float gsplit[2] = {0.f, 0.f};
     bool gall[2] = {true, true};
-    float t3[2] = {0.f, 0.f};   // grandchildren's first candidate, per side of the parent's split
-    int ax = 0, a2 = 0, a3 = 0;
+    int ax = 0, a2 = 0, a3 = 0, a4 = 0;
     float split = 0.f;
     bool all_left = false;
-    // lane j of every wavefront serves grandchild slot j & 3: its scan array
-    // and, per part, its destination base minus the scan at the part's first tile
+    // lanes 0..3 serve grandchild slot `lane`: its scan array
     const int js = lane & 3;
     const uint64_t *pj = js == 0 ? p0 : (js == 1 ? p1 : (js == 2 ? p2 : p3));
-    uint64_t gbase = 0;
-    bool gvalid = false;
+    // wavefront wid's grandchild slot in the second phase: the grandchild,
+    // its first candidate t3 and its children's t4, and the sums of both
+    // first passes over the tiles of the current parent
+    int32_t gw_ = -1;
+    float t3w = 0.f, t4w = 0.f;
+    uint32_t gcnt = 0, gmn = 0xffffffffu;
+    Acc gws = (Acc)0;
+    Acc fws0 = (Acc)0, fws1 = (Acc)0;
+    uint32_t fmn0 = 0xffffffffu, fmn1 = 0xffffffffu, fcn0 = 0, fcn1 = 0;
+    auto flush = [&]() {   // wavefront-local: no workgroup barrier
+        if (gw_ < 0) return;
+        const uint64_t b = 2 * (uint64_t)gw_;
+        const Acc gx = wave_sum(gws);
+        const uint32_t gm = wave_min(gmn);
+        if (lane == 0) {
+            if (gcnt) { atomicAdd(&bcnt[b], gcnt); atomic_add_acc(&bw[b], gx); }
+            if (gm != 0xffffffffu) atomicMin(&bmin[b + 1], gm);
+        }
+        gws = (Acc)0;
+        gmn = 0xffffffffu;
+        gcnt = 0;
+        if (!fwd) return;
+        const Acc x0 = wave_sum(fws0), x1 = wave_sum(fws1);
+        const uint32_t m0 = wave_min(fmn0), m1 = wave_min(fmn1);
+        if (lane == 0) {
+            if (fcn0) { atomicAdd(&fw.cnt[b], fcn0); atomic_add_acc(&fw.w[b], x0); }
+            if (fcn1) { atomicAdd(&fw.cnt[b + 1], fcn1); atomic_add_acc(&fw.w[b + 1], x1); }
+            if (m0 != 0xffffffffu) atomicMin(&fw.mn[b], m0);
+            if (m1 != 0xffffffffu) atomicMin(&fw.mn[b + 1], m1);
+        }
+        fws0 = fws1 = (Acc)0;
+        fmn0 = fmn1 = 0xffffffffu;
+        fcn0 = fcn1 = 0;
+    };
 
     uint32_t t_first, t_last;
     tile_range(nt, t_first, t_last);
@@ -355,13 +345,15 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
         const Part &P = parts[T.part];
         if (!pair_active(P)) continue;
         if (T.part != cur) {
-            if (cur != 0xffffffffu) { flush_grand_bins(r, sh, g, bcnt, bw, bmin); r.reset(); }
+            if (cur != 0xffffffffu) flush();
             cur = T.part;
             ax = (int)P.axis;
             a2 = (int)((P.axis + 1) % D);
             a3 = (int)((P.axis + 2) % D);
+            a4 = (int)((P.axis + 3) % D);
             split = P.split;
             all_left = P.status == kAllLeft;
+            int32_t g[4];
             for (int s = 0; s < 2; s++) {
                 const int32_t ci = P.child[s];
                 gsplit[s] = 0.f;
@@ -374,14 +366,19 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     g[2 * s] = C.child[0];
                     g[2 * s + 1] = C.child[1];
                 }
-                // the grandchildren of one side share their box along a3
-                // (in 3D the same for both sides): the first candidate (:472)
-                const int32_t gq = g[2 * s] >= 0 ? g[2 * s] : g[2 * s + 1];
-                t3[s] = gq >= 0 ? (grand[gq].blo + grand[gq].bhi) / 2.0f : 0.f;
             }
-            const int32_t gj = js == 0 ? g[0] : (js == 1 ? g[1] : (js == 2 ? g[2] : g[3]));
-            gvalid = gj >= 0;
-            gbase = gvalid ? grand[gj].begin - pj[P.tile_begin] : 0;
+            if (tid < 4) {
+                const int32_t gj = tid == 0 ? g[0] : (tid == 1 ? g[1] : (tid == 2 ? g[2] : g[3]));
+                s_gbeg[tid] = gj >= 0 ? grand[gj].begin : 0;
+                s_gtb[tid] = gj >= 0 ? grand[gj].tile_begin : 0;
+                s_gpj0[tid] = gj >= 0 ? pj[P.tile_begin] : 0;
+            }
+            gw_ = wid == 0 ? g[0] : (wid == 1 ? g[1] : (wid == 2 ? g[2] : g[3]));
+            if (gw_ >= 0) {
+                const Part &G = grand[gw_];
+                t3w = (G.blo + G.bhi) / 2.0f;   // the grandchild's first candidate (:472)
+                t4w = (sel3(a4, G.lo[0], G.lo[1], G.lo[2]) + sel3(a4, G.hi[0], G.hi[1], G.hi[2])) / 2.0f;
+            }
         }
         // this tile's scan values: loaded by every lane without a branch, so
         // the load overlaps the points' and is consumed after the classification
@@ -406,8 +403,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 wv[k] = bwt[c];
             }
         }
-        uint32_t pre[kItems];
-        uint8_t qq[kItems];
+        uint32_t pq[kItems];   // rank among the wavefront's points of the same slot | slot << 8
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const bool valid = (uint32_t)(k * kBlock + tid) < cnt;
@@ -415,7 +411,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const float xb = sel3(a2, x0[k], x1[k], x2[k]);
             const bool s2 = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
             const uint32_t q = (s1 ? 2u : 0u) + (s2 ? 1u : 0u);
-            qq[k] = valid ? (uint8_t)q : (uint8_t)4;
             // slot masks from three ballots (wave-uniform)
             const uint64_t bv = __ballot(valid), b1 = __ballot(s1), b2 = __ballot(s2);
             const uint64_t m0 = bv & ~b1 & ~b2, m1 = bv & ~b1 & b2, m2 = bv & b1 & ~b2, m3 = bv & b1 & b2;
@@ -427,26 +422,14 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 wtot[3][e] = __popcll(m3);
             }
             const uint64_t mine = s1 ? (s2 ? m3 : m2) : (s2 ? m1 : m0);
-            pre[k] = __popcll(mine & lt_mask);
-            // grandchildren's first pass
-            const float xc = sel3(a3, x0[k], x1[k], x2[k]);
-            const bool bel = !(xc >= (s1 ? t3[1] : t3[0]));
-            const uint64_t bb = __ballot(bel);
-            r.cnt[0] += __popcll(m0 & bb);
-            r.cnt[1] += __popcll(m1 & bb);
-            r.cnt[2] += __popcll(m2 & bb);
-            r.cnt[3] += __popcll(m3 & bb);
-            const Acc wa = AccOps<Acc>::of(wv[k]);
-            const uint32_t key = f32_key(xc);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const bool mj = valid && q == (uint32_t)j;
-                r.w[j] += (mj && bel) ? wa : (Acc)0;
-                const uint32_t kj = (mj && !bel) ? key : 0xffffffffu;
-                r.mn[j] = kj < r.mn[j] ? kj : r.mn[j];
-            }
+            pq[k] = (uint32_t)__popcll(mine & lt_mask) | ((valid ? q : 4u) << 8);
         }
-        if (tid < 4) s_dst[tid] = gvalid ? gbase + pv : 0;
+        if (tid < 4) {
+            const uint64_t off = pv - s_gpj0[tid];   // this tile's first position in the grandchild
+            s_dst[tid] = s_gbeg[tid] + off;
+            s_off[tid] = (uint32_t)(off % kTile);
+            s_tile[tid] = s_gtb[tid] + (uint32_t)(off / kTile);
+        }
         __syncthreads();
         if (tid < 64) {   // exclusive scans of the counts of each grandchild slot, item-major
             constexpr int kE = kItems * (kBlock / 64);
@@ -466,59 +449,102 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
         uint32_t cp[kItems];
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
-            const uint32_t q = qq[k] < 4 ? qq[k] : 0u;
-            cp[k] = wtot[q][k * (kBlock / 64) + wid] + pre[k];   // includes the run start of q
+            const uint32_t q = (pq[k] >> 8) < 4 ? (pq[k] >> 8) : 0u;
+            cp[k] = wtot[q][k * (kBlock / 64) + wid] + (pq[k] & 0xffu);   // includes the run start of q
         }
-        // Stores run by run: slot q holds positions [s_run[q], s_run[q+1]).
-#define COUPE_STORE4(EXPR)                                                   \
-    _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
-        const uint32_t rb = s_run[j], len = s_run[j + 1] - rb;               \
-        const uint64_t db = s_dst[j];                                        \
-        for (uint32_t i = tid; i < len; i += kBlock) {                       \
-            const uint32_t pos = rb + i;                                     \
-            const uint64_t d = db + i;                                       \
-            if (d < n_total) { EXPR; }                                       \
-        }                                                                    \
-    }
+        if constexpr (D == 3) {
+            // phase 1: the children's axis, all threads, run by run
+            float *oa2 = a2 == 0 ? o0 : (a2 == 1 ? o1 : o2);
+#pragma unroll
+            for (int k = 0; k < kItems; k++)
+                if (k * kBlock + tid < (int)cnt) sfa[cp[k]] = sel3(a2, x0[k], x1[k], x2[k]);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t rb = s_run[j], len = s_run[j + 1] - rb;
+                const uint64_t db = s_dst[j];
+                for (uint32_t i = tid; i < len; i += kBlock) {
+                    const uint64_t d = db + i;
+                    if (d < n_total) oa2[d] = sfa[rb + i];
+                }
+            }
+            __syncthreads();
+        }
+        // phase 2: a3, a4 and the weight; wavefront j stores run j
 #pragma unroll
         for (int k = 0; k < kItems; k++)
-            if (k * kBlock + tid < (int)cnt) { sf0[cp[k]] = x0[k]; sf1[cp[k]] = x1[k]; }
+            if (k * kBlock + tid < (int)cnt) {
+                sfa[cp[k]] = sel3(a3, x0[k], x1[k], x2[k]);
+                sfb[cp[k]] = sel3(a4, x0[k], x1[k], x2[k]);
+                sws[cp[k]] = wv[k];
+            }
         __syncthreads();
-        COUPE_STORE4(o0[d] = sf0[pos]; o1[d] = sf1[pos])
-        __syncthreads();
-        if constexpr (sizeof(WS) == 4) {
-#pragma unroll
-            for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) {
-                    if constexpr (D == 3) sf0[cp[k]] = x2[k];
-                    sw1[cp[k]] = wv[k];
+        {
+            float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
+            float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
+            const uint32_t rb = s_run[wid], len = s_run[wid + 1] - rb;
+            const uint64_t db = s_dst[wid];
+            const uint32_t split_i = kTile - s_off[wid];   // positions from here on are in the next tile
+            uint32_t c_left = 0, c_b0 = 0, c_b1 = 0, h_left = 0, h_b0 = 0, h_b1 = 0;
+            for (uint32_t i = lane; i < len; i += 64) {
+                const uint64_t d = db + i;
+                const float xa = sfa[rb + i], xb = sfb[rb + i];
+                const WS wq = sws[rb + i];
+                if (d < n_total) {
+                    oa3[d] = xa;
+                    oa4[d] = xb;
+                    ow[d] = wq;
                 }
-            __syncthreads();
-            if constexpr (D == 3) {
-                COUPE_STORE4(o2[d] = sf0[pos]; ow[d] = sw1[pos])
-            } else {
-                COUPE_STORE4(ow[d] = sw1[pos])
+                const bool bel3 = !(xa >= t3w);
+                const uint64_t bb = __ballot(bel3);
+                const Acc wa = AccOps<Acc>::of(wq);
+                // the grandchild's first pass
+                c_left += __popcll(bb);
+                gws += bel3 ? wa : (Acc)0;
+                const uint32_t ka = bel3 ? 0xffffffffu : f32_key(xa);
+                gmn = ka < gmn ? ka : gmn;
+                if (fwd) {
+                    const bool below4 = !(xb >= t4w);
+                    const uint64_t b4 = __ballot(below4), act = __ballot(true);
+                    // lanes at or past split_i lie in the next destination tile
+                    const uint32_t ln = i - lane;   // this iteration's first position
+                    const uint64_t hi_m = split_i <= ln ? act
+                                          : (split_i - ln >= 64 ? 0ull : act & ~(~0ull >> (64 - (split_i - ln))));
+                    c_b0 += __popcll(bb & b4);
+                    c_b1 += __popcll(~bb & act & b4);
+                    h_left += __popcll(bb & hi_m);
+                    h_b0 += __popcll(bb & b4 & hi_m);
+                    h_b1 += __popcll(~bb & b4 & hi_m);
+                    const uint32_t key = f32_key(xb);
+                    fws0 += (bel3 && below4) ? wa : (Acc)0;
+                    fws1 += (!bel3 && below4) ? wa : (Acc)0;
+                    const uint32_t k0 = (bel3 && !below4) ? key : 0xffffffffu;
+                    const uint32_t k1 = (!bel3 && !below4) ? key : 0xffffffffu;
+                    fmn0 = k0 < fmn0 ? k0 : fmn0;
+                    fmn1 = k1 < fmn1 ? k1 : fmn1;
+                }
             }
-            __syncthreads();
-        } else {
-            if constexpr (D == 3) {
-#pragma unroll
-                for (int k = 0; k < kItems; k++)
-                    if (k * kBlock + tid < (int)cnt) sf0[cp[k]] = x2[k];
-                __syncthreads();
-                COUPE_STORE4(o2[d] = sf0[pos])
-                __syncthreads();
+            // The counts come from ballots, so every lane that ran the last
+            // iteration holds them; lane 0 always runs it.
+            gcnt += c_left;
+            if (fwd && len > 0) {
+                fcn0 += c_b0;
+                fcn1 += c_b1;
+                if (lane == 0) {
+                    const uint64_t dt = s_tile[wid];
+                    const uint32_t l0 = c_left - h_left, b0 = c_b0 - h_b0, b1 = c_b1 - h_b1;
+                    if (l0) atomicAdd((unsigned long long *)&fw.left[dt], (unsigned long long)l0);
+                    if (b0) atomicAdd(&fw.below2[2 * dt], b0);
+                    if (b1) atomicAdd(&fw.below2[2 * dt + 1], b1);
+                    if (h_left) atomicAdd((unsigned long long *)&fw.left[dt + 1], (unsigned long long)h_left);
+                    if (h_b0) atomicAdd(&fw.below2[2 * dt + 2], h_b0);
+                    if (h_b1) atomicAdd(&fw.below2[2 * dt + 3], h_b1);
+                }
             }
-#pragma unroll
-            for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) sw[cp[k]] = wv[k];
-            __syncthreads();
-            COUPE_STORE4(ow[d] = sw[pos])
-            __syncthreads();
         }
-#undef COUPE_STORE4
+        __syncthreads();
     }
-    if (cur != 0xffffffffu) flush_grand_bins(r, sh, g, bcnt, bw, bmin);
+    if (cur != 0xffffffffu) flush();
 }
 
 }  // namespace coupe_hip
