@@ -83,6 +83,9 @@ class Stats(C.Structure):
         ("reruns", C.c_uint32),
         ("unit_points", C.c_uint64),
         ("device_bytes", C.c_uint64),
+        ("first_read", C.c_uint64),
+        ("forwarded", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
     def as_dict(self):

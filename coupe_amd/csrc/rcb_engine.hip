@@ -110,6 +110,8 @@ struct Counters {
     uint64_t min_id;
     unsigned long long swept;                   // points swept by standalone passes
     unsigned long long swept_fused;             // points binned inside partitions
+    unsigned long long first_read;              // points read by the children's first passes
+    uint32_t forwarded;                         // parts whose children's first pass the move forwarded
     uint32_t unfinished;                        // a speculative search did not end in its passes
     uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
     unsigned long long ticket[kMaxTickets];     // partition ticket counter, one per level
@@ -479,7 +481,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_children_write<Acc><<<(unsigned)pub, 64, 0, st>>>(
                     pf, &ctr->n_parts[from], D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr);
+                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr,
+                    &ctr->forwarded);
                 if (pub > kPlanParts)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[to],
                                                   &ctr->n_tiles[to]);
@@ -503,10 +506,12 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // the children's first pass over the parent's tiles
             {
                 const unsigned g = persistent_grid(cx, k_child_first<D, WS, Acc>, tl_ub);
-                // parent-axis and child-axis coordinates and the weight
-                timed(cx, st, COUPE_HIP_K_HIST, (8.0 + sizeof(WS)) * (double)n, [&] {
+                // bytes: parent-axis and child-axis coordinates and the weight
+                // of every point read (counted on the device, added below)
+                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
                     k_child_first<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, cl[0], cl[1], cl[2], wl, bcnt, bw,
-                                                                   bmin, pb2, pl, fwd_in ? 1u : 0u);
+                                                                   bmin, pb2, pl, fwd_in ? 1u : 0u,
+                                                                   &ctr->first_read);
                 });
             }
             const uint64_t pub1 = std::max<uint64_t>(B.parts_ub[level + 1], 1);
@@ -590,7 +595,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                     parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                     (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
                     node_base[std::min(level + 1, L)], heap_split, insts, Forward<Acc>{}, bcnt, bw, bmin,
-                    Forward<Acc>{});
+                    Forward<Acc>{}, &ctr->forwarded);
                 if (!plan1)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                                   &ctr->n_tiles[nxt]);
@@ -669,7 +674,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     check(hipStreamSynchronize(st));
     const Counters *h = (const Counters *)(h_rb - rb_off);   // only the copied fields are read
     if (h->error) throw HipFailure{hipErrorUnknown};
-    cx.swept_bytes = (double)h->swept * (4.0 + sizeof(WS));
+    cx.swept_bytes = (double)h->swept * (4.0 + sizeof(WS)) + (double)h->first_read * (8.0 + sizeof(WS));
+    cx.stats.first_read = h->first_read;
+    cx.stats.forwarded = h->forwarded;
     cx.bins_clean = true;
     if (spec_passes > 0) {
         // passes per level: the first decision plus every pass whose gate was open

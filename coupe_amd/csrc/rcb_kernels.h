@@ -723,7 +723,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
                  uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split,
                  Instance *__restrict__ insts, Forward<Acc> fsrc, uint32_t *__restrict__ bcnt,
-                 Acc *__restrict__ bw, uint32_t *__restrict__ bmin, Forward<Acc> fclr) {
+                 Acc *__restrict__ bw, uint32_t *__restrict__ bmin, Forward<Acc> fclr, uint32_t *n_forwarded) {
     // fsrc (non-null): a forwarded part's children get their first-pass bins
     // from it.  fclr (non-null): the new parts' forwarding sums and their
     // tiles' counts are cleared for the move that will fill them.
@@ -732,6 +732,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     const Part P = parts[s];
     if (P.status == kDone) return;
     const bool fwd_in = fsrc.cnt != nullptr && forwarded(P);
+    if (fwd_in && threadIdx.x == 0 && P.depth + 1 < P.levels) atomicAdd(n_forwarded, 1u);
     // A part at its instance's last level has leaves, not parts (:589-601).
     const bool inner = P.depth + 1 < P.levels;
     const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;

@@ -50,7 +50,7 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
               const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
               const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
               uint32_t *__restrict__ bmin, uint32_t *__restrict__ below2, uint64_t *__restrict__ tile_left,
-              uint32_t fwd_ok) {
+              uint32_t fwd_ok, unsigned long long *points_read) {
     __shared__ ChildBinsShared<Acc> sh;
     __shared__ uint32_t s4[3 * (kBlock / 64)];
     const int tid = threadIdx.x;
@@ -62,12 +62,14 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     const float *ca = c0, *cb = c1;
     float t2 = 0.f, split = 0.f;
     bool all_left = false;
+    uint64_t read = 0;   // points of the tiles this workgroup reads
     uint32_t t_first, t_last;
     tile_range(nt, t_first, t_last);
     for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (!children_searched(P) || (fwd_ok && forwarded(P))) continue;   // forwarded: nothing to read
+        read += T.count;
         if (T.part != cur) {
             if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
             cur = T.part;
@@ -120,6 +122,7 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
         }
         __syncthreads();
     }
+    if (tid == 0 && read) atomicAdd(points_read, (unsigned long long)read);
     if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
 }
 

@@ -210,6 +210,9 @@ struct coupe_hip_stats {
 	uint32_t reruns;          /* 1 when a split search outran the queued passes and the call ran again */
 	uint64_t unit_points;     /* points per work unit */
 	uint64_t device_bytes;    /* device memory held by the context */
+	uint64_t first_read;      /* points read by the children's first passes (two levels per move) */
+	uint32_t forwarded;       /* parts whose children's first pass came with the move that filled them */
+	uint32_t reserved;
 };
 enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out);
 

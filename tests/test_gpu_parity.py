@@ -198,6 +198,46 @@ def test_move_every_level_or_every_other(oracle_mod, monkeypatch, pairs):
         ctx.close()
 
 
+@pytest.mark.parametrize("forward", ["0", "1"])
+def test_move_forwards_the_next_first_pass(oracle_mod, monkeypatch, forward):
+    """The move to level l + 2 forwards the first pass of that level's
+    children (COUPE_HIP_FORWARD, default on): parts whose split ends at their
+    first candidate skip the children's read pass, the others read it.  Cases
+    mix both (grids and uniform clouds accept the midpoint, skewed clouds do
+    not), with parts of many tiles whose runs cross destination tiles."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_FORWARD", forward)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(1300)
+        seen_forwarded = seen_read = 0
+        for q in range(8):
+            n = int(rng.integers(20_000, 700_000))
+            dim = 2 + q % 2
+            pts = [rng.random((n, dim)), rng.normal(0, 1, (n, dim)) ** 3,
+                   (np.round(rng.random((n, dim)) * 64) + 0.5) / 64,
+                   np.concatenate([rng.random((n // 2, dim)), rng.normal(0.3, 0.01, (n - n // 2, dim))])][q % 4]
+            w = rng.integers(1, 1000, n).astype([np.int32, np.int64, np.float64][q % 3])
+            iters, tol = int(rng.integers(4, 14)), float(rng.choice([0.0, 0.01, 0.05, 0.2]))
+            say(f"forward {forward}, case {q}: n={n} d={dim} L={iters} tol={tol}")
+            out = torch.zeros(n, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            st = ctx.stats()
+            seen_forwarded += st["forwarded"]
+            seen_read += st["first_read"]
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"forward {forward} case {q}"
+        assert seen_read > 0
+        if forward == "1":
+            assert seen_forwarded > 0
+        else:
+            assert seen_forwarded == 0
+    finally:
+        ctx.close()
+
+
 def test_edge_cases(oracle_mod):
     rng = np.random.default_rng(7)
     one = rng.random((1, 3))
