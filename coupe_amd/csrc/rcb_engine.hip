@@ -546,12 +546,16 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             const int csn = csl == 1 ? 2 : 1, wsn = wsl ^ 1;
             float *const *co = cs[csn];
             {
-                const unsigned g = persistent_grid(cx, k_partition4<D, WS, Acc>, tl_ub);
-                timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-                    k_partition4<D, WS, Acc><<<g, kBlock, 0, st>>>(
-                        pa, pb, (const Part *)cx.parts[ic].ptr, tl, ntl, q4[0], q4[1], q4[2], q4[3], cl[0], cl[1],
-                        cl[2], wl, co[0], co[1], co[2], ws[wsn], n, bcnt, bw, bmin, fwd_out ? fwd_at(par ^ 1) : none);
-                });
+                auto move = [&](auto kernel) {
+                    const unsigned g = persistent_grid(cx, kernel, tl_ub);
+                    timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                        kernel<<<g, kBlock, 0, st>>>(pa, pb, (const Part *)cx.parts[ic].ptr, tl, ntl, q4[0], q4[1],
+                                                     q4[2], q4[3], cl[0], cl[1], cl[2], wl, co[0], co[1], co[2],
+                                                     ws[wsn], n, bcnt, bw, bmin, fwd_out ? fwd_at(par ^ 1) : none);
+                    });
+                };
+                if (fwd_out) move(k_partition4<D, WS, Acc, true>);
+                else move(k_partition4<D, WS, Acc, false>);
             }
             // the grandchildren's search over their own tiles
             const uint64_t pub2 = std::max<uint64_t>(B.parts_ub[level + 2], 1);

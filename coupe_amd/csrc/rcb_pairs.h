@@ -271,11 +271,10 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 // coordinates of the grandchildren's axis (a3) and of their children's axis
 // (a4) with the weight.  The second phase also sums, per wavefront and across
 // the tiles of one parent, the grandchild's first pass (count, weight and
-// smallest key at its first candidate t3) and, with fw.cnt non-null, the
-// forwarding of its children's first pass (per side of t3: count, weight and
+// smallest key at its first candidate t3) and, with FWD, the forwarding of its children's first pass (per side of t3: count, weight and
 // smallest key at their first candidate t4; per destination tile: the count
 // below t3 and per side the count below t4).
-template <int D, typename WS, typename Acc>
+template <int D, typename WS, typename Acc, bool FWD>
 __global__ void __launch_bounds__(kBlock)
 k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
              const Tile *__restrict__ tiles, const uint32_t *n_tiles, const uint64_t *__restrict__ p0,
@@ -296,7 +295,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nt = *n_tiles;
-    const bool fwd = fw.cnt != nullptr;
     uint32_t cur = 0xffffffffu;
     float gsplit[2] = {0.f, 0.f};
     bool gall[2] = {true, true};
@@ -327,7 +325,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
         gws = (Acc)0;
         gmn = 0xffffffffu;
         gcnt = 0;
-        if (!fwd) return;
+        if (!FWD) return;
         const Acc x0 = wave_sum(fws0), x1 = wave_sum(fws1);
         const uint32_t m0 = wave_min(fmn0), m1 = wave_min(fmn1);
         if (lane == 0) {
@@ -485,52 +483,66 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
         {
             float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
             float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
-            const uint32_t rb = s_run[wid], len = s_run[wid + 1] - rb;
+            const uint32_t rb = s_run[wid];
             const uint64_t db = s_dst[wid];
+            uint32_t len = s_run[wid + 1] - rb;
+            len = db + len <= n_total ? len : (db < n_total ? (uint32_t)(n_total - db) : 0u);   // never past the arrays
             const uint32_t split_i = kTile - s_off[wid];   // positions from here on are in the next tile
             uint32_t c_left = 0, c_b0 = 0, c_b1 = 0, h_left = 0, h_b0 = 0, h_b1 = 0;
-            for (uint32_t i = lane; i < len; i += 64) {
-                const uint64_t d = db + i;
-                const float xa = sfa[rb + i], xb = sfb[rb + i];
-                const WS wq = sws[rb + i];
-                if (d < n_total) {
-                    oa3[d] = xa;
-                    oa4[d] = xb;
-                    ow[d] = wq;
+            // two positions per lane per iteration, every lane runs every
+            // iteration (validity in the masks)
+            for (uint32_t base = 0; base < len; base += 128) {
+                float xa[2], xb[2];
+                WS wq[2];
+                bool v[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
+                    v[u] = i < len;
+                    const uint32_t c = rb + (v[u] ? i : 0u);
+                    xa[u] = sfa[c];
+                    xb[u] = sfb[c];
+                    wq[u] = sws[c];
                 }
-                const bool bel3 = !(xa >= t3w);
-                const uint64_t bb = __ballot(bel3);
-                const Acc wa = AccOps<Acc>::of(wq);
-                // the grandchild's first pass
-                c_left += __popcll(bb);
-                gws += bel3 ? wa : (Acc)0;
-                const uint32_t ka = bel3 ? 0xffffffffu : f32_key(xa);
-                gmn = ka < gmn ? ka : gmn;
-                if (fwd) {
-                    const bool below4 = !(xb >= t4w);
-                    const uint64_t b4 = __ballot(below4), act = __ballot(true);
-                    // lanes at or past split_i lie in the next destination tile
-                    const uint32_t ln = i - lane;   // this iteration's first position
-                    const uint64_t hi_m = split_i <= ln ? act
-                                          : (split_i - ln >= 64 ? 0ull : act & ~(~0ull >> (64 - (split_i - ln))));
-                    c_b0 += __popcll(bb & b4);
-                    c_b1 += __popcll(~bb & act & b4);
-                    h_left += __popcll(bb & hi_m);
-                    h_b0 += __popcll(bb & b4 & hi_m);
-                    h_b1 += __popcll(~bb & b4 & hi_m);
-                    const uint32_t key = f32_key(xb);
-                    fws0 += (bel3 && below4) ? wa : (Acc)0;
-                    fws1 += (!bel3 && below4) ? wa : (Acc)0;
-                    const uint32_t k0 = (bel3 && !below4) ? key : 0xffffffffu;
-                    const uint32_t k1 = (!bel3 && !below4) ? key : 0xffffffffu;
-                    fmn0 = k0 < fmn0 ? k0 : fmn0;
-                    fmn1 = k1 < fmn1 ? k1 : fmn1;
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
+                    if (v[u]) {
+                        const uint64_t d = db + i;
+                        oa3[d] = xa[u];
+                        oa4[d] = xb[u];
+                        ow[d] = wq[u];
+                    }
+                    const bool bel3 = v[u] && !(xa[u] >= t3w);
+                    const uint64_t bb = __ballot(bel3);
+                    const Acc wa = AccOps<Acc>::of(wq[u]);
+                    // the grandchild's first pass
+                    c_left += __popcll(bb);
+                    gws += bel3 ? wa : (Acc)0;
+                    const uint32_t ka = (v[u] && !bel3) ? f32_key(xa[u]) : 0xffffffffu;
+                    gmn = ka < gmn ? ka : gmn;
+                    if constexpr (FWD) {
+                        const bool below4 = v[u] && !(xb[u] >= t4w);
+                        const uint64_t b4 = __ballot(below4);
+                        const uint64_t hm = __ballot(v[u] && i >= split_i);   // in the next destination tile
+                        c_b0 += __popcll(bb & b4);
+                        c_b1 += __popcll(~bb & b4);
+                        h_left += __popcll(bb & hm);
+                        h_b0 += __popcll(bb & b4 & hm);
+                        h_b1 += __popcll(~bb & b4 & hm);
+                        const uint32_t key = f32_key(xb[u]);
+                        fws0 += (bel3 && below4) ? wa : (Acc)0;
+                        fws1 += (!bel3 && below4) ? wa : (Acc)0;
+                        const bool side1 = v[u] && !bel3;
+                        const uint32_t k0 = (bel3 && !below4) ? key : 0xffffffffu;
+                        const uint32_t k1 = (side1 && !below4) ? key : 0xffffffffu;
+                        fmn0 = k0 < fmn0 ? k0 : fmn0;
+                        fmn1 = k1 < fmn1 ? k1 : fmn1;
+                    }
                 }
             }
-            // The counts come from ballots, so every lane that ran the last
-            // iteration holds them; lane 0 always runs it.
             gcnt += c_left;
-            if (fwd && len > 0) {
+            if (FWD && len > 0) {
                 fcn0 += c_b0;
                 fcn1 += c_b1;
                 if (lane == 0) {
