@@ -242,6 +242,26 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     }
     cx.nchild.ensure(parts_cap * sizeof(uint64_t));
     cx.ntiles.ensure(parts_cap * sizeof(uint64_t));
+    // Two levels per move: per-tile counts (two buffers, alternating per
+    // move) and the children's first passes forwarded by the moves (and, at
+    // level 0, by the roots' first pass).
+    const bool root_fwd = pairs && cx.forward && L > 1;
+    if (pairs) {
+        for (int b = 0; b < 2; b++) {
+            cx.pleft[b].ensure(tiles_cap * sizeof(uint64_t));
+            cx.pbelow2[b].ensure(tiles_cap * 2 * sizeof(uint32_t));
+        }
+        cx.fwd_cnt.ensure(parts_cap * 2 * sizeof(uint32_t));
+        cx.fwd_w.ensure(parts_cap * 2 * sizeof(Acc));
+        cx.fwd_mn.ensure(parts_cap * 2 * sizeof(uint32_t));
+        cx.tile_bins2.ensure(tiles_cap * 2 * kMaxBins * sizeof(uint16_t));
+        cx.tile4.ensure(tiles_cap * 4 * sizeof(uint64_t));
+    }
+    const Forward<Acc> none{nullptr, nullptr, nullptr, nullptr, nullptr};
+    auto fwd_at = [&](int b) {
+        return Forward<Acc>{(uint32_t *)cx.fwd_cnt.ptr, (Acc *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr,
+                            (uint64_t *)cx.pleft[b].ptr, (uint32_t *)cx.pbelow2[b].ptr};
+    };
     // Split tree for the descent: a dense heap per instance for shallow
     // trees; node records with child links otherwise (single instance).
     const bool dense = B.dense;
@@ -279,7 +299,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // the entries of the next level's tiles and k_tile_left clears what it
     // reads, but the last level's counts are never read (no partition), so
     // a call cannot rely on the previous one leaving them clear.
-    if (!lookback) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
+    if (!lookback && !pairs) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
     const bool fresh_status = cx.status.bytes < tiles_cap * 8;
     cx.status.ensure(tiles_cap * 8);
     if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(L, 1u << 20) - 2) {
@@ -322,8 +342,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     const double payload = 4.0 * D + sizeof(WS);   // bytes of one point in the working arrays
     check(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(insts, n_inst, (Part *)cx.parts[0].ptr, D,
-                                                                    real_mode);
+        k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(
+            insts, n_inst, (Part *)cx.parts[0].ptr, D, real_mode, root_fwd ? (uint32_t *)cx.fwd_cnt.ptr : nullptr,
+            (uint64_t *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr);
     });
 
     cx.stats.levels = 0;
@@ -418,7 +439,16 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     }
     uint64_t parts_ub = n_inst;
     uint64_t tiles_ub = std::max<uint64_t>(B.tiles0, 1);
-    if (L > 0) {
+    if (L > 0 && root_fwd) {
+        const unsigned g = persistent_grid(cx, k_root_first<D, WS, Acc>, tiles_ub);
+        // the roots' first pass with their children's: two axes and the weight
+        timed(cx, st, COUPE_HIP_K_HIST, (8.0 + sizeof(WS)) * (double)n, [&] {
+            k_root_first<D, WS, Acc><<<g, kBlock, 0, st>>>((const Part *)cx.parts[0].ptr, tiles0,
+                                                          &ctr->n_tiles[0], cs[0][0], cs[0][1], cs[0][2], ws[0],
+                                                          bcnt, bw, bmin, fwd_at(0));
+        });
+        search(0, 0, parts_ub, tiles_ub, n_inst, true);
+    } else if (L > 0) {
         const unsigned g = persistent_grid(cx, k_sweep<WS, Acc>, tiles_ub);
         // the roots' first pass: split-axis coordinate and weight of every point
         timed(cx, st, COUPE_HIP_K_HIST, (4.0 + sizeof(WS)) * (double)n, [&] {
@@ -435,26 +465,12 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         // contiguous points (table ia, tiles tl, coordinates cl, weights wl).
         // Their children (table ib) are searched in place over the parent's
         // tiles; the grandchildren (table ic) receive the points.
-        for (int b = 0; b < 2; b++) {
-            cx.pleft[b].ensure(tiles_cap * sizeof(uint64_t));
-            cx.pbelow2[b].ensure(tiles_cap * 2 * sizeof(uint32_t));
-        }
-        cx.fwd_cnt.ensure(parts_cap * 2 * sizeof(uint32_t));
-        cx.fwd_w.ensure(parts_cap * 2 * sizeof(Acc));
-        cx.fwd_mn.ensure(parts_cap * 2 * sizeof(uint32_t));
-        cx.tile_bins2.ensure(tiles_cap * 2 * kMaxBins * sizeof(uint16_t));
-        cx.tile4.ensure(tiles_cap * 4 * sizeof(uint64_t));
         uint16_t *tile_bins2 = (uint16_t *)cx.tile_bins2.ptr;
         // Forwarding: the move to level l + 2 also sums, for each new part,
         // its children's first pass (per side of its first candidate) and
         // each of its tiles' counts; a part whose split ends at its first
         // candidate then needs no first pass of its children.  Per-tile
         // counts alternate between two buffers (par: the current level's).
-        const Forward<Acc> none{nullptr, nullptr, nullptr, nullptr, nullptr};
-        auto fwd_at = [&](int b) {
-            return Forward<Acc>{(uint32_t *)cx.fwd_cnt.ptr, (Acc *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr,
-                                (uint64_t *)cx.pleft[b].ptr, (uint32_t *)cx.pbelow2[b].ptr};
-        };
         int par = 0;
         uint64_t *q4[4];
         int ia = 0, csl = 0, wsl = 0;
@@ -497,7 +513,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             WS *wl = ws[wsl];
             cx.stats.levels++;
             // the move that made this level forwarded its children's first pass
-            const bool fwd_in = cx.forward && level >= 2 && level + 1 < L;
+            const bool fwd_in = cx.forward && (level >= 2 || root_fwd) && level + 1 < L;
             uint64_t *pl = (uint64_t *)cx.pleft[par].ptr;
             uint32_t *pb2 = (uint32_t *)cx.pbelow2[par].ptr;
             children_of(ia, ib, level, nullptr, fwd_in ? fwd_at(par) : none, none);   // in place: no tiles

@@ -890,9 +890,14 @@ k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restri
 // prologue's per-instance reductions; the id offset starts at "none".
 __global__ void __launch_bounds__(kBlock)
 k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts, int dim,
-                 int real_mode) {
+                 int real_mode, uint32_t *fwd_cnt, uint64_t *fwd_w, uint32_t *fwd_mn) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= n_inst) return;
+    if (fwd_cnt) {   // the roots' forwarded sums start empty (any accumulator: zero bits)
+        fwd_cnt[2 * q] = fwd_cnt[2 * q + 1] = 0;
+        fwd_w[2 * q] = fwd_w[2 * q + 1] = 0;
+        fwd_mn[2 * q] = fwd_mn[2 * q + 1] = 0xffffffffu;
+    }
     Instance &I = insts[q];
     Part &R = parts[q];
     R.sum = real_mode ? __builtin_bit_cast(uint64_t, I.sum_f64) : I.sum_u64;

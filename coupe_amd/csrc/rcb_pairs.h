@@ -68,7 +68,11 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
-        if (!children_searched(P) || (fwd_ok && forwarded(P))) continue;   // forwarded: nothing to read
+        if (!children_searched(P) || (fwd_ok && forwarded(P))) {   // forwarded: nothing to read
+            const uint32_t last = P.tile_begin + P.tile_count - 1;   // the part's tiles are contiguous
+            if (P.tile_count > 0 && last > t) t = last;
+            continue;
+        }
         read += T.count;
         if (T.part != cur) {
             if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
@@ -124,6 +128,136 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
     }
     if (tid == 0 && read) atomicAdd(points_read, (unsigned long long)read);
     if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
+}
+
+// The roots' first pass (level 0) with their children's first pass
+// forwarded: the root's 3 candidates along its axis in registers (as
+// k_sweep), and, for a split at its first candidate e1 (the box midpoint),
+// per side of e1 the count and weight below the children's first candidate
+// (the midpoint of the box on the next axis) and the smallest key at or
+// above it, and per tile the count below e1 and per side the count below the
+// children's candidate.  Reads the two axes and the weight once.
+template <int D, typename WS, typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_root_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
+             const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
+             const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
+             uint32_t *__restrict__ bmin, Forward<Acc> fw) {
+    __shared__ RegBinsShared<kFirstBins, Acc> sh;
+    __shared__ ChildBinsShared<Acc> shc;
+    __shared__ uint32_t s4[3 * (kBlock / 64)];
+    const int tid = threadIdx.x, lane = lane_id();
+    const uint32_t nt = *n_tiles;
+    RegBins<kFirstBins, Acc> r;
+    r.reset();
+    ChildBins<Acc> f;
+    f.reset();
+    uint32_t cur = 0xffffffffu;
+    bool active = false;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, t2 = 0.f;
+    const float *ca = c0, *cb = c1;
+    // forwarded sums of a root: fw.* [2 root + side]
+    auto flush_fwd = [&](uint32_t p) {
+        const uint32_t x0 = wave_sum(f.cnt0), x1 = wave_sum(f.cnt1);
+        const Acc y0 = wave_sum(f.w0), y1 = wave_sum(f.w1);
+        const uint32_t m0 = wave_min(f.mn0), m1 = wave_min(f.mn1);
+        const int wid = tid >> 6;
+        if (lane == 0) {
+            shc.cnt[wid][0] = x0; shc.cnt[wid][1] = x1;
+            shc.w[wid][0] = y0; shc.w[wid][1] = y1;
+            shc.mn[wid][0] = m0; shc.mn[wid][1] = m1;
+        }
+        __syncthreads();
+        if (tid < 2) {
+            uint32_t c = 0, m = 0xffffffffu;
+            Acc x = (Acc)0;
+#pragma unroll
+            for (int v = 0; v < kBlock / 64; v++) {
+                c += shc.cnt[v][tid];
+                x += shc.w[v][tid];
+                m = shc.mn[v][tid] < m ? shc.mn[v][tid] : m;
+            }
+            const uint64_t b = 2 * (uint64_t)p + tid;
+            if (c) { atomicAdd(&fw.cnt[b], c); atomic_add_acc(&fw.w[b], x); }
+            if (m != 0xffffffffu) atomicMin(&fw.mn[b], m);
+        }
+        __syncthreads();
+    };
+    uint32_t t_first, t_last;
+    tile_range(nt, t_first, t_last);
+    for (uint32_t t = t_first; t < t_last; t++) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {
+            if (active) {
+                flush_bins<kFirstBins, kFirstBins>(r, sh, cur, -1, bcnt, bw, bmin);
+                r.reset();
+                flush_fwd(cur);
+                f.reset();
+            }
+            cur = T.part;
+            const Part &P = parts[cur];
+            active = P.status == kSearching;
+            e0 = candidate(P.blo, P.bhi, 0, kFirstBins);
+            e1 = candidate(P.blo, P.bhi, 1, kFirstBins);
+            e2 = candidate(P.blo, P.bhi, 2, kFirstBins);
+            const int ax = (int)P.axis, a2 = (int)((P.axis + 1) % D);
+            ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+            cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+            t2 = (sel3(a2, P.lo[0], P.lo[1], P.lo[2]) + sel3(a2, P.hi[0], P.hi[1], P.hi[2])) / 2.0f;
+        }
+        if (!active) continue;
+        float xa[kItems], xb[kItems];
+        WS wk[kItems];
+        const uint32_t cnt = T.count;
+        {
+            const float *ba = ca + T.begin, *bb = cb + T.begin;
+            const WS *bwt = w + T.begin;
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                const uint32_t p = (uint32_t)(k * kBlock + tid);
+                const uint32_t c = p < cnt ? p : cnt - 1;   // clamped: loads issue back to back
+                xa[k] = ba[c];
+                xb[k] = bb[c];
+                wk[k] = bwt[c];
+            }
+        }
+        uint32_t nleft = 0, b0 = 0, b1 = 0;   // wave-uniform counts from ballots
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const bool valid = (uint32_t)(k * kBlock + tid) < cnt;
+            const Acc wa = AccOps<Acc>::of(wk[k]);
+            const bool right = xa[k] >= e1;
+            const bool bel = !(xb[k] >= t2);
+            if (valid) {
+                r.add(bin4(e0, e1, e2, xa[k]), wa, f32_key(xa[k]));
+                f.add(right, bel, wa, f32_key(xb[k]));
+            }
+            const uint64_t bv = __ballot(valid), br = __ballot(right), bb = __ballot(bel);
+            nleft += __popcll(bv & ~br);
+            b0 += __popcll(bv & ~br & bb);
+            b1 += __popcll(bv & br & bb);
+        }
+        const int wid = tid >> 6;
+        if (lane == 0) {
+            s4[wid * 3] = nleft;
+            s4[wid * 3 + 1] = b0;
+            s4[wid * 3 + 2] = b1;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t a = 0, c = 0, e = 0;
+#pragma unroll
+            for (int v = 0; v < kBlock / 64; v++) { a += s4[3 * v]; c += s4[3 * v + 1]; e += s4[3 * v + 2]; }
+            fw.left[t] = a;
+            fw.below2[2 * (uint64_t)t] = c;
+            fw.below2[2 * (uint64_t)t + 1] = e;
+        }
+        __syncthreads();
+    }
+    if (active) {
+        flush_bins<kFirstBins, kFirstBins>(r, sh, cur, -1, bcnt, bw, bmin);
+        flush_fwd(cur);
+    }
 }
 
 // Later passes of in-place children: LDS bins of both children of the tile's
@@ -344,7 +478,11 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     for (uint32_t t = t_first; t < t_last; t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
-        if (!pair_active(P)) continue;
+        if (!pair_active(P)) {   // nothing of this part moves: skip its tiles
+            const uint32_t last = P.tile_begin + P.tile_count - 1;
+            if (P.tile_count > 0 && last > t) t = last;
+            continue;
+        }
         if (T.part != cur) {
             if (cur != 0xffffffffu) flush();
             cur = T.part;
