@@ -399,11 +399,9 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 // of the per-tile counts of grandchild slots q = 0..3.
 //
 // Per tile: the points are classified (three ballots give each point's slot
-// and rank), one wavefront scans the slot counts, and the points are stored
-// through LDS in two phases: first the coordinate of the children's axis (3D
-// only), all threads, run by run; then, one wavefront per grandchild slot, the
-// coordinates of the grandchildren's axis (a3) and of their children's axis
-// (a4) with the weight.  The second phase also sums, per wavefront and across
+// and rank), one wavefront scans the slot counts, every array is staged in
+// LDS in slot order, and wavefront j stores run j (slot j's points, one
+// contiguous range of its grandchild).  That phase also sums, per wavefront and across
 // the tiles of one parent, the grandchild's first pass (count, weight and
 // smallest key at its first candidate t3) and, with FWD, the forwarding of its children's first pass (per side of t3: count, weight and
 // smallest key at their first candidate t4; per destination tile: the count
@@ -418,7 +416,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
              WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
              uint32_t *__restrict__ bmin, Forward<Acc> fw) {
     static_assert(kBlock / 64 == 4, "one wavefront per grandchild slot");
-    __shared__ float sfa[kTile], sfb[kTile];
+    __shared__ float sfa[kTile], sfb[kTile], sfc[D == 3 ? kTile : 1];
     __shared__ WS sws[kTile];
     __shared__ uint32_t wtot[4][kItems * (kBlock / 64)];
     __shared__ uint32_t s_run[5];
@@ -591,34 +589,18 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const uint32_t q = (pq[k] >> 8) < 4 ? (pq[k] >> 8) : 0u;
             cp[k] = wtot[q][k * (kBlock / 64) + wid] + (pq[k] & 0xffu);   // includes the run start of q
         }
-        if constexpr (D == 3) {
-            // phase 1: the children's axis, all threads, run by run
-            float *oa2 = a2 == 0 ? o0 : (a2 == 1 ? o1 : o2);
-#pragma unroll
-            for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) sfa[cp[k]] = sel3(a2, x0[k], x1[k], x2[k]);
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t rb = s_run[j], len = s_run[j + 1] - rb;
-                const uint64_t db = s_dst[j];
-                for (uint32_t i = tid; i < len; i += kBlock) {
-                    const uint64_t d = db + i;
-                    if (d < n_total) oa2[d] = sfa[rb + i];
-                }
-            }
-            __syncthreads();
-        }
-        // phase 2: a3, a4 and the weight; wavefront j stores run j
+        // every array staged in slot order, then wavefront j stores run j
 #pragma unroll
         for (int k = 0; k < kItems; k++)
             if (k * kBlock + tid < (int)cnt) {
                 sfa[cp[k]] = sel3(a3, x0[k], x1[k], x2[k]);
                 sfb[cp[k]] = sel3(a4, x0[k], x1[k], x2[k]);
+                if constexpr (D == 3) sfc[cp[k]] = sel3(a2, x0[k], x1[k], x2[k]);
                 sws[cp[k]] = wv[k];
             }
         __syncthreads();
         {
+            float *oa2 = a2 == 0 ? o0 : (a2 == 1 ? o1 : o2);
             float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
             float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
             const uint32_t rb = s_run[wid];
@@ -630,7 +612,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             // two positions per lane per iteration, every lane runs every
             // iteration (validity in the masks)
             for (uint32_t base = 0; base < len; base += 128) {
-                float xa[2], xb[2];
+                float xa[2], xb[2], xc[2];
                 WS wq[2];
                 bool v[2];
 #pragma unroll
@@ -640,6 +622,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     const uint32_t c = rb + (v[u] ? i : 0u);
                     xa[u] = sfa[c];
                     xb[u] = sfb[c];
+                    if constexpr (D == 3) xc[u] = sfc[c];
                     wq[u] = sws[c];
                 }
 #pragma unroll
@@ -647,6 +630,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
                     if (v[u]) {
                         const uint64_t d = db + i;
+                        if constexpr (D == 3) oa2[d] = xc[u];
                         oa3[d] = xa[u];
                         oa4[d] = xb[u];
                         ow[d] = wq[u];
