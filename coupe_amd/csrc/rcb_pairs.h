@@ -521,22 +521,26 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
         // the load overlaps the points' and is consumed after the classification
         const uint64_t pv = pj[t];
         const uint32_t cnt = T.count;
-        float x0[kItems], x1[kItems], x2[kItems];
+        // the coordinates by role: rx on the parent's axis, ry on the
+        // children's, rz on the grandchildren's (3D; in 2D it is rx's axis)
+        float rx[kItems], ry[kItems], rz[kItems];
         WS wv[kItems];
         {
             // clamped, unconditional loads from the tile's base with 32-bit
             // offsets: all of them issue back to back (the values past the
             // tile's end are never used)
-            const float *b0 = i0 + T.begin, *b1 = i1 + T.begin, *b2 = D == 3 ? i2 + T.begin : i2;
+            const float *bx = (ax == 0 ? i0 : (ax == 1 ? i1 : i2)) + T.begin;
+            const float *by = (a2 == 0 ? i0 : (a2 == 1 ? i1 : i2)) + T.begin;
+            const float *bz = (a3 == 0 ? i0 : (a3 == 1 ? i1 : i2)) + T.begin;
             const WS *bwt = iw + T.begin;
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const uint32_t p = (uint32_t)(k * kBlock + tid);
                 const uint32_t c = p < cnt ? p : cnt - 1;
-                x0[k] = b0[c];
-                x1[k] = b1[c];
-                if constexpr (D == 3) x2[k] = b2[c];
-                else x2[k] = 0.f;
+                rx[k] = bx[c];
+                ry[k] = by[c];
+                if constexpr (D == 3) rz[k] = bz[c];
+                else rz[k] = 0.f;
                 wv[k] = bwt[c];
             }
         }
@@ -544,8 +548,8 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const bool valid = (uint32_t)(k * kBlock + tid) < cnt;
-            const bool s1 = !all_left && (sel3(ax, x0[k], x1[k], x2[k]) >= split);
-            const float xb = sel3(a2, x0[k], x1[k], x2[k]);
+            const bool s1 = !all_left && (rx[k] >= split);
+            const float xb = ry[k];
             const bool s2 = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
             const uint32_t q = (s1 ? 2u : 0u) + (s2 ? 1u : 0u);
             // slot masks from three ballots (wave-uniform)
@@ -590,12 +594,18 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             cp[k] = wtot[q][k * (kBlock / 64) + wid] + (pq[k] & 0xffu);   // includes the run start of q
         }
         // every array staged in slot order, then wavefront j stores run j
+        // (3D: a3 = rz, a4 = the parent's axis rx, a2 = ry; 2D: a3 = rx, a4 = ry)
 #pragma unroll
         for (int k = 0; k < kItems; k++)
             if (k * kBlock + tid < (int)cnt) {
-                sfa[cp[k]] = sel3(a3, x0[k], x1[k], x2[k]);
-                sfb[cp[k]] = sel3(a4, x0[k], x1[k], x2[k]);
-                if constexpr (D == 3) sfc[cp[k]] = sel3(a2, x0[k], x1[k], x2[k]);
+                if constexpr (D == 3) {
+                    sfa[cp[k]] = rz[k];
+                    sfb[cp[k]] = rx[k];
+                    sfc[cp[k]] = ry[k];
+                } else {
+                    sfa[cp[k]] = rx[k];
+                    sfb[cp[k]] = ry[k];
+                }
                 sws[cp[k]] = wv[k];
             }
         __syncthreads();
