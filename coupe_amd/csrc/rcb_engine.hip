@@ -495,7 +495,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, &ctr->n_parts[from], pub, &ctr->tiles_total);
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_write<Acc><<<(unsigned)pub, 64, 0, st>>>(
+                k_children_write<Acc><<<(unsigned)pub, kBlock, 0, st>>>(
                     pf, &ctr->n_parts[from], D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                     (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr,
                     &ctr->forwarded);
@@ -553,7 +553,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // tile count) do not change.
             for (int j = 0; j < 4; j++) q4[j] = (uint64_t *)cx.tile4.ptr + (uint64_t)j * tl_ub;
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                const unsigned g = persistent_grid(cx, k_tile_count4<D>, (tl_ub + kBlock / 64 - 1) / (kBlock / 64));
+                const unsigned g = grid_for(tl_ub, kBlock);   // one thread per tile
                 k_tile_count4<D><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
                                                       pl, pb2, tile_bins2, q4[0],
                                                       q4[1], q4[2], q4[3]);
@@ -611,7 +611,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 scan_u64(cx, st, (uint64_t *)cx.ntiles.ptr, np, parts_ub, &ctr->tiles_total);
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_children_write<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                k_children_write<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(
                     parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
                     (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
                     node_base[std::min(level + 1, L)], heap_split, insts, Forward<Acc>{}, bcnt, bw, bmin,
@@ -722,8 +722,12 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
     check(hipMemcpyAsync(cx.prologue.ptr, &init, sizeof init, hipMemcpyHostToDevice, st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
         k_reset_instances<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>((Instance *)cx.instances.ptr, n_inst);
-        k_init_roots<<<n_inst, 64, 0, st>>>((const Instance *)cx.instances.ptr, n_inst, (Part *)cx.parts[0].ptr,
-                                            (Tile *)cx.tiles0.ptr);
+        // blocks (instance, share of its tiles): about 4 tiles per thread
+        uint64_t max_nt = 1;
+        for (const Instance &I : B.inst) max_nt = std::max<uint64_t>(max_nt, (I.count + kTile - 1) / kTile);
+        const unsigned gy = (unsigned)std::min<uint64_t>((max_nt + 4 * kBlock - 1) / (4 * kBlock), 1024);
+        k_init_roots<<<dim3(n_inst, gy), kBlock, 0, st>>>((const Instance *)cx.instances.ptr, n_inst,
+                                                           (Part *)cx.parts[0].ptr, (Tile *)cx.tiles0.ptr);
     });
     const uint32_t nt = (uint32_t)B.tiles0;
     if (nt > 0) {

@@ -717,7 +717,7 @@ k_children_plan(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_
 }
 
 template <typename Acc>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kBlock)
 k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  const uint64_t *__restrict__ child_off, const uint64_t *__restrict__ tile_off,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
@@ -799,7 +799,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                 fclr.mn[2 * co + 1] = 0xffffffffu;
             }
         }
-        for (uint32_t k = threadIdx.x; next_tiles && k < nt; k += 64) {
+        for (uint32_t k = threadIdx.x; next_tiles && k < nt; k += blockDim.x) {
             Tile U;
             U.begin = b + (uint64_t)k * kTile;
             uint64_t rem = cnt - (uint64_t)k * kTile;
@@ -850,14 +850,15 @@ __global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uin
 
 // Root parts, one per instance (part index = instance index), and their
 // tiles.  The box and the sum are set once the prologue has reduced them.
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kBlock)
 k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts,
              Tile *__restrict__ tiles) {
+    // block (q, y): instance q, a share y of its tiles
     const uint32_t q = blockIdx.x;
     if (q >= n_inst) return;
     const Instance I = insts[q];
     const uint32_t nt = tiles_for(I.count);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && blockIdx.y == 0) {
         Part R;
         memset(&R, 0, sizeof R);
         R.begin = I.begin;
@@ -876,7 +877,7 @@ k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restri
         R.tol = I.tol;
         parts[q] = R;
     }
-    for (uint32_t k = threadIdx.x; k < nt; k += 64) {
+    for (uint32_t k = blockIdx.y * blockDim.x + threadIdx.x; k < nt; k += gridDim.y * blockDim.x) {
         Tile U;
         U.begin = I.begin + (uint64_t)k * kTile;
         const uint64_t rem = I.count - (uint64_t)k * kTile;

@@ -339,7 +339,32 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
 }
 
 // Per tile of a parent, the points of each grandchild (q = 2 side + side of
-// the child's split), from the counts the passes kept when they can.
+// the child's split), from the counts the passes kept when they can.  One
+// thread per tile; a side whose count needs the per-tile bins or the points
+// (a child decided in a later pass) is done by the whole wavefront, one such
+// tile after the other.
+template <int D>
+__device__ uint32_t tile_count_slow(const Part &P, const Part &C, int s, uint32_t t, uint64_t tbegin, uint32_t tcount,
+                                    const float *__restrict__ c0, const float *__restrict__ c1,
+                                    const float *__restrict__ c2, const uint16_t *__restrict__ tile_bins2) {
+    const int lane = lane_id();
+    uint32_t k = 0;
+    if (C.counts_ready == kCountsBins) {
+        const uint16_t *tb = tile_bins2 + ((uint64_t)t * 2 + s) * kMaxBins;
+        for (int j = lane; j <= C.kstar; j += 64) k += tb[j];
+    } else {
+        const int ax = (int)P.axis, a2 = (int)C.axis;
+        const float *ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+        const float *cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+        const bool all_left = P.status == kAllLeft;
+        for (uint64_t i = tbegin + lane; i < tbegin + tcount; i += 64) {
+            const int si = (!all_left && (ca[i] >= P.split)) ? 1 : 0;
+            k += (si == s && !(cb[i] >= C.split)) ? 1u : 0u;
+        }
+    }
+    return wave_sum(k);
+}
+
 template <int D>
 __global__ void __launch_bounds__(kBlock)
 k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
@@ -349,43 +374,58 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
               uint64_t *__restrict__ q1, uint64_t *__restrict__ q2, uint64_t *__restrict__ q3) {
     const uint32_t nt = *n_tiles;
     const int lane = lane_id();
-    const uint32_t waves = gridDim.x * (kBlock / 64);
-    for (uint32_t t = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); t < nt; t += waves) {
-        const Tile T = tiles[t];
-        const Part &P = parts[T.part];
+    for (uint32_t base = blockIdx.x * kBlock; base < nt; base += gridDim.x * kBlock) {
+        const uint32_t t = base + threadIdx.x;
         uint64_t out[4] = {0, 0, 0, 0};
-        if (pair_active(P)) {
-            const uint64_t cnt[2] = {tile_left[t], T.count - tile_left[t]};
-            for (int s = 0; s < 2; s++) {
-                if (cnt[s] == 0 || P.child[s] < 0) continue;
-                const Part &C = children[P.child[s]];
-                uint64_t left;
-                if (C.status == kAllLeft) {
-                    left = cnt[s];
-                } else if (C.counts_ready == kCountsBelow) {
-                    left = below2[2 * (uint64_t)t + s];
-                } else if (C.counts_ready == kCountsBins) {
-                    const uint16_t *tb = tile_bins2 + ((uint64_t)t * 2 + s) * kMaxBins;
-                    uint32_t k = 0;
-                    for (int j = lane; j <= C.kstar; j += 64) k += tb[j];
-                    left = wave_sum(k);
-                } else {
-                    const int ax = (int)P.axis, a2 = (int)C.axis;
-                    const float *ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
-                    const float *cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
-                    const bool all_left = P.status == kAllLeft;
-                    uint32_t k = 0;
-                    for (uint64_t i = T.begin + lane; i < T.begin + T.count; i += 64) {
-                        const int si = (!all_left && (ca[i] >= P.split)) ? 1 : 0;
-                        k += (si == s && !(cb[i] >= C.split)) ? 1u : 0u;
+        uint64_t cnt[2] = {0, 0};
+        bool slow0 = false, slow1 = false;
+        Tile T{};
+        if (t < nt) {
+            T = tiles[t];
+            const Part &P = parts[T.part];
+            if (pair_active(P)) {
+                cnt[0] = tile_left[t];
+                cnt[1] = T.count - cnt[0];
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    if (cnt[s] == 0 || P.child[s] < 0) continue;
+                    const Part &C = children[P.child[s]];
+                    uint64_t left;
+                    if (C.status == kAllLeft) {
+                        left = cnt[s];
+                    } else if (C.counts_ready == kCountsBelow) {
+                        left = below2[2 * (uint64_t)t + s];
+                    } else {
+                        if (s == 0) slow0 = true; else slow1 = true;
+                        continue;
                     }
-                    left = wave_sum(k);
+                    out[2 * s] = left;
+                    out[2 * s + 1] = cnt[s] - left;
                 }
-                out[2 * s] = left;
-                out[2 * s + 1] = cnt[s] - left;
             }
         }
-        if (lane == 0) {
+        uint64_t pending = __ballot(slow0 || slow1);
+        while (pending) {   // wave-uniform loop over the lanes with a slow side
+            const int src = __ffsll((unsigned long long)pending) - 1;
+            pending &= pending - 1;
+            const uint32_t ts = __shfl(t, src, 64);
+            const uint32_t tp = __shfl(T.part, src, 64), tc = __shfl(T.count, src, 64);
+            const uint64_t tb = ((uint64_t)__shfl((uint32_t)(T.begin >> 32), src, 64) << 32) |
+                                __shfl((uint32_t)T.begin, src, 64);
+            const uint64_t c0s = ((uint64_t)__shfl((uint32_t)(cnt[0] >> 32), src, 64) << 32) |
+                                 __shfl((uint32_t)cnt[0], src, 64);
+            const bool sl0 = __shfl(slow0 ? 1 : 0, src, 64) != 0, sl1 = __shfl(slow1 ? 1 : 0, src, 64) != 0;
+            const Part &P = parts[tp];
+            if (sl0) {
+                const uint32_t left = tile_count_slow<D>(P, children[P.child[0]], 0, ts, tb, tc, c0, c1, c2, tile_bins2);
+                if (lane == src) { out[0] = left; out[1] = c0s - left; }
+            }
+            if (sl1) {
+                const uint32_t left = tile_count_slow<D>(P, children[P.child[1]], 1, ts, tb, tc, c0, c1, c2, tile_bins2);
+                if (lane == src) { out[2] = left; out[3] = (tc - c0s) - left; }
+            }
+        }
+        if (t < nt) {
             q0[t] = out[0];
             q1[t] = out[1];
             q2[t] = out[2];
@@ -424,8 +464,8 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     __shared__ uint32_t s_off[4], s_tile[4];
     __shared__ uint64_t s_gbeg[4], s_gpj0[4];
     __shared__ uint32_t s_gtb[4];
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int tid = threadIdx.x, lane = lane_id();
+    const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: per-wave state in SGPRs
     const uint32_t nt = *n_tiles;
     uint32_t cur = 0xffffffffu;
     float gsplit[2] = {0.f, 0.f};
@@ -563,7 +603,10 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 wtot[3][e] = __popcll(m3);
             }
             const uint64_t mine = s1 ? (s2 ? m3 : m2) : (s2 ? m1 : m0);
-            pq[k] = (uint32_t)__popcll(mine & lt_mask) | ((valid ? q : 4u) << 8);
+            // rank: the set bits of `mine` below this lane
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
+            pq[k] = rank | ((valid ? q : 4u) << 8);
         }
         if (tid < 4) {
             const uint64_t off = pv - s_gpj0[tid];   // this tile's first position in the grandchild
