@@ -339,28 +339,42 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
 }
 
 // Per tile of a parent, the points of each grandchild (q = 2 side + side of
-// the child's split), from the counts the passes kept when they can.  One
-// thread per tile; a side whose count needs the per-tile bins or the points
-// (a child decided in a later pass) is done by the whole wavefront, one such
-// tile after the other.
-template <int D>
-__device__ uint32_t tile_count_slow(const Part &P, const Part &C, int s, uint32_t t, uint64_t tbegin, uint32_t tcount,
-                                    const float *__restrict__ c0, const float *__restrict__ c1,
-                                    const float *__restrict__ c2, const uint16_t *__restrict__ tile_bins2) {
-    const int lane = lane_id();
+// the child's split), from the counts the passes kept: the count below the
+// child's first candidate, or the sum of the tile's bins up to the stopping
+// candidate of a later pass (kstar + 1 u16 counts, read 8 at a time).  One
+// thread per tile; a side whose count needs the points is done by the whole
+// wavefront, one such tile after the other.
+__device__ __forceinline__ uint32_t sum_tile_bins(const uint16_t *__restrict__ tb, int kstar) {
+    const uint4 *v = reinterpret_cast<const uint4 *>(tb);   // kMaxBins u16 per (tile, side): 16-byte aligned
+    const int nv = (kstar + 1 + 7) / 8;
     uint32_t k = 0;
-    if (C.counts_ready == kCountsBins) {
-        const uint16_t *tb = tile_bins2 + ((uint64_t)t * 2 + s) * kMaxBins;
-        for (int j = lane; j <= C.kstar; j += 64) k += tb[j];
-    } else {
-        const int ax = (int)P.axis, a2 = (int)C.axis;
-        const float *ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
-        const float *cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
-        const bool all_left = P.status == kAllLeft;
-        for (uint64_t i = tbegin + lane; i < tbegin + tcount; i += 64) {
-            const int si = (!all_left && (ca[i] >= P.split)) ? 1 : 0;
-            k += (si == s && !(cb[i] >= C.split)) ? 1u : 0u;
+    for (int i = 0; i < nv; i++) {
+        const uint4 x = v[i];
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int h = 0; h < 8; h++) {
+            const int j = i * 8 + h;
+            const uint32_t c = (w[h >> 1] >> ((h & 1) * 16)) & 0xffffu;
+            k += j <= kstar ? c : 0u;
         }
+    }
+    return k;
+}
+
+// The count of side s of a tile below the child's split, from the points
+// (the whole wavefront).
+__device__ uint32_t tile_count_points(const Part &P, const Part &C, int s, uint64_t tbegin, uint32_t tcount,
+                                      const float *__restrict__ c0, const float *__restrict__ c1,
+                                      const float *__restrict__ c2) {
+    const int lane = lane_id();
+    const int ax = (int)P.axis, a2 = (int)C.axis;
+    const float *ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
+    const float *cb = a2 == 0 ? c0 : (a2 == 1 ? c1 : c2);
+    const bool all_left = P.status == kAllLeft;
+    uint32_t k = 0;
+    for (uint64_t i = tbegin + lane; i < tbegin + tcount; i += 64) {
+        const int si = (!all_left && (ca[i] >= P.split)) ? 1 : 0;
+        k += (si == s && !(cb[i] >= C.split)) ? 1u : 0u;
     }
     return wave_sum(k);
 }
@@ -395,6 +409,8 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
                         left = cnt[s];
                     } else if (C.counts_ready == kCountsBelow) {
                         left = below2[2 * (uint64_t)t + s];
+                    } else if (C.counts_ready == kCountsBins) {
+                        left = sum_tile_bins(tile_bins2 + ((uint64_t)t * 2 + s) * kMaxBins, C.kstar);
                     } else {
                         if (s == 0) slow0 = true; else slow1 = true;
                         continue;
@@ -408,7 +424,6 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
         while (pending) {   // wave-uniform loop over the lanes with a slow side
             const int src = __ffsll((unsigned long long)pending) - 1;
             pending &= pending - 1;
-            const uint32_t ts = __shfl(t, src, 64);
             const uint32_t tp = __shfl(T.part, src, 64), tc = __shfl(T.count, src, 64);
             const uint64_t tb = ((uint64_t)__shfl((uint32_t)(T.begin >> 32), src, 64) << 32) |
                                 __shfl((uint32_t)T.begin, src, 64);
@@ -417,11 +432,11 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
             const bool sl0 = __shfl(slow0 ? 1 : 0, src, 64) != 0, sl1 = __shfl(slow1 ? 1 : 0, src, 64) != 0;
             const Part &P = parts[tp];
             if (sl0) {
-                const uint32_t left = tile_count_slow<D>(P, children[P.child[0]], 0, ts, tb, tc, c0, c1, c2, tile_bins2);
+                const uint32_t left = tile_count_points(P, children[P.child[0]], 0, tb, tc, c0, c1, c2);
                 if (lane == src) { out[0] = left; out[1] = c0s - left; }
             }
             if (sl1) {
-                const uint32_t left = tile_count_slow<D>(P, children[P.child[1]], 1, ts, tb, tc, c0, c1, c2, tile_bins2);
+                const uint32_t left = tile_count_points(P, children[P.child[1]], 1, tb, tc, c0, c1, c2);
                 if (lane == src) { out[2] = left; out[3] = (tc - c0s) - left; }
             }
         }
