@@ -94,6 +94,7 @@ Context *default_context() {
         c->search_passes = coupe_hip::search_passes_from_env();
         c->pairs = coupe_hip::pairs_from_env();
         c->forward = coupe_hip::forward_from_env();
+        c->weights16 = coupe_hip::weights16_from_env();
         g_default[dev] = c;
     }
     return g_default[dev];
@@ -168,6 +169,7 @@ extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
     c->search_passes = coupe_hip::search_passes_from_env();
     c->pairs = coupe_hip::pairs_from_env();
     c->forward = coupe_hip::forward_from_env();
+    c->weights16 = coupe_hip::weights16_from_env();
     return reinterpret_cast<coupe_hip_ctx *>(c);
 }
 

@@ -106,6 +106,8 @@ struct Prologue {
     uint64_t max_abs_bits;      // bits of max |w| (non-negative doubles order as integers)
     uint32_t non_integral;      // any weight with a fractional part or |w| > 2^53
     uint32_t not_i32;           // any weight outside int32
+    uint32_t not_u16;           // any weight that is not an integer in [0, 65535]
+    uint32_t pad;
 };
 
 __host__ __device__ inline uint32_t f32_key(float x) {

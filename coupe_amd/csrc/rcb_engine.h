@@ -51,6 +51,7 @@ struct Context {
     std::mutex lock;                 // one call at a time per context
     DevBuf cx[3][3];                 // SoA f32 coordinates: input order, then two sides
     DevBuf w[2];                     // stored weights, two sides
+    DevBuf w16;                      // the prologue's 16-bit copy of the weights
     DevBuf parts[3], tiles[3];       // part / tile tables (two levels per move uses three)
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
     DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
@@ -67,6 +68,7 @@ struct Context {
     int search_passes = 3;           // histogram passes queued per level without a host check
     bool pairs = true;               // move the points once per two levels
     bool forward = true;             // the move forwards the next children's first pass
+    bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
@@ -106,6 +108,13 @@ inline bool pairs_from_env() {
 // of the level it fills (every such pass reads the points).
 inline bool forward_from_env() {
     const char *v = getenv("COUPE_HIP_FORWARD");
+    return !(v && v[0] == '0');
+}
+
+// COUPE_HIP_WEIGHTS16=0: integer weights are stored in 32 bits even when
+// they all fit in 16.
+inline bool weights16_from_env() {
+    const char *v = getenv("COUPE_HIP_WEIGHTS16");
     return !(v && v[0] == '0');
 }
 

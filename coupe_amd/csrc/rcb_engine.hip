@@ -731,8 +731,14 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
     });
     const uint32_t nt = (uint32_t)B.tiles0;
     if (nt > 0) {
-        // reads AoS f64 points and the input weights; writes SoA f32 and int32 weights
-        const double bytes = (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0);
+        // reads AoS f64 points and the input weights; writes SoA f32, int32
+        // weights and (optionally) their 16-bit copy
+        uint16_t *w16 = nullptr;
+        if (cx.weights16) {
+            cx.w16.ensure(std::max<uint64_t>(B.n, 1) * sizeof(uint16_t));
+            w16 = (uint16_t *)cx.w16.ptr;
+        }
+        const double bytes = (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0 + (w16 ? 2.0 : 0.0));
         Rotation rot;
         memset(&rot, 0, sizeof rot);
         if (cx.rot) rot = *cx.rot;
@@ -741,7 +747,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
             timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
                 kernel<<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr,
                                              (float *)cx.cx[0][1].ptr, (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
-                                             (int32_t *)cx.w[0].ptr, (Instance *)cx.instances.ptr,
+                                             (int32_t *)cx.w[0].ptr, w16, (Instance *)cx.instances.ptr,
                                              (Prologue *)cx.prologue.ptr, rot);
             });
         };
@@ -775,8 +781,15 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     bool integral = wkind != COUPE_DOUBLE ||
                     (!h.non_integral && h.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
     bool fits32 = !h.not_i32;
+    const bool fits16 = cx.weights16 && !h.not_u16;
     unsigned grid = std::min<unsigned>(grid_for(n, kBlock), 4096);
-    if (integral && fits32) {
+    if (integral && fits16) {
+        // every weight an integer in [0, 65535]: 2 bytes per stored weight
+        cx.stats.weight_mode = 0;
+        cx.stats.weight_bytes = 2;
+        cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(uint16_t));
+        return run_levels<D, uint16_t, uint64_t>(cx, st, B, wrap32, d_out, (uint16_t *)cx.w16.ptr, spec_passes);
+    } else if (integral && fits32) {
         cx.stats.weight_mode = 0;
         cx.stats.weight_bytes = 4;
         cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
@@ -876,7 +889,7 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                           &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
+                           &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
                            &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                            &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
@@ -885,7 +898,7 @@ uint64_t Context::device_bytes() const {
 
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                     &cx[2][1], &cx[2][2], &w[0], &w[1], &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
+                     &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
                      &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
                      &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();

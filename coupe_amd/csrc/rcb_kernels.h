@@ -95,7 +95,10 @@ template <int D, typename WIN, bool ROT>
 __global__ void __launch_bounds__(kBlock)
 k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__restrict__ pts,
            const WIN *__restrict__ win, float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
-           int32_t *__restrict__ w32, Instance *__restrict__ insts, Prologue *out, Rotation rot) {
+           int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out,
+           Rotation rot) {
+    // w16 (optional): the weights again as 16-bit unsigned integers, used
+    // when every weight is an integer in [0, 65535] (not_u16 stays 0)
     __shared__ double sp[kProPoints * D];
     __shared__ double s_d[kBlock / 64 * 8];
     __shared__ uint64_t s_u[kBlock / 64];
@@ -105,7 +108,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
     for (int k = 0; k < D; k++) { mn[k] = 1.7976931348623157e308; mx[k] = -1.7976931348623157e308; }
     uint64_t su = 0;
     double sf = 0.0, sa = 0.0, ma = 0.0;
-    uint32_t nonint = 0, not32 = 0;
+    uint32_t nonint = 0, not32 = 0, not16 = 0;
     uint32_t cur = 0xffffffffu;
     for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const Tile T = tiles[t];
@@ -159,8 +162,12 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                             const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
                             not32 |= !fits;
                             w32[i] = fits ? (int32_t)w : 0;
+                            const bool f16 = w >= 0.0 && w <= 65535.0;
+                            not16 |= !f16;
+                            if (w16) w16[i] = f16 ? (uint16_t)w : (uint16_t)0;
                         } else {
                             not32 = 1;
+                            not16 = 1;
                         }
                     } else {
                         const int64_t wi = (int64_t)w;
@@ -172,6 +179,9 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                         const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
                         not32 |= !fits;
                         w32[i] = (int32_t)wi;
+                        const bool f16 = wi >= 0 && wi <= 65535;
+                        not16 |= !f16;
+                        if (w16) w16[i] = (uint16_t)wi;
                     }
                 }
             }
@@ -186,12 +196,14 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
         ma = m2 > ma ? m2 : ma;
         nonint |= __shfl_xor(nonint, o, 64);
         not32 |= __shfl_xor(not32, o, 64);
+        not16 |= __shfl_xor(not16, o, 64);
     }
     if (lane_id() == 0) {
         atomicAdd(&out->sum_abs, sa);
         atomicMax((unsigned long long *)&out->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, ma));
         if (nonint) atomicOr(&out->non_integral, 1u);
         if (not32) atomicOr(&out->not_i32, 1u);
+        if (not16) atomicOr(&out->not_u16, 1u);
     }
 }
 
@@ -1144,8 +1156,8 @@ k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, cons
         __syncthreads();
         COUPE_STORE_RUN(o0[d] = sf0[pos]; o1[d] = sf1[pos])
         __syncthreads();
-        // round 2: axis 2 and 4-byte weights together; 8-byte weights alone
-        if constexpr (sizeof(WS) == 4) {
+        // round 2: axis 2 and 2- or 4-byte weights together; 8-byte weights alone
+        if constexpr (sizeof(WS) <= 4) {
 #pragma unroll
             for (int k = 0; k < kItems; k++)
                 if (k * kBlock + tid < (int)cnt) {
@@ -1347,7 +1359,7 @@ k_partition_tiles(const Part *__restrict__ parts, const Part *__restrict__ next_
         __syncthreads();
         COUPE_STORE_RUN(o0[d] = sf0[pos]; o1[d] = sf1[pos])
         __syncthreads();
-        if constexpr (sizeof(WS) == 4) {
+        if constexpr (sizeof(WS) <= 4) {
 #pragma unroll
             for (int k = 0; k < kItems; k++)
                 if (k * kBlock + tid < (int)cnt) {

@@ -197,6 +197,7 @@ extern "C" coupe_hip_server *coupe_hip_server_create(const struct coupe_hip_serv
         W->cx.search_passes = coupe_hip::search_passes_from_env();
         W->cx.pairs = coupe_hip::pairs_from_env();
         W->cx.forward = coupe_hip::forward_from_env();
+        W->cx.weights16 = coupe_hip::weights16_from_env();
         S->workers.push_back(W);
     }
     for (Worker *W : S->workers) W->th = std::thread(worker_main, S, W);

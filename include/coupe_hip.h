@@ -205,7 +205,7 @@ struct coupe_hip_stats {
 	uint32_t passes;          /* histogram passes (all levels) */
 	uint32_t max_passes;      /* largest pass count of one level */
 	uint32_t weight_mode;     /* 0 int64 exact sums, 1 f64 sums */
-	uint32_t weight_bytes;    /* bytes per stored weight (4 or 8) */
+	uint32_t weight_bytes;    /* bytes per stored weight (2, 4 or 8) */
 	uint32_t bins;            /* histogram bins per part per pass */
 	uint32_t reruns;          /* 1 when a split search outran the queued passes and the call ran again */
 	uint64_t unit_points;     /* points per work unit */

@@ -238,6 +238,44 @@ def test_move_forwards_the_next_first_pass(oracle_mod, monkeypatch, forward):
         ctx.close()
 
 
+@pytest.mark.parametrize("w16", ["0", "1"])
+def test_stored_weight_widths(oracle_mod, monkeypatch, w16):
+    """Integer weights in [0, 65535] are stored in 16 bits (COUPE_HIP_WEIGHTS16,
+    default on), other integers in 32 or 64, non-integers as f64; ids equal
+    either way.  Cases at and past the 16-bit edges: 65535, 65536, 0, -1."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_WEIGHTS16", w16)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(1500)
+        widths = []
+        for q in range(8):
+            n = int(rng.integers(5_000, 200_000))
+            dim = 2 + q % 2
+            pts = rng.random((n, dim))
+            hi = [65535, 65536, 1000, 40000, 1 << 40, 3][q % 6]
+            lo = [0, 0, 1, -1, 1, 0][q % 6]
+            dt = [np.int32, np.int64, np.float64][q % 3]
+            if dt == np.int32:
+                hi = min(hi, (1 << 31) - 1)
+            w = rng.integers(lo, hi, n, endpoint=True).astype(dt)
+            if q % 6 == 0:
+                w[rng.integers(0, n, 10)] = 65535   # the top of the 16-bit range
+            iters, tol = int(rng.integers(3, 12)), float(rng.choice([0.0, 0.01, 0.05]))
+            say(f"weights16 {w16}, case {q}: n={n} d={dim} w in [{w.min()}, {w.max()}] {w.dtype} L={iters}")
+            out = torch.zeros(n, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            widths.append(ctx.stats()["weight_bytes"])
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"weights16 {w16} case {q}"
+        assert (2 in widths) == (w16 == "1")
+        assert any(x > 2 for x in widths)
+    finally:
+        ctx.close()
+
+
 def test_edge_cases(oracle_mod):
     rng = np.random.default_rng(7)
     one = rng.random((1, 3))
