@@ -69,6 +69,7 @@ struct Context {
     bool pairs = true;               // move the points once per two levels
     bool forward = true;             // the move forwards the next children's first pass
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
+    int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
@@ -83,6 +84,7 @@ struct Context {
     bool bins_clean = false;
     uint32_t epoch = 0;              // look-back epoch of the last partition launch
     int cu_count = 0;                // compute units of the device
+    std::vector<std::pair<const void *, int>> occupancy;   // resident workgroups per CU, per kernel
     double swept_bytes = 0.0;        // HIST algorithmic bytes of the last call
     coupe_hip_stats stats{};
     Profiler prof;

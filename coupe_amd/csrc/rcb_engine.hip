@@ -151,9 +151,17 @@ static int choose_bins(uint64_t parts_ub) {
 // Resident workgroups of a persistent kernel: CUs x blocks per CU.
 template <typename K>
 static unsigned persistent_grid(Context &cx, K kernel, uint64_t work_items) {
+    // the occupancy query is a runtime call: asked once per kernel and
+    // context, so that queuing a launch stays cheaper than running it
+    const void *key = reinterpret_cast<const void *>(kernel);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
-        per_cu = 2;
+    for (const auto &e : cx.occupancy)
+        if (e.first == key) { per_cu = e.second; break; }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1)
+            per_cu = 2;
+        cx.occupancy.emplace_back(key, per_cu);
+    }
     uint64_t g = (uint64_t)cx.cu_count * (uint64_t)per_cu;
     if (g > work_items) g = work_items;
     return (unsigned)(g < 1 ? 1 : g);
@@ -713,7 +721,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
 template <int D, typename WIN>
 static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const double *pts, const WIN *win,
-                     Prologue *h) {
+                     Prologue *h, bool wait) {
     const uint32_t n_inst = (uint32_t)B.inst.size();
     check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
     // bounding boxes start empty: min key = max, max key = 0
@@ -754,8 +762,9 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
         if (cx.rot) launch(k_prologue<D, WIN, true>);
         else launch(k_prologue<D, WIN, false>);
     }
+    // without `wait`, h is pinned memory read once the call's last sync is done
     check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
-    check(hipStreamSynchronize(st));
+    if (wait) check(hipStreamSynchronize(st));
 }
 
 template <int D, typename WIN>
@@ -771,47 +780,66 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     cx.instances.ensure(std::max<size_t>(B.inst.size(), 1) * sizeof(Instance));
     cx.parts[0].ensure(std::max<uint64_t>(B.parts_cap, B.inst.size()) * sizeof(Part));
     cx.tiles0.ensure(std::max<uint64_t>(B.tiles0, 1) * sizeof(Tile));
-    Prologue h;
-    prologue<D, WIN>(cx, st, B, pts, win, &h);
 
     // Weight mode.  Integer kinds and integer-valued doubles whose partial
     // sums stay exact in f64 (|sum| <= 2^53) use exact int64 sums: the same
-    // bits as coupe's sums in any order.  Other doubles use f64 sums.
+    // bits as coupe's sums in any order; they are stored in 16 bits when all
+    // are in [0, 65535], else in 32 or 64.  Other doubles use f64 sums.
+    // Modes: 0 u16, 1 i32, 2 i64, 3 f64.
     const bool wrap32 = wkind == COUPE_INT;
-    bool integral = wkind != COUPE_DOUBLE ||
-                    (!h.non_integral && h.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
-    bool fits32 = !h.not_i32;
-    const bool fits16 = cx.weights16 && !h.not_u16;
-    unsigned grid = std::min<unsigned>(grid_for(n, kBlock), 4096);
-    if (integral && fits16) {
-        // every weight an integer in [0, 65535]: 2 bytes per stored weight
-        cx.stats.weight_mode = 0;
-        cx.stats.weight_bytes = 2;
-        cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(uint16_t));
-        return run_levels<D, uint16_t, uint64_t>(cx, st, B, wrap32, d_out, (uint16_t *)cx.w16.ptr, spec_passes);
-    } else if (integral && fits32) {
-        cx.stats.weight_mode = 0;
-        cx.stats.weight_bytes = 4;
-        cx.w[1].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
-        return run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr, spec_passes);
-    } else if (integral) {
-        cx.stats.weight_mode = 0;
-        cx.stats.weight_bytes = 8;
-        cx.w[0].ensure(n * sizeof(int64_t));
-        cx.w[1].ensure(n * sizeof(int64_t));
-        timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-            k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
-        });
-        return run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr, spec_passes);
-    } else {
-        cx.stats.weight_mode = 1;
-        cx.stats.weight_bytes = 8;
-        cx.w[0].ensure(n * sizeof(double));
-        cx.w[1].ensure(n * sizeof(double));
+    auto best_mode = [&](const Prologue &p) {
+        const bool integral = wkind != COUPE_DOUBLE ||
+                              (!p.non_integral && p.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
+        if (!integral) return 3;
+        if (cx.weights16 && !p.not_u16) return 0;
+        return p.not_i32 ? 2 : 1;
+    };
+    // an integer mode holds any narrower one; f64 and the integer modes differ
+    auto holds = [](int mode, int best) { return (mode == 3) == (best == 3) && mode >= best; };
+    auto weight_bytes = [](int mode) { return mode == 0 ? 2u : (mode == 1 ? 4u : 8u); };
+    const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), 4096);
+    auto run_mode = [&](int mode) {
+        cx.stats.weight_mode = mode == 3 ? 1 : 0;
+        cx.stats.weight_bytes = weight_bytes(mode);
+        cx.w[1].ensure(std::max<uint64_t>(n, 1) * weight_bytes(mode));
+        if (mode == 0)
+            return run_levels<D, uint16_t, uint64_t>(cx, st, B, wrap32, d_out, (uint16_t *)cx.w16.ptr, spec_passes);
+        if (mode == 1)
+            return run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr, spec_passes);
+        if (mode == 2) {
+            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+                k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
+            });
+            return run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr, spec_passes);
+        }
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
             k_store_weights<WIN, double><<<grid, kBlock, 0, st>>>(win, (double *)cx.w[0].ptr, n);
         });
         return run_levels<D, double, double>(cx, st, B, 0, d_out, (double *)cx.w[0].ptr, spec_passes);
+    };
+    // The mode of this context's last call is taken as this call's: the
+    // prologue's flags are then read with the call's last counters, without a
+    // wait after the prologue, and a call whose data does not fit that mode
+    // runs again in its own.  A new context waits for the flags first.
+    static_assert(kHostScratch >= 8192 + sizeof(Prologue), "prologue readback");
+    Prologue *hp = (Prologue *)((char *)cx.host_scratch + 8192);
+    for (;;) {
+        const int hint = cx.weight_hint;
+        int mode = hint;
+        // buffers for the expected mode before anything is queued (a buffer
+        // that grows is freed and allocated again)
+        cx.w[0].ensure(std::max<uint64_t>(n, 1) * (hint >= 2 ? 8u : 4u));
+        if (hint >= 0) cx.w[1].ensure(std::max<uint64_t>(n, 1) * weight_bytes(hint));
+        prologue<D, WIN>(cx, st, B, pts, win, hp, hint < 0);
+        if (hint < 0) {
+            mode = best_mode(*hp);
+            if (mode >= 2) cx.w[0].ensure(std::max<uint64_t>(n, 1) * 8u);
+        }
+        const bool unfinished = run_mode(mode);
+        const int best = best_mode(*hp);   // complete: run_levels waited for the stream
+        cx.weight_hint = best;
+        if (!holds(mode, best)) continue;
+        return unfinished;
     }
 }
 

@@ -135,6 +135,54 @@ def test_consecutive_calls_on_one_context(oracle_mod):
         ctx.close()
 
 
+def test_weight_modes_change_between_calls(oracle_mod):
+    """A context takes its last call's weight mode for the next call and reads
+    the prologue's flags at the end; a call whose weights do not fit that
+    mode (16-bit after wide integers, integers after fractional doubles, ...)
+    runs again in its own mode.  Ids equal the oracle's on every call."""
+    torch = pytest.importorskip("torch")
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(1700)
+        kinds = [("u16", np.int64), ("wide", np.int64), ("u16", np.float64), ("frac", np.float64),
+                 ("u16", np.int32), ("neg", np.int32), ("wide", np.int64), ("u16", np.int64)]
+        for q, (kind, dt) in enumerate(kinds):
+            n = int(rng.integers(20_000, 120_000))
+            pts = rng.random((n, 3))
+            if kind == "u16":
+                w = rng.integers(0, 65536, n).astype(dt)
+            elif kind == "wide":
+                w = rng.integers(1, 1 << 33, n).astype(dt)
+            elif kind == "neg":
+                w = rng.integers(-50, 50, n).astype(dt)
+            else:
+                w = rng.random(n) * 10.0 + 0.5
+            iters, tol = 9, 0.02
+            own = {"u16": 2, "wide": 8, "neg": 4, "frac": 8}[kind]
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            # twice: the first call may keep a wider integer mode from the
+            # previous data (ids are the same); the second uses the data's own
+            for rep in range(2):
+                say(f"weight mode change, call {q}.{rep}: {kind} {np.dtype(dt).name} n={n}")
+                out = torch.zeros(n, dtype=torch.int64, device="cuda")
+                coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                    out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+                torch.cuda.synchronize()
+                got = out.cpu().numpy().astype(np.uint64)
+                if kind == "frac":
+                    a, b = part_weights(got, w), part_weights(exp, w)
+                    np.testing.assert_allclose(a, b, rtol=1e-12, atol=0, err_msg=f"call {q}.{rep}")
+                else:
+                    assert np.array_equal(got, exp), f"call {q}.{rep} ({kind})"
+                st = ctx.stats()
+                if rep == 0:
+                    assert st["weight_bytes"] >= own, (q, kind, st)
+                else:
+                    assert st["weight_bytes"] == own, (q, kind, st)
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("queued", [0, 1, 3])
 def test_split_search_queue_depths(oracle_mod, monkeypatch, queued):
     """Split searches queued without host checks (COUPE_HIP_SEARCH_PASSES):
