@@ -377,25 +377,25 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
-                                                               &slot[0], swept_first, counts_first, nullptr);
+                                                               &slot[0], swept_first, counts_first, nullptr,
+                                                               nullptr);
             });
             for (int p = 1; p <= spec_passes; p++) {
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
+                // the last queued decision also closes the searches it leaves open
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                    k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                                   &slot[p], &ctr->swept, counts_hist, &slot[p - 1]);
+                    k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                        parts, np, bcnt, bw, bmin, nb, wrap32, &slot[p], &ctr->swept, counts_hist, &slot[p - 1],
+                        p == spec_passes ? &ctr->unfinished : nullptr);
                 });
             }
-            timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_force_finish<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(parts, np, &slot[spec_passes],
-                                                                            &ctr->unfinished);
-            });
             return;
         }
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
             k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
-                                                           &ctr->searching, swept_first, counts_first, nullptr);
+                                                           &ctr->searching, swept_first, counts_first, nullptr,
+                                                           nullptr);
         });
         uint32_t passes = 1;
         for (;;) {
@@ -408,7 +408,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, nullptr); });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept, counts_hist, nullptr);
+                                                               &ctr->searching, &ctr->swept, counts_hist, nullptr,
+                                                               nullptr);
             });
             passes++;
         }

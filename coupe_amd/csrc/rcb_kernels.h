@@ -402,7 +402,11 @@ __global__ void __launch_bounds__(64)
 k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
          Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
          uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
-         const uint32_t *gate) {
+         const uint32_t *gate, uint32_t *unfinished) {
+    // unfinished (non-null): the last queued decision of a speculative
+    // search; a part still searching after it is closed as "all left" (a
+    // consistent tree for the kernels that follow) and the call is flagged,
+    // so the host runs it again with a search that checks every pass
     if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
@@ -513,26 +517,16 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     }
     P->blo = lo; P->bhi = hi; P->prev = prev;             // next pass continues here
     atomicAdd(n_searching, 1u);
+    if (unfinished) {
+        P->status = kAllLeft;
+        P->split = hi;
+        P->cnt_left = n;
+        P->wl = P->sum;
+        P->kstar = -1;
+        atomicOr(unfinished, 1u);
+    }
 }
 
-// End of a speculative split search: a part still searching after the last
-// queued pass is closed as "all left" (a consistent tree for the kernels that
-// follow) and the call is flagged, so the host runs it again with a search
-// that checks every pass.
-__global__ void __launch_bounds__(kBlock)
-k_force_finish(Part *__restrict__ parts, const uint32_t *n_parts, const uint32_t *gate, uint32_t *unfinished) {
-    if (*gate == 0) return;
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= *n_parts) return;
-    Part *P = &parts[s];
-    if (P->status != kSearching) return;
-    P->status = kAllLeft;
-    P->split = P->bhi;
-    P->cnt_left = P->count;
-    P->wl = P->sum;
-    P->kstar = -1;
-    atomicOr(unfinished, 1u);
-}
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan of uint64 (3 kernels).  n is read on the device;
