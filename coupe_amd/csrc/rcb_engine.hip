@@ -722,7 +722,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
 template <int D, typename WIN>
 static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const double *pts, const WIN *win,
-                     Prologue *h, bool wait) {
+                     Prologue *h, bool wait, int hint) {
     const uint32_t n_inst = (uint32_t)B.inst.size();
     check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
     // bounding boxes start empty: min key = max, max key = 0
@@ -742,12 +742,16 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
     if (nt > 0) {
         // reads AoS f64 points and the input weights; writes SoA f32, int32
         // weights and (optionally) their 16-bit copy
+        // the weight copies the expected mode reads (both without a hint;
+        // none for the 64-bit modes, which store their own)
+        int32_t *w32 = hint < 0 || hint == 1 ? (int32_t *)cx.w[0].ptr : nullptr;
         uint16_t *w16 = nullptr;
-        if (cx.weights16) {
+        if (cx.weights16 && hint <= 0) {
             cx.w16.ensure(std::max<uint64_t>(B.n, 1) * sizeof(uint16_t));
             w16 = (uint16_t *)cx.w16.ptr;
         }
-        const double bytes = (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + 4.0 + (w16 ? 2.0 : 0.0));
+        const double bytes =
+            (double)B.n * (8.0 * D + sizeof(WIN) + 4.0 * D + (w32 ? 4.0 : 0.0) + (w16 ? 2.0 : 0.0));
         Rotation rot;
         memset(&rot, 0, sizeof rot);
         if (cx.rot) rot = *cx.rot;
@@ -756,7 +760,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
             timed(cx, st, COUPE_HIP_K_PROLOGUE, bytes, [&] {
                 kernel<<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr,
                                              (float *)cx.cx[0][1].ptr, (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
-                                             (int32_t *)cx.w[0].ptr, w16, (Instance *)cx.instances.ptr,
+                                             w32, w16, (Instance *)cx.instances.ptr,
                                              (Prologue *)cx.prologue.ptr, rot);
             });
         };
@@ -831,7 +835,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         // that grows is freed and allocated again)
         cx.w[0].ensure(std::max<uint64_t>(n, 1) * (hint >= 2 ? 8u : 4u));
         if (hint >= 0) cx.w[1].ensure(std::max<uint64_t>(n, 1) * weight_bytes(hint));
-        prologue<D, WIN>(cx, st, B, pts, win, hp, hint < 0);
+        prologue<D, WIN>(cx, st, B, pts, win, hp, hint < 0, hint);
         if (hint < 0) {
             mode = best_mode(*hp);
             if (mode >= 2) cx.w[0].ensure(std::max<uint64_t>(n, 1) * 8u);

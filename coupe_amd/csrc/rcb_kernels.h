@@ -97,8 +97,9 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
            const WIN *__restrict__ win, float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
            int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out,
            Rotation rot) {
-    // w16 (optional): the weights again as 16-bit unsigned integers, used
-    // when every weight is an integer in [0, 65535] (not_u16 stays 0)
+    // w32, w16 (each optional): the weights as int32 and as 16-bit unsigned
+    // integers; the latter are used when every weight is an integer in
+    // [0, 65535] (not_u16 stays 0)
     __shared__ double sp[kProPoints * D];
     __shared__ double s_d[kBlock / 64 * 8];
     __shared__ uint64_t s_u[kBlock / 64];
@@ -161,7 +162,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                             su += (uint64_t)(int64_t)w;
                             const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
                             not32 |= !fits;
-                            w32[i] = fits ? (int32_t)w : 0;
+                            if (w32) w32[i] = fits ? (int32_t)w : 0;
                             const bool f16 = w >= 0.0 && w <= 65535.0;
                             not16 |= !f16;
                             if (w16) w16[i] = f16 ? (uint16_t)w : (uint16_t)0;
@@ -178,7 +179,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                         ma = aw > ma ? aw : ma;
                         const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
                         not32 |= !fits;
-                        w32[i] = (int32_t)wi;
+                        if (w32) w32[i] = (int32_t)wi;
                         const bool f16 = wi >= 0 && wi <= 65535;
                         not16 |= !f16;
                         if (w16) w16[i] = (uint16_t)wi;
