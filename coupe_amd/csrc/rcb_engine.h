@@ -70,6 +70,7 @@ struct Context {
     bool forward = true;             // the move forwards the next children's first pass
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
+    int passes_hint = 0;             // most decisions one level's search needed in the last call
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
@@ -121,7 +122,8 @@ inline bool weights16_from_env() {
 }
 
 // COUPE_HIP_SEARCH_PASSES=k: histogram passes queued per level before the
-// host looks (default 3; 0 = the host checks after every pass).
+// host looks (default 3; 0 = the host checks after every pass).  A context
+// queues fewer, never under 2, when its last call needed fewer.
 inline int search_passes_from_env() {
     const char *v = getenv("COUPE_HIP_SEARCH_PASSES");
     if (!v || !*v) return 3;

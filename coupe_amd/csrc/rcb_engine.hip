@@ -888,7 +888,13 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
             return dim == 2 ? run_dim<2>(cx, st, d_out, B, d_points, wkind, d_weights, spec)
                             : run_dim<3>(cx, st, d_out, B, d_points, wkind, d_weights, spec);
         };
-        if (run(cx.search_passes)) {
+        // Passes queued per level: the configured depth, or fewer (never
+        // under 2) when this context's last call needed fewer (its deepest
+        // search's decisions); each queued pass costs two launches even when
+        // its gate is closed.
+        int spec = cx.search_passes;
+        if (cx.passes_hint > 0 && spec > 2) spec = std::max(2, std::min(spec, cx.passes_hint));
+        if (run(spec)) {
             // a split search needed more passes than were queued: again,
             // checking every pass on the host
             cx.stats = coupe_hip_stats{};
@@ -896,6 +902,7 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
             cx.stats.reruns = 1;
             run(0);
         }
+        cx.passes_hint = (int)cx.stats.max_passes;
         cx.prof.collect();
         cx.prof.out[COUPE_HIP_K_HIST].algo_bytes += cx.swept_bytes;   // first passes + standalone passes
     } catch (const HipFailure &f) {
