@@ -591,12 +591,14 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const uint32_t p = (uint32_t)(k * kBlock + tid);
-                const uint32_t c = p < cnt ? p : cnt - 1;
-                rx[k] = bx[c];
-                ry[k] = by[c];
-                if constexpr (D == 3) rz[k] = bz[c];
+                // min() keeps c <= p < kTile visible to the compiler: the
+                // offsets are 32-bit, the loads take the base from SGPRs
+                const uint32_t c = min(p, cnt - 1u);
+                rx[k] = __builtin_nontemporal_load(&bx[c]);
+                ry[k] = __builtin_nontemporal_load(&by[c]);
+                if constexpr (D == 3) rz[k] = __builtin_nontemporal_load(&bz[c]);
                 else rz[k] = 0.f;
-                wv[k] = bwt[c];
+                wv[k] = __builtin_nontemporal_load(&bwt[c]);
             }
         }
         uint32_t pq[kItems];   // rank among the wavefront's points of the same slot | slot << 8
@@ -622,6 +624,9 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mine >> 32),
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mine, 0u));
             pq[k] = rank | ((valid ? q : 4u) << 8);
+            // computed here, not after the barrier: otherwise the compiler
+            // keeps the four slot masks of every item in SGPRs until then
+            asm volatile("" : "+v"(pq[k]));
         }
         if (tid < 4) {
             const uint64_t off = pv - s_gpj0[tid];   // this tile's first position in the grandchild
@@ -698,10 +703,10 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
                     if (v[u]) {
                         const uint64_t d = db + i;
-                        if constexpr (D == 3) oa2[d] = xc[u];
-                        oa3[d] = xa[u];
-                        oa4[d] = xb[u];
-                        ow[d] = wq[u];
+                        if constexpr (D == 3) __builtin_nontemporal_store(xc[u], &oa2[d]);
+                        __builtin_nontemporal_store(xa[u], &oa3[d]);
+                        __builtin_nontemporal_store(xb[u], &oa4[d]);
+                        __builtin_nontemporal_store(wq[u], &ow[d]);
                     }
                     const bool bel3 = v[u] && !(xa[u] >= t3w);
                     const uint64_t bb = __ballot(bel3);
