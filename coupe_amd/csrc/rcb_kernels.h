@@ -121,7 +121,19 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
             const uint64_t base = T.begin + r0;
             const uint32_t cnt = T.count - r0 < (uint32_t)kProPoints ? T.count - r0 : (uint32_t)kProPoints;
             const double *src = pts + base * D;
-            for (uint32_t j = tid; j < cnt * D; j += kBlock) sp[j] = src[j];
+            // every load of the round issued back to back: the AoS doubles
+            // (clamped to the round's last one) and the points' weights
+            constexpr int kDv = kProPoints * D / kBlock;
+            const uint32_t nd = cnt * D;
+            double dv[kDv];
+            WIN wv[4];
+#pragma unroll
+            for (int j = 0; j < kDv; j++) dv[j] = __builtin_nontemporal_load(&src[min((uint32_t)(j * kBlock + tid), nd - 1u)]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) wv[k] = __builtin_nontemporal_load(&win[base + min((uint32_t)(k * kBlock + tid), cnt - 1u)]);
+#pragma unroll
+            for (int j = 0; j < kDv; j++)
+                if ((uint32_t)(j * kBlock + tid) < nd) sp[j * kBlock + tid] = dv[j];
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -150,7 +162,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                         float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
                         cs[i] = (float)v;
                     }
-                    const WIN w = win[i];
+                    const WIN w = wv[k];
                     if constexpr (std::is_same<WIN, double>::value) {
                         const double aw = fabs(w);
                         const bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
@@ -1542,8 +1554,19 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
             x2[k] = D == 3 ? c2[c] : 0.f;
             h[k] = 0;
         }
+        // D levels at a time from the root: their axes are 0..D-1, so each
+        // step compares a coordinate register directly
+        uint32_t l = 0;
+        for (; l + D <= cached_levels; l += D) {
+#pragma unroll
+            for (int k = 0; k < kItems; k++) {
+                h[k] = 2 * h[k] + (x0[k] >= cache[h[k]] ? 2u : 1u);
+                h[k] = 2 * h[k] + (x1[k] >= cache[h[k]] ? 2u : 1u);
+                if constexpr (D == 3) h[k] = 2 * h[k] + (x2[k] >= cache[h[k]] ? 2u : 1u);
+            }
+        }
         int a = 0;
-        for (uint32_t l = 0; l < cached_levels; l++) {
+        for (; l < cached_levels; l++) {
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const float t = cache[h[k]];
@@ -1551,7 +1574,7 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
             }
             a = a + 1 == D ? 0 : a + 1;
         }
-        for (uint32_t l = cached_levels; l < levels; l++) {
+        for (; l < levels; l++) {
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const float t = heap_split[hb + h[k]];

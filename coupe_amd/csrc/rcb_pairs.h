@@ -483,11 +483,12 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: per-wave state in SGPRs
     const uint32_t nt = *n_tiles;
     uint32_t cur = 0xffffffffu;
-    float gsplit[2] = {0.f, 0.f};
-    bool gall[2] = {true, true};
+    // the thresholds of the parent and of its two children; NaN where every
+    // point goes left (kAllLeft, or no child), as !(x >= NaN) holds for all x
+    const float kNone = __builtin_nanf("");
+    float gsplit[2] = {kNone, kNone};
     int ax = 0, a2 = 0, a3 = 0, a4 = 0;
-    float split = 0.f;
-    bool all_left = false;
+    float split = kNone;
     // lanes 0..3 serve grandchild slot `lane`: its scan array
     const int js = lane & 3;
     const uint64_t *pj = js == 0 ? p0 : (js == 1 ? p1 : (js == 2 ? p2 : p3));
@@ -543,18 +544,15 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             a2 = (int)((P.axis + 1) % D);
             a3 = (int)((P.axis + 2) % D);
             a4 = (int)((P.axis + 3) % D);
-            split = P.split;
-            all_left = P.status == kAllLeft;
+            split = P.status == kAllLeft ? kNone : P.split;
             int32_t g[4];
             for (int s = 0; s < 2; s++) {
                 const int32_t ci = P.child[s];
-                gsplit[s] = 0.f;
-                gall[s] = true;
+                gsplit[s] = kNone;
                 g[2 * s] = g[2 * s + 1] = -1;
                 if (ci >= 0) {
                     const Part &C = children[ci];
-                    gsplit[s] = C.split;
-                    gall[s] = C.status == kAllLeft;
+                    gsplit[s] = C.status == kAllLeft ? kNone : C.split;
                     g[2 * s] = C.child[0];
                     g[2 * s + 1] = C.child[1];
                 }
@@ -605,9 +603,8 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const bool valid = (uint32_t)(k * kBlock + tid) < cnt;
-            const bool s1 = !all_left && (rx[k] >= split);
-            const float xb = ry[k];
-            const bool s2 = s1 ? (!gall[1] && (xb >= gsplit[1])) : (!gall[0] && (xb >= gsplit[0]));
+            const bool s1 = rx[k] >= split;
+            const bool s2 = ry[k] >= (s1 ? gsplit[1] : gsplit[0]);
             const uint32_t q = (s1 ? 2u : 0u) + (s2 ? 1u : 0u);
             // slot masks from three ballots (wave-uniform)
             const uint64_t bv = __ballot(valid), b1 = __ballot(s1), b2 = __ballot(s2);
@@ -677,22 +674,32 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
             float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
             const uint32_t rb = s_run[wid];
-            const uint64_t db = s_dst[wid];
+            const uint64_t dv = s_dst[wid];   // wave-uniform: into SGPRs once, for the stores' bases
+            const uint64_t db = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(dv >> 32)) << 32) |
+                                __builtin_amdgcn_readfirstlane((uint32_t)dv);
             uint32_t len = s_run[wid + 1] - rb;
             len = db + len <= n_total ? len : (db < n_total ? (uint32_t)(n_total - db) : 0u);   // never past the arrays
             const uint32_t split_i = kTile - s_off[wid];   // positions from here on are in the next tile
-            uint32_t c_left = 0, c_b0 = 0, c_b1 = 0, h_left = 0, h_b0 = 0, h_b1 = 0;
+            uint32_t c_left = 0, c_b0 = 0, c_b1 = 0;
+            // FWD: the same counts over the positions in the first destination
+            // tile (positions below split_i), taken where the run crosses it
+            uint32_t f_left = 0, f_b0 = 0, f_b1 = 0;
+            bool crossed = false;
+            // the run's destination as uniform bases with 32-bit offsets (a
+            // run lies in one tile: i < kTile + 128, so the mask below changes
+            // nothing and shows the compiler the offsets' range)
+            float *pa2 = oa2 + db, *pa3 = oa3 + db, *pa4 = oa4 + db;
+            WS *pw = ow + db;
             // two positions per lane per iteration, every lane runs every
-            // iteration (validity in the masks)
+            // iteration: validity is in uniform lane masks, and the ballots
+            // take the comparisons alone
             for (uint32_t base = 0; base < len; base += 128) {
                 float xa[2], xb[2], xc[2];
                 WS wq[2];
-                bool v[2];
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
                     const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
-                    v[u] = i < len;
-                    const uint32_t c = rb + (v[u] ? i : 0u);
+                    const uint32_t c = rb + (i < len ? i : 0u);
                     xa[u] = sfa[c];
                     xb[u] = sfb[c];
                     if constexpr (D == 3) xc[u] = sfc[c];
@@ -700,37 +707,46 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 }
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
-                    const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
-                    if (v[u]) {
-                        const uint64_t d = db + i;
-                        if constexpr (D == 3) __builtin_nontemporal_store(xc[u], &oa2[d]);
-                        __builtin_nontemporal_store(xa[u], &oa3[d]);
-                        __builtin_nontemporal_store(xb[u], &oa4[d]);
-                        __builtin_nontemporal_store(wq[u], &ow[d]);
+                    const uint32_t pos = base + (uint32_t)u * 64;   // wave-uniform
+                    const bool vl = pos + (uint32_t)lane < len;
+                    const uint64_t vm = __builtin_amdgcn_ballot_w64(vl);   // lanes holding a position of the run
+                    const uint32_t i = (pos + (uint32_t)lane) & (4 * kTile - 1);
+                    if (vl) {
+                        if constexpr (D == 3) __builtin_nontemporal_store(xc[u], &pa2[i]);
+                        __builtin_nontemporal_store(xa[u], &pa3[i]);
+                        __builtin_nontemporal_store(xb[u], &pa4[i]);
+                        __builtin_nontemporal_store(wq[u], &pw[i]);
                     }
-                    const bool bel3 = v[u] && !(xa[u] >= t3w);
-                    const uint64_t bb = __ballot(bel3);
+                    const bool lt3 = !(xa[u] >= t3w);
+                    const uint64_t bb = __builtin_amdgcn_ballot_w64(lt3) & vm;
+                    const bool bel3 = vl && lt3;
                     const Acc wa = AccOps<Acc>::of(wq[u]);
                     // the grandchild's first pass
                     c_left += __popcll(bb);
                     gws += bel3 ? wa : (Acc)0;
-                    const uint32_t ka = (v[u] && !bel3) ? f32_key(xa[u]) : 0xffffffffu;
+                    const uint32_t ka = (vl && !lt3) ? f32_key(xa[u]) : 0xffffffffu;
                     gmn = ka < gmn ? ka : gmn;
                     if constexpr (FWD) {
-                        const bool below4 = v[u] && !(xb[u] >= t4w);
-                        const uint64_t b4 = __ballot(below4);
-                        const uint64_t hm = __ballot(v[u] && i >= split_i);   // in the next destination tile
+                        const bool lt4 = !(xb[u] >= t4w);
+                        const uint64_t b4 = __builtin_amdgcn_ballot_w64(lt4) & vm;
+                        if (!crossed && split_i < pos + 64) {
+                            // the chunk that holds the destination tile's end (once
+                            // per run, uniform): the counts of the positions before it
+                            const uint64_t lo = (1ull << (split_i - pos)) - 1ull;   // split_i - pos in [0, 63]
+                            f_left = c_left - __popcll(bb) + __popcll(bb & lo);
+                            f_b0 = c_b0 + __popcll(bb & b4 & lo);
+                            f_b1 = c_b1 + __popcll(~bb & b4 & lo);
+                            crossed = true;
+                        }
                         c_b0 += __popcll(bb & b4);
                         c_b1 += __popcll(~bb & b4);
-                        h_left += __popcll(bb & hm);
-                        h_b0 += __popcll(bb & b4 & hm);
-                        h_b1 += __popcll(~bb & b4 & hm);
                         const uint32_t key = f32_key(xb[u]);
-                        fws0 += (bel3 && below4) ? wa : (Acc)0;
-                        fws1 += (!bel3 && below4) ? wa : (Acc)0;
-                        const bool side1 = v[u] && !bel3;
-                        const uint32_t k0 = (bel3 && !below4) ? key : 0xffffffffu;
-                        const uint32_t k1 = (side1 && !below4) ? key : 0xffffffffu;
+                        const bool below4 = vl && lt4;
+                        fws0 += (below4 && lt3) ? wa : (Acc)0;
+                        fws1 += (below4 && !lt3) ? wa : (Acc)0;
+                        const bool above4 = vl && !lt4;
+                        const uint32_t k0 = (above4 && lt3) ? key : 0xffffffffu;
+                        const uint32_t k1 = (above4 && !lt3) ? key : 0xffffffffu;
                         fmn0 = k0 < fmn0 ? k0 : fmn0;
                         fmn1 = k1 < fmn1 ? k1 : fmn1;
                     }
@@ -740,9 +756,11 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             if (FWD && len > 0) {
                 fcn0 += c_b0;
                 fcn1 += c_b1;
+                if (!crossed) { f_left = c_left; f_b0 = c_b0; f_b1 = c_b1; }
+                const uint32_t h_left = c_left - f_left, h_b0 = c_b0 - f_b0, h_b1 = c_b1 - f_b1;
                 if (lane == 0) {
                     const uint64_t dt = s_tile[wid];
-                    const uint32_t l0 = c_left - h_left, b0 = c_b0 - h_b0, b1 = c_b1 - h_b1;
+                    const uint32_t l0 = f_left, b0 = f_b0, b1 = f_b1;
                     if (l0) atomicAdd((unsigned long long *)&fw.left[dt], (unsigned long long)l0);
                     if (b0) atomicAdd(&fw.below2[2 * dt], b0);
                     if (b1) atomicAdd(&fw.below2[2 * dt + 1], b1);
