@@ -71,6 +71,11 @@ struct Context {
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
     int passes_hint = 0;             // most decisions one level's search needed in the last call
+    // decisions each level's search needed in the last call, and that call's
+    // shape (points, levels, instances, dimension): a call of the same shape
+    // queues per level as many passes as that level needed
+    std::vector<uint8_t> level_passes;
+    uint64_t level_shape = 0;
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes

@@ -240,6 +240,15 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     const uint64_t parts_cap = B.parts_cap, tiles_cap = B.tiles_cap;
     if (tiles_cap > 0xffffffffull || parts_cap > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
     if ((uint64_t)L * (uint64_t)(spec_passes + 1) > (uint64_t)kSpecSlots) spec_passes = 0;
+    // passes queued per level: spec_passes, or, when the context's last call
+    // had this shape, as many as that level's search needed then (at least 1;
+    // a search that needs more is closed by its last queued decision and the
+    // batch runs again with host checks)
+    const uint64_t shape = (n * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)L << 48) ^ ((uint64_t)n_inst << 8) ^ (uint64_t)D;
+    std::vector<int> depth(L, spec_passes);
+    if (spec_passes > 0 && cx.level_shape == shape && cx.level_passes.size() == L)
+        for (size_t l = 0; l < L; l++) depth[l] = std::max(1, std::min(spec_passes, (int)cx.level_passes[l]));
+    std::vector<uint8_t> used(L, 0);   // decisions per level in this call
 
     // Two levels per move (dense trees, prefix partition): three part / tile
     // tables (a level, its children searched in place, its grandchildren).
@@ -375,18 +384,19 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // slot p counts the parts still searching after decision p; pass
             // p + 1 runs only if it is non-zero
             uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
+            const int queued = depth[level];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
                                                                &slot[0], swept_first, counts_first, nullptr,
                                                                nullptr);
             });
-            for (int p = 1; p <= spec_passes; p++) {
+            for (int p = 1; p <= queued; p++) {
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
                 // the last queued decision also closes the searches it leaves open
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                     k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
                         parts, np, bcnt, bw, bmin, nb, wrap32, &slot[p], &ctr->swept, counts_hist, &slot[p - 1],
-                        p == spec_passes ? &ctr->unfinished : nullptr);
+                        p == queued ? &ctr->unfinished : nullptr);
                 });
             }
             return;
@@ -415,6 +425,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         }
         cx.stats.passes += passes;
         cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+        used[level] = (uint8_t)std::min<uint32_t>(passes, 255);
     };
 
     // Search of parts whose points are contiguous in `tiles` (coordinates cl,
@@ -711,11 +722,18 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         // passes per level: the first decision plus every pass whose gate was open
         for (size_t l = 0; l < L; l++) {
             uint32_t passes = 1;
-            for (int p = 1; p <= spec_passes; p++)
+            for (int p = 1; p <= depth[l]; p++)
                 if (h->spec[l * (spec_passes + 1) + p - 1]) passes++;
             cx.stats.passes += passes;
             cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+            used[l] = (uint8_t)passes;
         }
+    }
+    if (h->unfinished == 0) {
+        cx.level_passes = used;
+        cx.level_shape = shape;
+    } else {
+        cx.level_shape = 0;
     }
     return h->unfinished != 0;
 }
