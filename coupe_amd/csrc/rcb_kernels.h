@@ -128,9 +128,9 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
             double dv[kDv];
             WIN wv[4];
 #pragma unroll
-            for (int j = 0; j < kDv; j++) dv[j] = __builtin_nontemporal_load(&src[min((uint32_t)(j * kBlock + tid), nd - 1u)]);
+            for (int j = 0; j < kDv; j++) dv[j] = src[min((uint32_t)(j * kBlock + tid), nd - 1u)];
 #pragma unroll
-            for (int k = 0; k < 4; k++) wv[k] = __builtin_nontemporal_load(&win[base + min((uint32_t)(k * kBlock + tid), cnt - 1u)]);
+            for (int k = 0; k < 4; k++) wv[k] = win[base + min((uint32_t)(k * kBlock + tid), cnt - 1u)];
 #pragma unroll
             for (int j = 0; j < kDv; j++)
                 if ((uint32_t)(j * kBlock + tid) < nd) sp[j * kBlock + tid] = dv[j];
