@@ -1519,8 +1519,9 @@ k_descend(const Node *__restrict__ nodes, uint64_t n_nodes, uint32_t levels, uin
 constexpr int kDenseLevels = 24;
 constexpr int kDenseCached = 12;   // 4095 splits, 16 KB of LDS
 
+// Six wavefronts per SIMD (72 VGPRs): more loads in flight than at five.
 template <int D>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
 k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance *__restrict__ insts,
                 const float *__restrict__ heap_split, const float *__restrict__ c0, const float *__restrict__ c1,
                 const float *__restrict__ c2, uint64_t *__restrict__ out) {

@@ -44,8 +44,9 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t *sh4) {
     return s;
 }
 
+// Six wavefronts per SIMD (80 VGPRs, a few spilled): faster than five.
 template <int D, typename WS, typename Acc>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
 k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
               const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
               const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
@@ -137,8 +138,9 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 // (the midpoint of the box on the next axis) and the smallest key at or
 // above it, and per tile the count below e1 and per side the count below the
 // children's candidate.  Reads the two axes and the weight once.
+// Six wavefronts per SIMD (80 VGPRs, a few spilled): faster than five.
 template <int D, typename WS, typename Acc>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
 k_root_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
              const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
              const WS *__restrict__ w, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
