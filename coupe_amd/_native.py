@@ -85,7 +85,7 @@ class Stats(C.Structure):
         ("device_bytes", C.c_uint64),
         ("first_read", C.c_uint64),
         ("forwarded", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("full_first", C.c_uint32),
     ]
 
     def as_dict(self):

@@ -95,6 +95,7 @@ Context *default_context() {
         c->pairs = coupe_hip::pairs_from_env();
         c->forward = coupe_hip::forward_from_env();
         c->weights16 = coupe_hip::weights16_from_env();
+        c->full_first = coupe_hip::full_first_from_env();
         g_default[dev] = c;
     }
     return g_default[dev];
@@ -170,6 +171,7 @@ extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
     c->pairs = coupe_hip::pairs_from_env();
     c->forward = coupe_hip::forward_from_env();
     c->weights16 = coupe_hip::weights16_from_env();
+    c->full_first = coupe_hip::full_first_from_env();
     return reinterpret_cast<coupe_hip_ctx *>(c);
 }
 

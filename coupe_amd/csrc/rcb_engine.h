@@ -69,12 +69,15 @@ struct Context {
     bool pairs = true;               // move the points once per two levels
     bool forward = true;             // the move forwards the next children's first pass
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
+    bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
     int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
     int passes_hint = 0;             // most decisions one level's search needed in the last call
     // decisions each level's search needed in the last call, and that call's
     // shape (points, levels, instances, dimension): a call of the same shape
     // queues per level as many passes as that level needed
     std::vector<uint8_t> level_passes;
+    // parts of each level still searching after their first decision in that call
+    std::vector<uint32_t> level_open;
     uint64_t level_shape = 0;
     bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
@@ -123,6 +126,14 @@ inline bool forward_from_env() {
 // they all fit in 16.
 inline bool weights16_from_env() {
     const char *v = getenv("COUPE_HIP_WEIGHTS16");
+    return !(v && v[0] == '0');
+}
+
+// COUPE_HIP_FULL_FIRST=0: the children's first pass always takes the bins
+// of their first candidate, even where the context's last call of the same
+// shape saw most of their parents still searching after the first decision.
+inline bool full_first_from_env() {
+    const char *v = getenv("COUPE_HIP_FULL_FIRST");
     return !(v && v[0] == '0');
 }
 

@@ -249,6 +249,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     if (spec_passes > 0 && cx.level_shape == shape && cx.level_passes.size() == L)
         for (size_t l = 0; l < L; l++) depth[l] = std::max(1, std::min(spec_passes, (int)cx.level_passes[l]));
     std::vector<uint8_t> used(L, 0);   // decisions per level in this call
+    std::vector<uint32_t> open1(L, 0);  // parts still searching after each level's first decision
+    const bool same_shape = cx.level_shape == shape && cx.level_open.size() == L;
 
     // Two levels per move (dense trees, prefix partition): three part / tile
     // tables (a level, its children searched in place, its grandchildren).
@@ -367,6 +369,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     cx.stats.levels = 0;
     cx.stats.passes = 0;
     cx.stats.max_passes = 0;
+    cx.stats.full_first = 0;
     cx.stats.unit_points = kTile;
     cx.stats.bins = kFirstBins;
 
@@ -411,6 +414,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         for (;;) {
             check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             check(hipStreamSynchronize(st));
+            if (passes == 1) open1[level] = *h_search;
             if (*h_search == 0) break;
             // f32 bisection ends within a few hundred candidates; each pass takes 8
             if (passes >= 4096) throw HipFailure{hipErrorUnknown};
@@ -532,15 +536,39 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             float *const *cl = cs[csl];
             WS *wl = ws[wsl];
             cx.stats.levels++;
+            // The children's first pass: the bins of their first candidate
+            // (forwarded by the move that made this level, or read by
+            // k_child_first for the parents whose split is not there, which
+            // also counts per tile the points left of the parent's split for
+            // the next move).  For the last pair of levels (no move follows),
+            // when in the context's last call of this shape most parents of
+            // this level were still searching after their first decision (so
+            // most children were not forwarded and needed later passes): one
+            // pass with every candidate over all the parents' tiles instead.
+            const uint64_t pub0 = std::max<uint64_t>(B.parts_ub[level], 1);
+            const bool full_first = cx.full_first && same_shape && level + 2 == L &&
+                                    2 * (uint64_t)cx.level_open[level] > pub0;
             // the move that made this level forwarded its children's first pass
-            const bool fwd_in = cx.forward && (level >= 2 || root_fwd) && level + 1 < L;
+            const bool fwd_in = cx.forward && (level >= 2 || root_fwd) && level + 1 < L && !full_first;
             uint64_t *pl = (uint64_t *)cx.pleft[par].ptr;
             uint32_t *pb2 = (uint32_t *)cx.pbelow2[par].ptr;
             children_of(ia, ib, level, nullptr, fwd_in ? fwd_at(par) : none, none);   // in place: no tiles
             if (level + 1 == L) break;
             cx.stats.levels++;
-            // the children's first pass over the parent's tiles
-            {
+            const uint64_t pub1 = std::max<uint64_t>(B.parts_ub[level + 1], 1);
+            const unsigned gv = persistent_grid(cx, k_hist_virtual<D, WS, Acc>, tl_ub);
+            auto hist_v = [&](int nb, const uint32_t *gate) {
+                k_hist_virtual<D, WS, Acc><<<gv, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2], wl, nb, bcnt,
+                                                                 bw, bmin, gate, tile_bins2);
+            };
+            if (full_first) {
+                cx.stats.full_first++;
+                // every child searching: the pass reads both axes and the weight
+                // of every point (counted with the first passes by the decision)
+                const int nbf = choose_bins(pub1);
+                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist_v(nbf, nullptr); });
+                search_gen(ib, level + 1, pub1, nbf, &ctr->first_read, kCountsBins, kCountsBins, hist_v);
+            } else {
                 const unsigned g = persistent_grid(cx, k_child_first<D, WS, Acc>, tl_ub);
                 // bytes: parent-axis and child-axis coordinates and the weight
                 // of every point read (counted on the device, added below)
@@ -549,16 +577,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                                                                    bmin, pb2, pl, fwd_in ? 1u : 0u,
                                                                    &ctr->first_read);
                 });
-            }
-            const uint64_t pub1 = std::max<uint64_t>(B.parts_ub[level + 1], 1);
-            {
-                const unsigned g = persistent_grid(cx, k_hist_virtual<D, WS, Acc>, tl_ub);
-                search_gen(ib, level + 1, pub1, 2, &ctr->swept_fused, kCountsBelow, kCountsBins,
-                           [&](int nb, const uint32_t *gate) {
-                               k_hist_virtual<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
-                                                                               wl, nb, bcnt, bw, bmin, gate,
-                                                                               tile_bins2);
-                           });
+                search_gen(ib, level + 1, pub1, 2, &ctr->swept_fused, kCountsBelow, kCountsBins, hist_v);
             }
             // grandchildren, with tiles
             Tile *tn = (Tile *)cx.tiles[ic].ptr;
@@ -727,13 +746,13 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             cx.stats.passes += passes;
             cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
             used[l] = (uint8_t)passes;
+            open1[l] = h->spec[l * (spec_passes + 1)];
         }
     }
-    if (h->unfinished == 0) {
+    if (h->unfinished == 0) {   // (an unfinished run keeps the last call's figures for its rerun)
         cx.level_passes = used;
+        cx.level_open = open1;
         cx.level_shape = shape;
-    } else {
-        cx.level_shape = 0;
     }
     return h->unfinished != 0;
 }

@@ -212,7 +212,7 @@ struct coupe_hip_stats {
 	uint64_t device_bytes;    /* device memory held by the context */
 	uint64_t first_read;      /* points read by the children's first passes (two levels per move) */
 	uint32_t forwarded;       /* parts whose children's first pass came with the move that filled them */
-	uint32_t reserved;
+	uint32_t full_first;      /* levels whose children's first pass took every candidate (last pair, adaptive) */
 };
 enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out);
 

@@ -286,6 +286,47 @@ def test_move_forwards_the_next_first_pass(oracle_mod, monkeypatch, forward):
         ctx.close()
 
 
+@pytest.mark.parametrize("full", ["0", "1"])
+def test_last_pair_full_first_pass(oracle_mod, monkeypatch, full):
+    """A second call of the same shape takes the last pair's children's first
+    pass with every candidate (COUPE_HIP_FULL_FIRST, default on) when, in the
+    first call, most parts two levels up were still searching after their
+    first decision: uniform clouds at tolerance 0, skewed clouds at tight
+    tolerances.  Both calls equal the
+    oracle, and with the switch on the second call takes that pass (the
+    `full_first` statistic)."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_FULL_FIRST", full)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        rng = np.random.default_rng(1500)
+        for q in range(6):
+            n = int(rng.integers(30_000, 400_000))
+            dim = 2 + q % 2
+            # uniform clouds at tolerance 0: no split ends at its first candidate
+            pts = rng.normal(0, 1, (n, dim)) ** 3 if q % 3 else rng.random((n, dim))
+            w = rng.integers(1, 1000, n).astype([np.int32, np.int64, np.float64][q % 3])
+            iters = [4, 6, 8][q % 3]
+            tol = float([0.0, 0.001, 0.01][q % 3])
+            say(f"full_first {full}, case {q}: n={n} d={dim} L={iters} tol={tol}")
+            exp, _ = oracle_rcb(oracle_mod, pts, w, iters, tol)
+            used = []
+            for call in range(2):
+                out = torch.zeros(n, dtype=torch.int64, device="cuda")
+                coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(
+                    out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+                torch.cuda.synchronize()
+                used.append(ctx.stats()["full_first"])
+                assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"full {full} case {q} call {call}"
+            assert used[0] == 0, f"case {q}: no hint on the first call"
+            if full == "0":
+                assert used[1] == 0
+            elif tol == 0.0:   # no part splits at its first candidate
+                assert used[1] == 1, f"case {q}: the second call's last pair took the 2-bin first pass"
+    finally:
+        ctx.close()
+
+
 @pytest.mark.parametrize("w16", ["0", "1"])
 def test_stored_weight_widths(oracle_mod, monkeypatch, w16):
     """Integer weights in [0, 65535] are stored in 16 bits (COUPE_HIP_WEIGHTS16,
