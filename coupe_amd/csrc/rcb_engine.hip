@@ -600,13 +600,21 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             scan_u64(cx, st, q4[0], nullptr, 4 * tl_ub, nullptr);
             const int csn = csl == 1 ? 2 : 1, wsn = wsl ^ 1;
             float *const *co = cs[csn];
+            // level l splits axis l mod D; after this move only levels
+            // level + 2 and level + 3 are split (or fewer): in 3D the axis of
+            // level + 1, which would be next split at level + 4, is not
+            // written (the ids come from the prologue's copy)
+            const uint32_t skip_axis = (D == 3 && level + 4 >= L) ? (uint32_t)((level + 1) % D) : 3u;
             {
                 auto move = [&](auto kernel) {
                     const unsigned g = persistent_grid(cx, kernel, tl_ub);
-                    timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
+                    // bytes: every point's payload read, and written less the skipped axis
+                    const double moved = 2.0 * payload - (skip_axis < 3 ? 4.0 : 0.0);
+                    timed(cx, st, COUPE_HIP_K_PARTITION, moved * (double)n, [&] {
                         kernel<<<g, kBlock, 0, st>>>(pa, pb, (const Part *)cx.parts[ic].ptr, tl, ntl, q4[0], q4[1],
                                                      q4[2], q4[3], cl[0], cl[1], cl[2], wl, co[0], co[1], co[2],
-                                                     ws[wsn], n, bcnt, bw, bmin, fwd_out ? fwd_at(par ^ 1) : none);
+                                                     ws[wsn], n, bcnt, bw, bmin, fwd_out ? fwd_at(par ^ 1) : none,
+                                                     skip_axis);
                     });
                 };
                 if (fwd_out) move(k_partition4<D, WS, Acc, true>);

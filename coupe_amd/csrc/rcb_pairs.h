@@ -463,6 +463,9 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 // smallest key at its first candidate t3) and, with FWD, the forwarding of its children's first pass (per side of t3: count, weight and
 // smallest key at their first candidate t4; per destination tile: the count
 // below t3 and per side the count below t4).
+// skip_axis (0..2, or 3 for none): a coordinate array that no later level
+// reads (3D, the last move: the children's axis is not split again), read for
+// the classification but not written.
 template <int D, typename WS, typename Acc, bool FWD>
 __global__ void __launch_bounds__(kBlock)
 k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
@@ -471,7 +474,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
              const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
              const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
              WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-             uint32_t *__restrict__ bmin, Forward<Acc> fw) {
+             uint32_t *__restrict__ bmin, Forward<Acc> fw, uint32_t skip_axis) {
     static_assert(kBlock / 64 == 4, "one wavefront per grandchild slot");
     __shared__ float sfa[kTile], sfb[kTile], sfc[D == 3 ? kTile : 1];
     __shared__ WS sws[kTile];
@@ -691,6 +694,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             // run lies in one tile: i < kTile + 128, so the mask below changes
             // nothing and shows the compiler the offsets' range)
             float *pa2 = oa2 + db, *pa3 = oa3 + db, *pa4 = oa4 + db;
+            const bool keep2 = (uint32_t)a2 != skip_axis;   // wave-uniform
             WS *pw = ow + db;
             // two positions per lane per iteration, every lane runs every
             // iteration: validity is in uniform lane masks, and the ballots
@@ -714,7 +718,8 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                     const uint64_t vm = __builtin_amdgcn_ballot_w64(vl);   // lanes holding a position of the run
                     const uint32_t i = (pos + (uint32_t)lane) & (4 * kTile - 1);
                     if (vl) {
-                        if constexpr (D == 3) __builtin_nontemporal_store(xc[u], &pa2[i]);
+                        if constexpr (D == 3)
+                            if (keep2) __builtin_nontemporal_store(xc[u], &pa2[i]);
                         __builtin_nontemporal_store(xa[u], &pa3[i]);
                         __builtin_nontemporal_store(xb[u], &pa4[i]);
                         __builtin_nontemporal_store(wq[u], &pw[i]);
