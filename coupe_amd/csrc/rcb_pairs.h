@@ -658,6 +658,15 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const uint32_t q = (pq[k] >> 8) < 4 ? (pq[k] >> 8) : 0u;
             cp[k] = wtot[q][k * (kBlock / 64) + wid] + (pq[k] & 0xffu);   // includes the run start of q
         }
+        // this wavefront's run, read before the staging barrier: no barrier
+        // ends the tile, so wavefront 0 may write the next tile's run
+        // starts and destinations while another still stores this tile's
+        // (the staged arrays, the counts and the scan are rewritten only after
+        // the next tile's first barrier, which every wavefront reaches after
+        // its store phase)
+        const uint32_t rb = s_run[wid], re = s_run[wid + 1];
+        const uint64_t dv = s_dst[wid];
+        const uint32_t run_off = s_off[wid], run_tile = s_tile[wid];
         // every array staged in slot order, then wavefront j stores run j
         // (3D: a3 = rz, a4 = the parent's axis rx, a2 = ry; 2D: a3 = rx, a4 = ry)
 #pragma unroll
@@ -678,13 +687,12 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             float *oa2 = a2 == 0 ? o0 : (a2 == 1 ? o1 : o2);
             float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
             float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
-            const uint32_t rb = s_run[wid];
-            const uint64_t dv = s_dst[wid];   // wave-uniform: into SGPRs once, for the stores' bases
+            // wave-uniform: into SGPRs once, for the stores' bases
             const uint64_t db = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(dv >> 32)) << 32) |
                                 __builtin_amdgcn_readfirstlane((uint32_t)dv);
-            uint32_t len = s_run[wid + 1] - rb;
+            uint32_t len = re - rb;
             len = db + len <= n_total ? len : (db < n_total ? (uint32_t)(n_total - db) : 0u);   // never past the arrays
-            const uint32_t split_i = kTile - s_off[wid];   // positions from here on are in the next tile
+            const uint32_t split_i = kTile - run_off;   // positions from here on are in the next tile
             uint32_t c_left = 0, c_b0 = 0, c_b1 = 0;
             // FWD: the same counts over the positions in the first destination
             // tile (positions below split_i), taken where the run crosses it
@@ -766,7 +774,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 if (!crossed) { f_left = c_left; f_b0 = c_b0; f_b1 = c_b1; }
                 const uint32_t h_left = c_left - f_left, h_b0 = c_b0 - f_b0, h_b1 = c_b1 - f_b1;
                 if (lane == 0) {
-                    const uint64_t dt = s_tile[wid];
+                    const uint64_t dt = run_tile;
                     const uint32_t l0 = f_left, b0 = f_b0, b1 = f_b1;
                     if (l0) atomicAdd((unsigned long long *)&fw.left[dt], (unsigned long long)l0);
                     if (b0) atomicAdd(&fw.below2[2 * dt], b0);
@@ -777,7 +785,6 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 }
             }
         }
-        __syncthreads();
     }
     if (cur != 0xffffffffu) flush();
 }
