@@ -135,8 +135,7 @@ void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev
             return;
         }
         k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, nullptr);
-        k_scan_top<<<1, kBlock, 0, st>>>(bsum, n_dev, n_ub, total);
-        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub);
+        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub, total);
     });
 }
 

@@ -597,32 +597,36 @@ k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
-k_scan_top(uint64_t *__restrict__ bsum, const uint32_t *n_dev, uint64_t n_host, uint64_t *total) {
-    __shared__ uint64_t wsum[kBlock / 64];
-    __shared__ uint64_t tot;
-    const uint64_t nb = ((n_dev ? *n_dev : n_host) + kScanTile - 1) / kScanTile;
-    uint64_t carry = 0;
-    for (uint64_t b0 = 0; b0 < nb; b0 += kBlock) {
-        uint64_t i = b0 + threadIdx.x;
-        uint64_t v = i < nb ? bsum[i] : 0;
-        uint64_t e = block_excl_scan(v, wsum, &tot);
-        if (i < nb) bsum[i] = carry + e;
-        carry += tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
-}
-
+// The second half of a multi-block scan: block b adds the totals of blocks
+// 0..b-1 (read from bsum, the block totals k_scan_local wrote) to its
+// elements; block 0 writes the scan's total.  Each block sums the preceding
+// totals itself, so no separate scan of the totals is launched.
 __global__ void __launch_bounds__(kBlock)
 k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const uint32_t *n_dev,
-           uint64_t n_host) {
+           uint64_t n_host, uint64_t *total) {
+    __shared__ uint64_t wsum[kBlock / 64];
     const uint64_t n = n_dev ? *n_dev : n_host;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
-    if (base >= n) return;
-    const uint64_t add = bsum[blockIdx.x];
+    if (base >= n) {   // past the elements (block 0 only when there are none)
+        if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = 0;
+        return;
+    }
+    const uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    const uint64_t upto = blockIdx.x == 0 ? nb : blockIdx.x;   // block 0: every total
+    uint64_t s = 0;
+    for (uint64_t i = threadIdx.x; i < upto; i += kBlock) s += bsum[i];
+    s = wave_sum(s);
+    if (lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    s = 0;
+#pragma unroll
+    for (int v = 0; v < kBlock / 64; v++) s += wsum[v];
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0 && total) *total = s;
+        return;   // block 0's elements need no carry
+    }
     for (int k = threadIdx.x; k < kScanTile; k += kBlock)
-        if (base + k < n) out[base + k] += add;
+        if (base + k < n) out[base + k] += s;
 }
 
 // ---------------------------------------------------------------------------
