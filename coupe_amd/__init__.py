@@ -67,6 +67,21 @@ def _is_device_tensor(x) -> bool:
     return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
 
 
+def _common_device(**tensors) -> int:
+    """The device index shared by every tensor; TypeError when one is not a
+    GPU tensor or they sit on different devices (a host pointer handed to a
+    kernel would fault the GPU)."""
+    devs = set()
+    for name, t in tensors.items():
+        if not _is_device_tensor(t):
+            raise TypeError(f"{name} must be a GPU tensor when any input is (got {type(t).__name__}"
+                            f"{' on ' + str(t.device) if hasattr(t, 'device') else ''})")
+        devs.add(t.device.index or 0)
+    if len(devs) != 1:
+        raise TypeError(f"inputs on several devices: {sorted(devs)}")
+    return devs.pop()
+
+
 def _weight_type(dtype) -> int:
     key = np.dtype(str(dtype).replace("torch.", ""))
     if key not in _WTYPES:
@@ -199,8 +214,10 @@ class Rcb(Partition):
             raise TypeError("points must be a contiguous float64 tensor of shape (N, D)")
         if not weights.is_contiguous():
             raise TypeError("weights must be contiguous")
-        dev = points.device.index or 0
+        dev = _common_device(part_ids=part_ids, points=points, weights=weights)
         ctx = context or _context_for(dev)
+        if ctx.device != dev:
+            raise TypeError(f"the context is on device {ctx.device}, the tensors on device {dev}")
         st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
         dim = points.shape[1] if points.dim() == 2 else 0
         wtype = _weight_type(weights.dtype)
@@ -250,9 +267,8 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
             n, iters, tol = it
             specs.append((int(n), int(iters), float(tol)))
     total = sum(n for n, _, _ in specs)
+    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
     for name, t in (("part_ids", part_ids), ("points", points), ("weights", weights)):
-        if not _is_device_tensor(t):
-            raise TypeError(f"{name} must be a device tensor")
         if t.shape[0] != total:
             raise InputLenMismatch(total, t.shape[0])
         if not t.is_contiguous():
@@ -262,8 +278,9 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
     arr = (_native.Instance * max(len(specs), 1))()
     for q, (n, iters, tol) in enumerate(specs):
         arr[q].n, arr[q].iter_count, arr[q].tolerance = n, iters, tol
-    dev = points.device.index or 0
     ctx = context or _context_for(dev)
+    if ctx.device != dev:
+        raise TypeError(f"the context is on device {ctx.device}, the tensors on device {dev}")
     st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
     dim = points.shape[1] if points.dim() == 2 else 0
     err = _native.lib().coupe_hip_rcb_batch(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), dim,

@@ -90,12 +90,7 @@ Context *default_context() {
         Context *c = new (std::nothrow) Context();
         if (!c) return nullptr;
         c->device = dev;
-        c->lookback = coupe_hip::lookback_from_env();
-        c->search_passes = coupe_hip::search_passes_from_env();
-        c->pairs = coupe_hip::pairs_from_env();
-        c->forward = coupe_hip::forward_from_env();
-        c->weights16 = coupe_hip::weights16_from_env();
-        c->full_first = coupe_hip::full_first_from_env();
+        coupe_hip::configure_from_env(*c);
         g_default[dev] = c;
     }
     return g_default[dev];
@@ -166,12 +161,7 @@ extern "C" coupe_hip_ctx *coupe_hip_ctx_create(int device) {
     Context *c = new (std::nothrow) Context();
     if (!c) return nullptr;
     c->device = device;
-    c->lookback = coupe_hip::lookback_from_env();
-    c->search_passes = coupe_hip::search_passes_from_env();
-    c->pairs = coupe_hip::pairs_from_env();
-    c->forward = coupe_hip::forward_from_env();
-    c->weights16 = coupe_hip::weights16_from_env();
-    c->full_first = coupe_hip::full_first_from_env();
+    coupe_hip::configure_from_env(*c);
     return reinterpret_cast<coupe_hip_ctx *>(c);
 }
 

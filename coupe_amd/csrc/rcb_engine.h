@@ -56,7 +56,6 @@ struct Context {
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
     DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
     DevBuf bsum;                     // scan block sums
-    DevBuf status;                   // look-back status words, one per tile
     DevBuf nodes, leaf;              // split tree of all levels, part ids of the leaves
     DevBuf heap_split;               // split values by heap index (shallow trees)
     DevBuf instances;                // instance table of the current call
@@ -70,6 +69,7 @@ struct Context {
     bool forward = true;             // the move forwards the next children's first pass
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
+    uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
     int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
     int passes_hint = 0;             // most decisions one level's search needed in the last call
     // decisions each level's search needed in the last call, and that call's
@@ -79,7 +79,6 @@ struct Context {
     // parts of each level still searching after their first decision in that call
     std::vector<uint32_t> level_open;
     uint64_t level_shape = 0;
-    bool lookback = false;           // partition with decoupled look-back instead
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
     DevBuf gparts[2], gaxis, gchunk, gcount, gctr;   // Grid::rcb: subgrids, axis sums, chunk offsets
@@ -91,7 +90,6 @@ struct Context {
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     void *host_scratch = nullptr;    // pinned, for small readbacks
     bool bins_clean = false;
-    uint32_t epoch = 0;              // look-back epoch of the last partition launch
     int cu_count = 0;                // compute units of the device
     std::vector<std::pair<const void *, int>> occupancy;   // resident workgroups per CU, per kernel
     double swept_bytes = 0.0;        // HIST algorithmic bytes of the last call
@@ -102,12 +100,6 @@ struct Context {
 };
 
 enum coupe_err map_hip_error(hipError_t e);
-
-// COUPE_HIP_PARTITION=lookback selects the look-back partition.
-inline bool lookback_from_env() {
-    const char *v = getenv("COUPE_HIP_PARTITION");
-    return v && v[0] == 'l';
-}
 
 // COUPE_HIP_PAIRS=0: move the points at every level instead of every other.
 inline bool pairs_from_env() {
@@ -137,6 +129,15 @@ inline bool full_first_from_env() {
     return !(v && v[0] == '0');
 }
 
+// COUPE_HIP_SCAN_DIRECT=b: a scan of more than b blocks scans the block
+// totals instead of re-summing them per block (default 2048, at least 1).
+inline uint64_t scan_direct_from_env() {
+    const char *v = getenv("COUPE_HIP_SCAN_DIRECT");
+    if (!v || !*v) return 2048;
+    const long long k = atoll(v);
+    return k < 1 ? 1 : (uint64_t)k;
+}
+
 // COUPE_HIP_SEARCH_PASSES=k: histogram passes queued per level before the
 // host looks (default 3; 0 = the host checks after every pass).  A context
 // queues fewer, never under 2, when its last call needed fewer.
@@ -145,6 +146,16 @@ inline int search_passes_from_env() {
     if (!v || !*v) return 3;
     const int k = atoi(v);
     return k < 0 ? 0 : (k > 16 ? 16 : k);
+}
+
+// A context's switches from the environment (the COUPE_HIP_* variables above).
+inline void configure_from_env(Context &c) {
+    c.search_passes = search_passes_from_env();
+    c.pairs = pairs_from_env();
+    c.forward = forward_from_env();
+    c.weights16 = weights16_from_env();
+    c.full_first = full_first_from_env();
+    c.scan_direct = scan_direct_from_env();
 }
 
 // RCB of a batch of independent instances packed in input order (see

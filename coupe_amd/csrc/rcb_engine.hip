@@ -98,7 +98,6 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 
 // Small device counters shared by the kernels of one call.
 constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
-constexpr int kMaxTickets = 4096;
 constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
 struct Counters {
     uint32_t n_parts[3];
@@ -114,7 +113,7 @@ struct Counters {
     uint32_t forwarded;                         // parts whose children's first pass the move forwarded
     uint32_t unfinished;                        // a speculative search did not end in its passes
     uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
-    unsigned long long ticket[kMaxTickets];     // partition ticket counter, one per level
+    uint32_t end;                               // end of the call's readback
 };
 
 static inline unsigned grid_for(uint64_t items, unsigned per_block) {
@@ -122,20 +121,37 @@ static inline unsigned grid_for(uint64_t items, unsigned per_block) {
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// One level of the exclusive scan below: block totals at bsum, and the
+// space past them for the levels under it.
+static void scan_level(hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub, uint64_t *total,
+                       uint64_t *bsum, uint64_t direct) {
+    const uint64_t nblocks = std::max<uint64_t>((n_ub + kScanTile - 1) / kScanTile, 1);
+    if (nblocks == 1) {
+        k_scan_local<<<1, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, total ? total : bsum + 1);
+        return;
+    }
+    k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, nullptr);
+    if (nblocks <= direct) {
+        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub, total);
+        return;
+    }
+    // many blocks: scan their totals (zero past the elements), then add
+    scan_level(st, bsum, nullptr, nblocks, total, bsum + nblocks + 1, direct);
+    k_scan_add_scanned<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub);
+}
+
 // Exclusive scan of `n_dev` (device) uint64 values, in place, upper bound n_ub.
 void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev, uint64_t n_ub,
                      uint64_t *total) {
-    uint64_t nblocks = (n_ub + kScanTile - 1) / kScanTile;
-    if (nblocks < 1) nblocks = 1;
-    cx.bsum.ensure((nblocks + 1) * sizeof(uint64_t));
-    uint64_t *bsum = (uint64_t *)cx.bsum.ptr;
+    // block totals of every level of the recursion, each followed by a spare entry
+    uint64_t need = 2, m = std::max<uint64_t>(n_ub, 1);
+    while (m > 1) {
+        m = (m + kScanTile - 1) / kScanTile;
+        need += m + 1;
+    }
+    cx.bsum.ensure(need * sizeof(uint64_t));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        if (nblocks == 1) {
-            k_scan_local<<<1, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, total ? total : bsum + 1);
-            return;
-        }
-        k_scan_local<<<(unsigned)nblocks, kBlock, 0, st>>>(data, data, bsum, n_dev, n_ub, nullptr);
-        k_scan_add<<<(unsigned)nblocks, kBlock, 0, st>>>(data, bsum, n_dev, n_ub, total);
+        scan_level(st, data, n_dev, n_ub, total, (uint64_t *)cx.bsum.ptr, cx.scan_direct);
     });
 }
 
@@ -253,7 +269,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
     // Two levels per move (dense trees, prefix partition): three part / tile
     // tables (a level, its children searched in place, its grandchildren).
-    const bool pairs = cx.pairs && B.dense && !cx.lookback;
+    const bool pairs = cx.pairs && B.dense;
     for (int b = 0; b < (pairs ? 3 : 2); b++) {
         cx.parts[b].ensure(parts_cap * sizeof(Part));
         cx.tiles[b].ensure(tiles_cap * sizeof(Tile));
@@ -300,30 +316,19 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // read it, while the two ping-pong tables serve levels 1, 2, ...
     Tile *tiles0 = (Tile *)cx.tiles0.ptr;
 
-    // Partition without look-back (default): per-tile left counts and their
-    // scan.  COUPE_HIP_PARTITION=lookback selects the decoupled look-back.
-    const bool lookback = cx.lookback;
+    // Partition: per-tile left counts and their scan.
     cx.tile_left.ensure(tiles_cap * sizeof(uint64_t));
     cx.below.ensure(tiles_cap * sizeof(uint32_t));
     // Per-tile bin counts of the histogram passes (prefix partition): the
     // partition's per-tile left counts of a part decided in such a pass.
-    uint16_t *tile_bins = nullptr;
-    if (!lookback) {
-        cx.tile_bins.ensure(tiles_cap * kMaxBins * sizeof(uint16_t));
-        tile_bins = (uint16_t *)cx.tile_bins.ptr;
-    }
-    const uint32_t hist_counts = lookback ? kCountsRead : kCountsBins;
+    cx.tile_bins.ensure(tiles_cap * kMaxBins * sizeof(uint16_t));
+    uint16_t *tile_bins = (uint16_t *)cx.tile_bins.ptr;
+    const uint32_t hist_counts = kCountsBins;
     // The per-tile counts start at zero: each partition's fused pass adds to
     // the entries of the next level's tiles and k_tile_left clears what it
     // reads, but the last level's counts are never read (no partition), so
     // a call cannot rely on the previous one leaving them clear.
-    if (!lookback && !pairs) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
-    const bool fresh_status = cx.status.bytes < tiles_cap * 8;
-    cx.status.ensure(tiles_cap * 8);
-    if (fresh_status || cx.epoch >= (1u << 30) - (uint32_t)std::min<size_t>(L, 1u << 20) - 2) {
-        check(hipMemsetAsync(cx.status.ptr, 0, cx.status.bytes, st));
-        cx.epoch = 0;
-    }
+    if (!pairs) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
     // Bins: parts x bins of the widest pass of any level.
     uint64_t bins = 1;
     for (size_t l = 0; l < L; l++)
@@ -450,7 +455,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         (void)n_parts_host;
         search_at(sl, level, parts_ub, tiles_ub, first_standalone, level == 0 ? tiles0 : (Tile *)cx.tiles[sl].ptr,
                   cs[cslot(level)], ws[wslot(level)],
-                  (first_standalone || lookback) ? kCountsRead : kCountsBelow);
+                  first_standalone ? kCountsRead : kCountsBelow);
     };
 
     // Level 0: the root parts' tables come from k_init_roots (in prologue());
@@ -671,18 +676,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // Partition of this level, fused with the children's first pass.
             float *const *ci = cs[cslot(level)];
             float *const *co = cs[cslot(level + 1)];
-            if (lookback) {
-                const uint32_t epoch = ++cx.epoch;
-                unsigned long long *ticket = &ctr->ticket[level % kMaxTickets];
-                if (level > 0 && level % kMaxTickets == 0)
-                    check(hipMemsetAsync(&ctr->ticket[0], 0, sizeof(ctr->ticket), st));
-                const unsigned g = persistent_grid(cx, k_partition<D, WS, Acc>, tiles_ub);
-                timed(cx, st, COUPE_HIP_K_PARTITION, 2.0 * payload * (double)n, [&] {
-                    k_partition<D, WS, Acc><<<g, kBlock, 0, st>>>(
-                        parts, tiles, ntl, ticket, (unsigned long long *)cx.status.ptr, epoch, &ctr->error, ci[0],
-                        ci[1], ci[2], ws[wslot(level)], co[0], co[1], co[2], ws[wslot(level + 1)], n, bcnt, bw, bmin);
-                });
-            } else {
+            {
                 const unsigned gt = persistent_grid(cx, k_tile_left, (tiles_ub + kBlock / 64 - 1) / (kBlock / 64));
                 timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                     k_tile_left<<<gt, kBlock, 0, st>>>(parts, tiles, ntl, ci[0], ci[1], ci[2], (uint32_t *)cx.below.ptr,
@@ -733,8 +727,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // One readback for the whole call: the counters from `swept` to the end
     // of the speculative slots.
     const size_t rb_off = offsetof(Counters, swept);
-    const size_t rb_len = offsetof(Counters, ticket) - rb_off;
-    static_assert(offsetof(Counters, ticket) - offsetof(Counters, swept) <= kHostScratch - 256, "readback");
+    const size_t rb_len = offsetof(Counters, end) - rb_off;
+    static_assert(offsetof(Counters, end) - offsetof(Counters, swept) <= kHostScratch - 256, "readback");
     char *h_rb = (char *)cx.host_scratch + 256;
     check(hipMemcpyAsync(h_rb, (char *)ctr + rb_off, rb_len, hipMemcpyDeviceToHost, st));
     check(hipStreamSynchronize(st));
@@ -974,7 +968,7 @@ uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                            &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
-                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
+                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf, &heap_split,
                            &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (const DevBuf *d : all) b += d->bytes;
     return b;
@@ -983,7 +977,7 @@ uint64_t Context::device_bytes() const {
 void Context::release_all() {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
                      &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
-                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &status, &nodes, &leaf, &heap_split,
+                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf, &heap_split,
                      &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) d->release();
     if (host_scratch) (void)hipHostFree(host_scratch);

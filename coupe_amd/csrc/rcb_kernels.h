@@ -12,8 +12,8 @@
 //   k_decide     one wavefront per searching part: prefix sums over the bins,
 //                then lane 0 replays coupe's split loop exactly (:470-571)
 //   k_children   next level's parts and tiles (:603-641)
-//   k_partition  stable split of every part into {c < t} | {c >= t} with a
-//                decoupled look-back across the part's tiles and LDS-staged
+//   k_partition_tiles  stable split of every part into {c < t} | {c >= t}
+//                from per-tile left counts and their scan, with LDS-staged
 //                contiguous stores; in the same sweep it bins the children's
 //                next-axis coordinates (their first split pass)
 //   k_descend    part ids in input order by walking the split tree (:589-601,
@@ -547,6 +547,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
+constexpr uint64_t kScanAddDirect = 2048;   // blocks whose totals k_scan_add re-sums per block
 
 __device__ inline uint64_t block_excl_scan(uint64_t v, uint64_t *wsum, uint64_t *total) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
@@ -575,7 +576,12 @@ k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
     const uint64_t n = n_dev ? *n_dev : n_host;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     if ((uint64_t)blockIdx.x * kScanTile >= n) {
-        if (single && threadIdx.x == 0) *single = 0;
+        // past the elements: a zero total, so a scan of the totals over the
+        // launch's blocks stays exact
+        if (threadIdx.x == 0) {
+            bsum[blockIdx.x] = 0;
+            if (single) *single = 0;
+        }
         return;
     }
     uint64_t v[kScanItems];
@@ -597,10 +603,11 @@ k_scan_local(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
     }
 }
 
-// The second half of a multi-block scan: block b adds the totals of blocks
-// 0..b-1 (read from bsum, the block totals k_scan_local wrote) to its
-// elements; block 0 writes the scan's total.  Each block sums the preceding
-// totals itself, so no separate scan of the totals is launched.
+// The second half of a multi-block scan of at most kScanAddDirect blocks:
+// block b adds the totals of blocks 0..b-1 (read from bsum, the block totals
+// k_scan_local wrote) to its elements; block 0 writes the scan's total.  Each
+// block sums the preceding totals itself (at most kScanAddDirect loads per
+// block), so no separate scan of the totals is launched.
 __global__ void __launch_bounds__(kBlock)
 k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const uint32_t *n_dev,
            uint64_t n_host, uint64_t *total) {
@@ -625,6 +632,19 @@ k_scan_add(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const 
         if (threadIdx.x == 0 && total) *total = s;
         return;   // block 0's elements need no carry
     }
+    for (int k = threadIdx.x; k < kScanTile; k += kBlock)
+        if (base + k < n) out[base + k] += s;
+}
+
+// The second half of a scan of more blocks: the block totals were scanned
+// (exclusive, in place); block b adds bsum[b] to its elements.
+__global__ void __launch_bounds__(kBlock)
+k_scan_add_scanned(uint64_t *__restrict__ out, const uint64_t *__restrict__ bsum, const uint32_t *n_dev,
+                   uint64_t n_host) {
+    const uint64_t n = n_dev ? *n_dev : n_host;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    if (base >= n || blockIdx.x == 0) return;
+    const uint64_t s = bsum[blockIdx.x];
     for (int k = threadIdx.x; k < kScanTile; k += kBlock)
         if (base + k < n) out[base + k] += s;
 }
@@ -937,21 +957,8 @@ k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Stable partition of every part with a decoupled look-back over the part's
-// tiles, fused with the children's first split pass.
-//
-// Tiles are taken in ticket order, so a tile's predecessors are already being
-// processed.  Status word of a tile: bits 0-31 value, 32-33 flag
-// (1 aggregate: this tile's left count; 2 inclusive: left count of the part up
-// to and including this tile), 34-63 epoch of the launch.
+// Children's first split pass, fused into a partition.
 // ---------------------------------------------------------------------------
-constexpr uint64_t kFlagAgg = 1ull << 32;
-constexpr uint64_t kFlagInc = 2ull << 32;
-
-__device__ inline uint64_t lb_word(uint32_t epoch, uint64_t flag, uint32_t value) {
-    return ((uint64_t)epoch << 34) | flag | value;
-}
-
 // Fused first pass of the children: one candidate t2 (the midpoint of their
 // box on the next axis, the same for both children).  Per child only the
 // count and weight of {c < t2} and the smallest c >= t2 are needed by the
@@ -1022,197 +1029,8 @@ __device__ __forceinline__ void flush_child_bins(ChildBins<Acc> &r, ChildBinsSha
     __syncthreads();
 }
 
-template <int D, typename WS, typename Acc>
-__global__ void __launch_bounds__(kBlock)
-k_partition(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
-            unsigned long long *ticket, unsigned long long *status, uint32_t epoch, uint32_t *error,
-            const float *__restrict__ i0, const float *__restrict__ i1, const float *__restrict__ i2,
-            const WS *__restrict__ iw, float *__restrict__ o0, float *__restrict__ o1, float *__restrict__ o2,
-            WS *__restrict__ ow, uint64_t n_total, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-            uint32_t *__restrict__ bmin) {
-    __shared__ uint64_t stage[kTile];            // two 4-byte arrays, or one 8-byte array
-    __shared__ uint32_t wtot[kItems][kBlock / 64];
-    __shared__ uint32_t s_tile_left, s_prefix, s_t;
-    __shared__ ChildBinsShared<Acc> sh;
-    float *sf0 = reinterpret_cast<float *>(stage);
-    float *sf1 = sf0 + kTile;
-    WS *sw = reinterpret_cast<WS *>(stage);
-    WS *sw1 = reinterpret_cast<WS *>(sf1);       // second half, for 4-byte weights
-
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid >> 6;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t nt = *n_tiles;
-    ChildBins<Acc> r;
-    r.reset();
-    uint32_t cur = 0xffffffffu;
-    int32_t ch0 = -1, ch1 = -1;
-    int a2 = 0;
-    float t2 = 0.f;
-
-    for (;;) {
-        if (tid == 0) s_t = (uint32_t)atomicAdd(ticket, 1ull);
-        __syncthreads();
-        const uint32_t t = s_t;
-        if (t >= nt) break;
-        const Tile T = tiles[t];
-        const Part &P = parts[T.part];
-        if (T.part != cur) {
-            if (cur != 0xffffffffu) { flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin); r.reset(); }
-            cur = T.part;
-            ch0 = P.child[0];
-            ch1 = P.child[1];
-            a2 = (int)((P.axis + 1) % D);
-            // both children keep the parent's box on the next axis (:613-616)
-            const float blo = sel3(a2, P.lo[0], P.lo[1], P.lo[2]);
-            const float bhi = sel3(a2, P.hi[0], P.hi[1], P.hi[2]);
-            t2 = (blo + bhi) / 2.0f;                     // their first candidate (:472)
-        }
-        // parts at their instance's last level are not moved (their leaves
-        // are read back by the descent)
-        if (P.status == kDone || P.depth + 1 >= P.levels) {
-            __syncthreads();
-            continue;
-        }
-        const int ax = (int)P.axis;
-        const bool all_left = P.status == kAllLeft;
-        const float split = P.split;
-        const uint64_t end = T.begin + T.count;
-
-        float x0[kItems], x1[kItems], x2[kItems];
-        WS wv[kItems];
-        bool f[kItems];
-        uint32_t pre[kItems];
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            // clamped, unconditional loads: all of them issue back to back
-            // (the values past the tile's end are never used)
-            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            const uint64_t c = p < end ? p : end - 1;
-            x0[k] = i0[c];
-            x1[k] = i1[c];
-            if constexpr (D == 3) x2[k] = i2[c];
-            else x2[k] = 0.f;
-            wv[k] = iw[c];
-        }
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            const uint64_t p = T.begin + (uint64_t)k * kBlock + tid;
-            // !(c >= t) is c < t for numbers and sends NaN left, as bin_of does
-            f[k] = p < end && (all_left || !(sel3(ax, x0[k], x1[k], x2[k]) >= split));
-            const uint64_t bal = __ballot(f[k]);
-            pre[k] = __popcll(bal & lt_mask);
-            if (lane == 0) wtot[k][wid] = __popcll(bal);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < kItems; k++)
-                for (int v = 0; v < kBlock / 64; v++) { uint32_t c = wtot[k][v]; wtot[k][v] = run; run += c; }
-            s_tile_left = run;
-            // decoupled look-back within the part
-            uint64_t prefix = 0;
-            if (t == P.tile_begin) {
-                __hip_atomic_store(&status[t], lb_word(epoch, kFlagInc, run), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                __hip_atomic_store(&status[t], lb_word(epoch, kFlagAgg, run), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t j = t - 1;
-                uint32_t spins = 0;
-                for (;;) {
-                    const uint64_t w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const uint64_t fl = w & (3ull << 32);
-                    if ((uint32_t)(w >> 34) == epoch && fl != 0) {
-                        prefix += (uint32_t)w;
-                        if (fl == kFlagInc || j == P.tile_begin) break;
-                        j--;
-                    } else {
-                        // bounded wait: a lost predecessor becomes an error, not a hang
-                        if (++spins > (1u << 22)) { atomicOr(error, 1u); break; }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                __hip_atomic_store(&status[t], lb_word(epoch, kFlagInc, (uint32_t)(prefix + run)),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            s_prefix = (uint32_t)prefix;
-        }
-        __syncthreads();
-        const uint32_t tl = s_tile_left;
-        const uint64_t prefix = s_prefix;
-        const uint64_t lbase = P.begin + prefix;
-        const uint64_t rbase = P.begin + P.cnt_left + ((T.begin - P.begin) - prefix);
-        const uint32_t cnt = T.count;
-        uint32_t cp[kItems];
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            const uint32_t off = k * kBlock + tid;
-            const uint32_t rl = wtot[k][wid] + pre[k];
-            cp[k] = f[k] ? rl : tl + (off - rl);
-        }
-        // LDS holds the tile in output order; each store then covers
-        // contiguous runs (the left run, then the right run).
-#define COUPE_STORE_RUN(EXPR)                                                  \
-    _Pragma("unroll") for (int k = 0; k < kItems; k++) {                       \
-        const uint32_t pos = k * kBlock + tid;                                 \
-        if (pos < cnt) {                                                       \
-            const uint64_t d = pos < tl ? lbase + pos : rbase + (pos - tl);    \
-            if (d < n_total) { EXPR; }                                         \
-        }                                                                      \
-    }
-        // round 1: axes 0 and 1
-#pragma unroll
-        for (int k = 0; k < kItems; k++)
-            if (k * kBlock + tid < (int)cnt) { sf0[cp[k]] = x0[k]; sf1[cp[k]] = x1[k]; }
-        __syncthreads();
-        COUPE_STORE_RUN(o0[d] = sf0[pos]; o1[d] = sf1[pos])
-        __syncthreads();
-        // round 2: axis 2 and 2- or 4-byte weights together; 8-byte weights alone
-        if constexpr (sizeof(WS) <= 4) {
-#pragma unroll
-            for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) {
-                    if constexpr (D == 3) sf0[cp[k]] = x2[k];
-                    sw1[cp[k]] = wv[k];
-                }
-            __syncthreads();
-            if constexpr (D == 3) {
-                COUPE_STORE_RUN(o2[d] = sf0[pos]; ow[d] = sw1[pos])
-            } else {
-                COUPE_STORE_RUN(ow[d] = sw1[pos])
-            }
-            __syncthreads();
-        } else {
-            if constexpr (D == 3) {
-#pragma unroll
-                for (int k = 0; k < kItems; k++)
-                    if (k * kBlock + tid < (int)cnt) sf0[cp[k]] = x2[k];
-                __syncthreads();
-                COUPE_STORE_RUN(o2[d] = sf0[pos])
-                __syncthreads();
-            }
-#pragma unroll
-            for (int k = 0; k < kItems; k++)
-                if (k * kBlock + tid < (int)cnt) sw[cp[k]] = wv[k];
-            __syncthreads();
-            COUPE_STORE_RUN(ow[d] = sw[pos])
-            __syncthreads();
-        }
-#undef COUPE_STORE_RUN
-        // children's first pass
-#pragma unroll
-        for (int k = 0; k < kItems; k++) {
-            if (k * kBlock + tid < (int)cnt) {
-                const float xb = sel3(a2, x0[k], x1[k], x2[k]);
-                r.add(!f[k], !(xb >= t2), AccOps<Acc>::of(wv[k]), f32_key(xb));
-            }
-        }
-    }
-    if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
-}
-
 // ---------------------------------------------------------------------------
-// Partition without look-back.  Every tile's left count is known before the
+// Partition of one level.  Every tile's left count is known before the
 // partition: k_tile_left gathers it (from the fused counts of the previous
 // partition when the part stopped at its first candidate, else by counting),
 // a scan turns the counts into offsets, and every tile is then independent.

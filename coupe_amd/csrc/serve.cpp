@@ -193,12 +193,7 @@ extern "C" coupe_hip_server *coupe_hip_server_create(const struct coupe_hip_serv
             return nullptr;
         }
         W->cx.device = config->device;
-        W->cx.lookback = coupe_hip::lookback_from_env();
-        W->cx.search_passes = coupe_hip::search_passes_from_env();
-        W->cx.pairs = coupe_hip::pairs_from_env();
-        W->cx.forward = coupe_hip::forward_from_env();
-        W->cx.weights16 = coupe_hip::weights16_from_env();
-        W->cx.full_first = coupe_hip::full_first_from_env();
+        coupe_hip::configure_from_env(W->cx);
         S->workers.push_back(W);
     }
     for (Worker *W : S->workers) W->th = std::thread(worker_main, S, W);

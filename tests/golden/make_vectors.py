@@ -3,7 +3,7 @@
 No reference code is run: every expected array below was traced by hand
 through coupe's algorithm (coupe/src/algorithms/recursive_bisection.rs
 :456-705) and each trace is written next to its vector.  V1-V3 come from
-the reference's own tests and examples; V4-V7 are built to hit each exit of
+the reference's own tests and examples; V4-V9 are built to hit each exit of
 the split loop.  The AABB vectors are the reference's geometry tests.
 
 Run:  python tests/golden/make_vectors.py
@@ -89,6 +89,28 @@ VECTORS = [
         expected=[0, 0, 0, 1],
         trace="t=1: only point 0 is < t, imb .5, go right; t=1.5: left {0,1,2}, nearest 2, "
               "d=0.5, max 2 <= 2 -> stop.",
+    ),
+    dict(
+        id="V8",
+        source="hand-derived: the imbalance stop at equality, imb == tolerance (:547-554)",
+        points=[[0, 0], [1, 0], [2, 0], [6, 0], [7, 0], [10, 0]],
+        weights=[1.0, 1.0, 1.0, 2.0, 2.0, 1.0], weight_type="double", constant=False,
+        iter_count=1, tolerance=0.25,
+        expected=[0, 0, 0, 1, 1, 1],
+        trace="S=8. t=5: left {0,1,2}, wl=3, imb=|3-4|/4=0.25 <= 0.25 -> stop (nearest 6, "
+              "6 <= t+d=6 < max 10, so only the imbalance test stops here). With imb < tol the loop "
+              "would go right (3 < 5) to t=7.5 and stop there on max <= t + d with left {0,..,4}.",
+    ),
+    dict(
+        id="V9",
+        source="hand-derived: equal halves move left (wl < S - wl is false), negative tolerance (:564-571)",
+        points=[[0, 0], [1, 0], [2, 0], [6, 0], [7, 0], [10, 0]],
+        weights=[1.0, 1.0, 2.0, 1.0, 1.0, 2.0], weight_type="double", constant=False,
+        iter_count=1, tolerance=-1.0,
+        expected=[0, 0, 0, 1, 1, 1],
+        trace="S=8. t=5: left {0,1,2}, wl=4 = S-wl, imb 0 is not <= -1, 6 < max 10: no stop; 4 < 4 is "
+              "false -> max=5. t=2.5: left {0,1,2}, count 3 == prev -> stop.  Moving right on equality "
+              "would reach t=7.5 and stop with left {0,..,4}.",
     ),
 ]
 

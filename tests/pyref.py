@@ -5,6 +5,16 @@ Written independently of oracle/rcb_oracle.c, from the algorithm's statement
 ``c < t`` directly (the C oracle uses ``c - t < 0``), candidate sets are
 built as Python lists, and the canonical pivot is the smallest ``c >= t``.
 Used to cross-check the oracle on seeded small cases.
+
+``mutant`` plants one mistake in the transcription (tests/test_mutants.py
+checks that the golden vectors and the oracle comparison catch each one):
+  "max_lt"        ``vmax < t + d`` instead of ``<=`` (:556)
+  "side_le"       ``c <= t`` goes left instead of ``c < t`` (:487)
+  "weight_le"     ``wl <= S - wl`` moves right instead of ``<`` (:566)
+  "imb_lt"        ``imb < tol`` instead of ``<=`` (:554)
+  "no_prev_stop"  the ``count == prev`` stop is dropped (:552)
+  "none_min"      the none-branch raises vmin instead of lowering vmax (:540-544)
+  "axis_rev"      the axis cycle runs backwards (:624, :636)
 """
 import numpy as np
 
@@ -19,7 +29,11 @@ def _wrap(kind, v):
     return float(v)
 
 
-def rcb(points, weights, iter_count, tolerance, kind):
+MUTANTS = ("max_lt", "side_le", "weight_le", "imb_lt", "no_prev_stop", "none_min", "axis_rev")
+
+
+def rcb(points, weights, iter_count, tolerance, kind, mutant=None):
+    assert mutant is None or mutant in MUTANTS, mutant
     n = len(points)
     dim = len(points[0]) if n else 0
     c = [[f32(p[k]) for p in points] for k in range(dim)]
@@ -38,17 +52,25 @@ def rcb(points, weights, iter_count, tolerance, kind):
 
     def split(P, a, vmin, vmax, S):
         prev = None
-        while True:
+        # f32 bisection ends within a few hundred candidates; a mutant may not
+        for _ in range(4096):
             t = f32(f32(vmin + vmax) / f32(2.0))
-            left = [i for i in P if c[a][i] < t]
-            right = [i for i in P if not (c[a][i] < t)]
+            if mutant == "side_le":
+                left = [i for i in P if c[a][i] <= t]
+                right = [i for i in P if not (c[a][i] <= t)]
+            else:
+                left = [i for i in P if c[a][i] < t]
+                right = [i for i in P if not (c[a][i] < t)]
             wl = 0 if kind != "double" else 0.0
             for i in left:
                 wl = _wrap(kind, wl + w[i])
             if not right:
                 if prev == len(left):
                     return P, [], S, vmax
-                vmax = t
+                if mutant == "none_min":
+                    vmin = t
+                else:
+                    vmax = t
                 prev = len(left)
                 continue
             m = min(c[a][i] for i in right)
@@ -56,13 +78,18 @@ def rcb(points, weights, iter_count, tolerance, kind):
             with np.errstate(divide="ignore", invalid="ignore"):
                 ideal = to_f64(S) / np.float64(2.0)
                 imb = abs((to_f64(wl) - ideal) / ideal)
-            if len(left) == prev or vmax <= f32(t + d) or imb <= tolerance:
+            stop_prev = len(left) == prev and mutant != "no_prev_stop"
+            stop_max = vmax < f32(t + d) if mutant == "max_lt" else vmax <= f32(t + d)
+            stop_imb = imb < tolerance if mutant == "imb_lt" else imb <= tolerance
+            if stop_prev or stop_max or stop_imb:
                 return left, right, wl, t
             prev = len(left)
-            if wl < _wrap(kind, S - wl):
+            go_right = wl <= _wrap(kind, S - wl) if mutant == "weight_le" else wl < _wrap(kind, S - wl)
+            if go_right:
                 vmin = t
             else:
                 vmax = t
+        raise RuntimeError("the split loop did not end")
 
     def recurse(P, level, hid, a, S, blo, bhi):
         if not P:
@@ -76,7 +103,7 @@ def rcb(points, weights, iter_count, tolerance, kind):
         bl[a] = float(s)
         br = list(blo)
         br[a] = float(s)
-        nxt = (a + 1) % dim
+        nxt = (a - 1) % dim if mutant == "axis_rev" else (a + 1) % dim
         recurse(L, level - 1, (2 * hid + 1) % (1 << 64), nxt, wl, blo, bl)
         recurse(R, level - 1, (2 * hid + 2) % (1 << 64), nxt, _wrap(kind, S - wl), br, bhi)
 
