@@ -17,12 +17,15 @@ WORKLOADS_PATH = os.path.join(HERE, "libcoupe_workloads.so")
 OK, ALLOC, CRASH, BAD_DIMENSION, BAD_TYPE, BIPART_ONLY, LEN_MISMATCH, NOT_FOUND, NEG_VALUES = range(9)
 # enum coupe_type
 COUPE_INT, COUPE_INT64, COUPE_DOUBLE = 0, 1, 2
+# limits of coupe_hip_rcb_device_noalloc (include/coupe_hip.h)
+NOALLOC_MAX_POINTS, NOALLOC_MAX_ITER_COUNT = 1 << 19, 12
 
 # Every symbol the headers declare, in declaration order.
 EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb", "coupe_rib",
-    "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_batch",
+    "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_device_noalloc",
+    "coupe_hip_rcb_batch",
     "coupe_hip_rib_device", "coupe_hip_last_rib", "coupe_hip_grid_rcb",
     "coupe_hip_barycentres", "coupe_hip_parts_load", "coupe_hip_imbalance",
     "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
@@ -130,6 +133,8 @@ def lib():
         L.coupe_hip_rcb_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t,
                                            C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_double]
         L.coupe_hip_rcb_device.restype = C.c_int
+        L.coupe_hip_rcb_device_noalloc.argtypes = L.coupe_hip_rcb_device.argtypes[1:]
+        L.coupe_hip_rcb_device_noalloc.restype = C.c_int
         L.coupe_hip_rib_device.argtypes = L.coupe_hip_rcb_device.argtypes
         L.coupe_hip_rib_device.restype = C.c_int
         L.coupe_hip_last_rib.argtypes = [C.c_void_p, C.POINTER(RibInfo)]

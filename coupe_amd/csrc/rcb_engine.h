@@ -15,17 +15,37 @@ namespace coupe_hip {
 
 struct Rotation;   // rcb_common.h
 
+constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
+
 // A HIP failure inside the engine; entry points map it onto coupe's codes.
 struct HipFailure {
     hipError_t err;
 };
 
-// A grow-only device allocation.
+// A region of device memory that DevBuf::ensure carves buffers from instead
+// of allocating, while it is installed on the calling thread (ArenaScope).
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+
+// Installs an arena for the lifetime of the scope (calling thread only).
+struct ArenaScope {
+    Arena *prev;
+    explicit ArenaScope(Arena *a);
+    ~ArenaScope();
+};
+
+// A grow-only device allocation (or, under an ArenaScope, a carve of the
+// arena: `borrowed` buffers are never freed).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     void ensure(size_t need);   // throws HipFailure on allocation failure
     void release();
+    void forget();              // drops a borrowed buffer without freeing anything
 };
 
 // Opt-in per-launch timing with HIP events on the launch stream.
@@ -95,8 +115,12 @@ struct Context {
     double swept_bytes = 0.0;        // HIST algorithmic bytes of the last call
     coupe_hip_stats stats{};
     Profiler prof;
+    bool borrowed_host = false;      // host_scratch is not the context's own pinned allocation
     uint64_t device_bytes() const;
     void release_all();
+    void forget_all();               // every buffer dropped without freeing (arena contexts)
+    template <typename F>
+    void for_each_buf(F &&f);
 };
 
 enum coupe_err map_hip_error(hipError_t e);

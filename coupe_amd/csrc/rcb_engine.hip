@@ -28,22 +28,43 @@ enum coupe_err map_hip_error(hipError_t e) {
     return (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? COUPE_ERR_ALLOC : COUPE_ERR_CRASH;
 }
 
+static thread_local Arena *t_arena = nullptr;
+
+ArenaScope::ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
 void DevBuf::ensure(size_t need) {
     if (need <= bytes) return;
-    if (ptr) {
-        check(hipFree(ptr));
-        ptr = nullptr;
-        bytes = 0;
+    const size_t sz = need < 256 ? 256 : need;
+    if (t_arena) {
+        // carve: 256-byte aligned, never freed (the old carve stays unused)
+        const size_t off = (t_arena->used + 255) & ~(size_t)255;
+        if (off > t_arena->size || sz > t_arena->size - off) throw HipFailure{hipErrorOutOfMemory};
+        ptr = t_arena->base + off;
+        bytes = sz;
+        borrowed = true;
+        t_arena->used = off + sz;
+        return;
     }
-    size_t sz = need < 256 ? 256 : need;
+    if (ptr && !borrowed) check(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+    borrowed = false;
     check(hipMalloc(&ptr, sz));
     bytes = sz;
 }
 
 void DevBuf::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !borrowed) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
+    borrowed = false;
+}
+
+void DevBuf::forget() {
+    ptr = nullptr;
+    bytes = 0;
+    borrowed = false;
 }
 
 void Profiler::reset() {
@@ -97,7 +118,6 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 }
 
 // Small device counters shared by the kernels of one call.
-constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
 constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
 struct Counters {
     uint32_t n_parts[3];
@@ -964,25 +984,87 @@ enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t
     return rcb_batch(cx, st, d_out, dim, d_points, wkind, d_weights, &n, &iter_count, &tol, 1);
 }
 
+template <typename F>
+void Context::for_each_buf(F &&f) {
+    DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0], &cx[2][1],
+                     &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1],
+                     &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn,
+                     &tile_bins2, &tile4, &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
+                     &heap_split, &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins,
+                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points,
+                     &in_weights, &out_ids};
+    for (DevBuf *d : all) f(*d);
+}
+
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
-    const DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                           &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
-                           &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf, &heap_split,
-                           &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
-    for (const DevBuf *d : all) b += d->bytes;
+    const_cast<Context *>(this)->for_each_buf([&](DevBuf &d) { b += d.bytes; });
     return b;
 }
 
 void Context::release_all() {
-    DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0],
-                     &cx[2][1], &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1], &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn, &tile_bins2, &tile4,
-                     &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf, &heap_split,
-                     &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins, &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points, &in_weights, &out_ids};
-    for (DevBuf *d : all) d->release();
-    if (host_scratch) (void)hipHostFree(host_scratch);
+    for_each_buf([](DevBuf &d) { d.release(); });
+    if (host_scratch && !borrowed_host) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;
     prof.release();
 }
 
+void Context::forget_all() {
+    for_each_buf([](DevBuf &d) { d.forget(); });
+    bins_clean = false;
+}
+
 }  // namespace coupe_hip
+
+// ---------------------------------------------------------------------------
+// coupe_hip_rcb_device_noalloc (include/coupe_hip.h): coupe_hip_rcb_device
+// with every work buffer carved from one static device array per GPU, for
+// callers that cannot allocate.  The host name `scratch` is only a stand-in:
+// its device address comes from hipGetSymbolAddress, once per device, and
+// only that address reaches the kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[COUPE_HIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+
+struct NoAllocDevice {
+    std::mutex lock;                  // calls on one device run one at a time
+    char *scratch_ptr = nullptr;      // device address of `scratch` on this device
+    coupe_hip::Context cx;            // hints and bookkeeping; every buffer is a carve of scratch
+    alignas(256) char host[coupe_hip::kHostScratch];   // readbacks (pageable; no pinned allocation)
+};
+
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+}  // namespace
+
+extern "C" enum coupe_err coupe_hip_rcb_device_noalloc(void *stream, uint64_t *d_partition, uintptr_t n,
+                                                       uintptr_t dimension, const double *d_points,
+                                                       enum coupe_type weight_type, const void *d_weights,
+                                                       uintptr_t iter_count, double tolerance) {
+    if (n > COUPE_HIP_NOALLOC_MAX_POINTS || iter_count > COUPE_HIP_NOALLOC_MAX_ITER_COUNT) return COUPE_ERR_ALLOC;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return COUPE_ERR_CRASH;
+    NoAllocDevice &D = g_noalloc[dev];
+    std::lock_guard<std::mutex> g(D.lock);
+    if (!D.scratch_ptr) {   // first call on this device, which is current
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return COUPE_ERR_CRASH;
+        }
+        D.scratch_ptr = (char *)p;
+        D.cx.device = dev;
+        coupe_hip::configure_from_env(D.cx);
+        D.cx.host_scratch = D.host;
+        D.cx.borrowed_host = true;
+    }
+    // the previous call's carves are dropped; this call carves from offset 0
+    D.cx.forget_all();
+    coupe_hip::Arena arena{D.scratch_ptr, (size_t)COUPE_HIP_NOALLOC_SCRATCH_BYTES, 0};
+    coupe_hip::ArenaScope scope(&arena);
+    return coupe_hip::rcb_device(D.cx, (hipStream_t)stream, d_partition, n, dimension, d_points, weight_type,
+                                 d_weights, iter_count, tolerance);
+}

@@ -45,6 +45,29 @@ enum coupe_err coupe_hip_rcb_device(coupe_hip_ctx *ctx, void *stream,
 		uintptr_t iter_count, double tolerance);
 
 /*
+ * coupe_hip_rcb_device for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in coupe.
+ *
+ * Same arguments, semantics, ids and errors as coupe_hip_rcb_device on the
+ * current device, without a context: every work buffer is a carve of one
+ * static device array of COUPE_HIP_NOALLOC_SCRATCH_BYTES per GPU, which the
+ * GPU holds from the library's load until the process ends.  The call
+ * allocates no device or pinned memory (its host bookkeeping uses a few
+ * small heap vectors).  Limits: n <= COUPE_HIP_NOALLOC_MAX_POINTS and
+ * iter_count <= COUPE_HIP_NOALLOC_MAX_ITER_COUNT, else COUPE_ERR_ALLOC with
+ * nothing written.  Calls on one GPU share its array: they are serialised
+ * by a lock, and work that overlaps a call on that GPU from outside the
+ * lock (another process, device code racing the call) races on the array.
+ */
+#define COUPE_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define COUPE_HIP_NOALLOC_MAX_POINTS (1ull << 19)
+#define COUPE_HIP_NOALLOC_MAX_ITER_COUNT 12
+enum coupe_err coupe_hip_rcb_device_noalloc(void *stream, uint64_t *d_partition, uintptr_t n,
+		uintptr_t dimension, const double *d_points, enum coupe_type weight_type, const void *d_weights,
+		uintptr_t iter_count, double tolerance);
+
+/*
  * Batched RCB: many independent instances in one call.
  *
  * Instance q holds instances[q].n points; the instances are packed one after
