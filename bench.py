@@ -151,6 +151,16 @@ def main():
         del src, dst
         log(f"copy probe {args.copy_probe_mib} MiB: {ms:.3f} ms, {copy_probe['GBps_read_plus_write']} GB/s")
 
+    # A first call on a fresh context (buffers allocated, no hints).
+    cold = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_cold = time.perf_counter()
+    cold[0].record(stream)
+    step()
+    cold[1].record(stream)
+    torch.cuda.synchronize()
+    cold_wall_ms = (time.perf_counter() - t_cold) * 1e3
+    cold_ms = cold[0].elapsed_time(cold[1])
+    log(f"cold call (fresh context): {cold_ms:.3f} ms on the stream, {cold_wall_ms:.1f} ms wall")
     for i in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -175,6 +185,31 @@ def main():
     step_ms = sorted(a.elapsed_time(b) for a, b in per_step)
     stats = ctx.stats()
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, median {step_ms[len(step_ms) // 2]:.3f} ms")
+
+    # The same shape with other data (another seed) on the warm context: its
+    # hints come from the benchmark input, not from this input.
+    fresh = None
+    if args.workload in ("jgrid512", "gauss3d"):
+        fp, fw = (workloads.jgrid(512, 7, True) if args.workload == "jgrid512"
+                  else workloads.gauss_mix(pts.shape[0], 7, 0.05))
+        d_fp, d_fw = torch.from_numpy(fp).to(dev), torch.from_numpy(fw).to(dev)
+        f_out = torch.zeros(n, dtype=torch.int64, device=dev)
+        torch.cuda.synchronize()
+        fe = []
+        for _ in range(2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            rcb.partition(f_out, (d_fp, d_fw), context=ctx, stream=stream.cuda_stream)
+            e1.record(stream)
+            fe.append((e0, e1))
+            torch.cuda.synchronize()
+        fresh = {"seed": 7, "first_ms": round(fe[0][0].elapsed_time(fe[0][1]), 4),
+                 "second_ms": round(fe[1][0].elapsed_time(fe[1][1]), 4),
+                 "reruns": ctx.stats()["reruns"]}
+        del d_fp, d_fw, f_out, fp, fw
+        step()   # back to the benchmark input's hints
+        torch.cuda.synchronize()
+        log(f"fresh data (seed 7), warm context: {fresh['first_ms']} ms, then {fresh['second_ms']} ms")
 
     # Per-kernel timing: the same steps with events around every launch.
     ctx.set_profiling(True)
@@ -210,6 +245,13 @@ def main():
                    key=lambda k: kernels[k]["ms_per_step"])
     dk = kernels[dominant]
     stream_copy = stream_probe["copy_GBps"] if stream_probe else None
+    # The same launches priced on the survey's basis: the partition moves two
+    # levels per launch, each (8D + 24) B/pt with coupe's working types (f32
+    # coordinates, f64 weight, u64 index read and written).
+    s8d_bytes = None
+    if dominant == "partition" and stats["weight_mode"] == 0:
+        s8d_bytes = 2 * (8 * dim + 24) * n
+    s8d_gbs = s8d_bytes / (dk["avg_launch_ms"] * 1e-3) / 1e9 if s8d_bytes else None
     roofline = {
         "kernel": dominant,
         "bound": "hbm",
@@ -218,6 +260,12 @@ def main():
         "unit": "GB/s",
         "frac": round(dk["GBps"] / HBM_PEAK_GBS, 4),
         "traffic": measured_traffic(args.workload, dominant),
+        "basis": "engine: the bytes the move must read and write in this layout (f32 coordinates, 16-bit "
+                 "weights, no index stream; two levels per launch), checked against FETCH_SIZE/WRITE_SIZE "
+                 "(traffic); frac_s8d prices the same launches at the survey's (8D+24) B/pt per level",
+        "frac_engine": round(dk["GBps"] / HBM_PEAK_GBS, 4),
+        "achieved_s8d": round(s8d_gbs, 1) if s8d_gbs else None,
+        "frac_s8d": round(s8d_gbs / HBM_PEAK_GBS, 4) if s8d_gbs else None,
         "avg_launch_ms": round(dk["avg_launch_ms"], 4),
         "algo_bytes_per_launch": dk["algo_bytes_per_launch"],
         "stream_copy_GBps": stream_copy,
@@ -290,6 +338,10 @@ def main():
         "stream": stream_probe,
         "step_ms": {"median": round(step_ms[len(step_ms) // 2], 4), "min": round(step_ms[0], 4),
                     "max": round(step_ms[-1], 4), "profiled_median": round(sorted(prof_ms)[len(prof_ms) // 2], 4)},
+        "cold_ms": round(cold_ms, 4),
+        "cold_wall_ms": round(cold_wall_ms, 2),
+        "fresh_data_ms": fresh["first_ms"] if fresh else None,
+        "fresh_data": fresh,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
@@ -529,8 +581,14 @@ def bench_batched(torch, coupe_amd, workloads, count=4096, seed=3):
             "ids_equal_to_per_instance": bool(same)}
 
 
-def bench_served(torch, coupe_amd, workloads, n_requests, seed=4, rate=200.0, resident=8, pool=24):
-    """Poisson request stream through the request server."""
+def bench_served(torch, coupe_amd, workloads, n_requests, seed=4, rate=200.0, resident=8, pool=24,
+                 sat_rates=(400.0, 800.0, 1600.0, 3200.0), sat_requests=1500, plain_requests=120):
+    """Poisson request stream through the request server: new requests from
+    pinned host memory, and repartitions of meshes resident in HBM (mesh
+    handles) after every point moved by N(0, 0.01) and 5% of the weights were
+    resampled, on the device.  Then the same stream at rising rates until
+    p99 breaks (served capacity), and the same kind of requests as one plain
+    coupe_rcb call at a time (the per-instance baseline)."""
     import threading
 
     rng = np.random.default_rng(seed)
@@ -541,102 +599,140 @@ def bench_served(torch, coupe_amd, workloads, n_requests, seed=4, rate=200.0, re
     host = []
     for k, m in enumerate(sizes):
         p, w = workloads.uniform(int(m), 3, seed * 1000 + k, 100)
-        hp = torch.from_numpy(p).pin_memory()
-        hw = torch.from_numpy(w).pin_memory()
-        host.append((hp, hw, torch.zeros(int(m), dtype=torch.int64).pin_memory()))
-    # Meshes resident in HBM for re-partition requests.
+        host.append((torch.from_numpy(p).pin_memory(), torch.from_numpy(w).pin_memory(),
+                     torch.zeros(int(m), dtype=torch.int64).pin_memory()))
+    srv = coupe_amd.Server(device=0, slots=4, memory_budget=64 << 30, max_batch_points=1 << 26,
+                           max_batch_requests=64)
+    # Meshes resident in HBM: the server's working copies, moved on the device.
     rsizes = np.round(10 ** rng.uniform(4, 7, resident)).astype(np.int64)
     res = []
     for k, m in enumerate(rsizes):
         p, w = workloads.uniform(int(m), 3, seed * 2000 + k, 100)
-        res.append([torch.from_numpy(p).to(dev), torch.from_numpy(w).to(dev),
-                    torch.zeros(int(m), dtype=torch.int64, device=dev), None])
+        dp, dw = torch.from_numpy(p).to(dev), torch.from_numpy(w).to(dev)
+        res.append({"mesh": srv.create_mesh(dp, dw), "p": dp, "w": dw,
+                    "out": torch.zeros(int(m), dtype=torch.int64).pin_memory()})
     torch.cuda.synchronize()
     log(f"served: pools ready in {time.time() - t0:.1f}s")
-    srv = coupe_amd.Server(device=0, slots=4, memory_budget=64 << 30, max_batch_points=1 << 26,
-                           max_batch_requests=64)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
-    lat, pts_done, tickets = [], [], []
-    lock = threading.Lock()
-    done_evt = threading.Event()
 
-    def waiter():
-        i = 0
-        while True:
-            with lock:
-                if i < len(tickets):
-                    t, npts = tickets[i]
-                else:
-                    t = None
-            if t is None:
-                if done_evt.is_set():
-                    with lock:
-                        if i >= len(tickets):
-                            return
-                    continue
-                time.sleep(0.0005)
-                continue
-            st = srv.wait(t)
-            lat.append(st)
-            pts_done.append(npts)
-            i += 1
+    def stream(rate, count, srng):
+        lat, tickets = [], []
+        lock = threading.Lock()
+        done_evt = threading.Event()
 
-    th = threading.Thread(target=waiter)
-    th.start()
-    start = time.perf_counter()
-    next_t = start
-    n_repart = 0
-    for r in range(n_requests):
-        next_t += rng.exponential(1.0 / rate)
-        iters = int(6 + np.floor(7 * rng.random()))
-        repart = rng.random() < 0.3
-        k = int(rng.integers(0, resident if repart else pool))
-        now = time.perf_counter()
-        if next_t > now:
-            time.sleep(next_t - now)
-        if repart:
-            mesh = res[k]
-            if mesh[3] is not None:        # the mesh moves only after its last request is done
-                while True:
-                    with lock:
-                        finished = len(lat) > mesh[3]
-                    if finished:
-                        break
+        def waiter():
+            i = 0
+            while True:
+                with lock:
+                    item = tickets[i] if i < len(tickets) else None
+                if item is None:
+                    if done_evt.is_set():
+                        with lock:
+                            if i >= len(tickets):
+                                return
+                        continue
                     time.sleep(0.0002)
-            dp, dw = mesh[0], mesh[1]
-            dp.add_(torch.randn(dp.shape, generator=gen, device=dev, dtype=dp.dtype), alpha=0.01)
-            sel = torch.rand(dw.shape[0], generator=gen, device=dev) < 0.05
-            dw[sel] = torch.floor(torch.rand(int(sel.sum()), generator=gen, device=dev,
-                                             dtype=dw.dtype) * 100) + 1
-            torch.cuda.synchronize()
-            with lock:
-                t = srv.submit(mesh[2], dp, dw, iters, 0.01)
-                mesh[3] = len(tickets)
-                tickets.append((t, dp.shape[0]))
-            n_repart += 1
-        else:
-            hp, hw, ho = host[k]
-            with lock:
+                    continue
+                st = srv.wait(item[0])
+                st["points"] = item[1]
+                lat.append(st)
+                i += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        next_t = start
+        n_repart = 0
+        for r in range(count):
+            next_t += srng.exponential(1.0 / rate)
+            iters = int(6 + np.floor(7 * srng.random()))
+            repart = srng.random() < 0.3
+            k = int(srng.integers(0, resident if repart else pool))
+            now = time.perf_counter()
+            if next_t > now:
+                time.sleep(next_t - now)
+            if repart:
+                m = res[k]
+                # the mesh moves (on the device), then its working copy is updated
+                # (after its queued requests) and it is repartitioned
+                m["p"].add_(torch.randn(m["p"].shape, generator=gen, device=dev, dtype=m["p"].dtype), alpha=0.01)
+                sel = torch.rand(m["w"].shape[0], generator=gen, device=dev) < 0.05
+                m["w"][sel] = torch.floor(torch.rand(int(sel.sum()), generator=gen, device=dev,
+                                                     dtype=m["w"].dtype) * 100) + 1
+                m["mesh"].update(points=m["p"], weights=m["w"])
+                t = srv.submit_mesh(m["mesh"], m["out"].numpy(), iters, 0.01)
+                n_repart += 1
+                npts = m["p"].shape[0]
+            else:
+                hp, hw, ho = host[k]
                 t = srv.submit(ho.numpy(), hp.numpy(), hw.numpy(), iters, 0.01)
-                tickets.append((t, hp.shape[0]))
-    done_evt.set()
-    th.join()
-    wall = time.perf_counter() - start
+                npts = hp.shape[0]
+            with lock:
+                tickets.append((t, npts))
+        done_evt.set()
+        th.join()
+        wall = time.perf_counter() - start
+        tot = np.array([x["total_ms"] for x in lat])
+        run = np.array([x["run_ms"] for x in lat])
+        bsz = np.array([x["batch_requests"] for x in lat])
+        pts_total = float(np.sum([x["points"] for x in lat]))
+        return {"rate": rate, "requests": count, "repartitions": n_repart,
+                "p50_ms": round(float(np.percentile(tot, 50)), 3), "p99_ms": round(float(np.percentile(tot, 99)), 3),
+                "p50_run_ms": round(float(np.percentile(run, 50)), 3),
+                "p99_run_ms": round(float(np.percentile(run, 99)), 3),
+                "mean_batch_requests": round(float(bsz.mean()), 2),
+                "achieved_req_s": round(count / wall, 1),
+                "Mpts_s": round(pts_total / wall / 1e6, 1), "wall_s": round(wall, 2)}
+
+    main = stream(rate, n_requests, np.random.default_rng(seed))
+    log(f"served at {rate:.0f} req/s: p50 {main['p50_ms']} ms, p99 {main['p99_ms']} ms, {main['Mpts_s']} Mpts/s")
+    # Saturation: rising rates until the server no longer keeps up (achieved
+    # rate under 90% of the offered one) or p99 exceeds 4x the nominal p99.
+    sweep, capacity = [], None
+    for r in sat_rates:
+        x = stream(r, sat_requests, np.random.default_rng(seed + int(r)))
+        sweep.append(x)
+        log(f"served at {r:.0f} req/s offered: {x['achieved_req_s']} req/s, p99 {x['p99_ms']} ms, "
+            f"{x['Mpts_s']} Mpts/s, mean batch {x['mean_batch_requests']}")
+        ok = x["achieved_req_s"] >= 0.9 * r and x["p99_ms"] <= 4 * main["p99_ms"]
+        if ok:
+            capacity = x
+        else:
+            break
+    peak = max([main] + sweep, key=lambda x: x["Mpts_s"])
+    st = srv.stats()
+    for m in res:
+        m["mesh"].destroy()
     srv.close()
-    tot = np.array([s["total_ms"] for s in lat])
-    run = np.array([s["run_ms"] for s in lat])
-    bsz = np.array([s["batch_requests"] for s in lat])
-    return {"workload": f"Poisson {rate:.0f} req/s, sizes log-uniform in [1e4, 1e7] 3D points, weights 1..100, "
-                        f"iter_count 6..12, tolerance 0.01, 30% re-partitions of {resident} HBM-resident meshes "
-                        f"(N(0,0.01) drift, 5% weights resampled), seed {seed}",
-            "requests": n_requests, "repartitions": n_repart,
-            "p50_ms": round(float(np.percentile(tot, 50)), 3), "p99_ms": round(float(np.percentile(tot, 99)), 3),
-            "p50_run_ms": round(float(np.percentile(run, 50)), 3), "p99_run_ms": round(float(np.percentile(run, 99)), 3),
-            "mean_batch_requests": round(float(bsz.mean()), 2),
-            "Mpts_s": round(float(np.sum(pts_done)) / wall / 1e6, 1),
-            "offered_Mpts_s": round(float(np.sum([p for _, p in tickets])) / wall / 1e6, 1),
-            "wall_s": round(wall, 2)}
+    # Per-instance baseline: the same kind of requests, one coupe_rcb call at a time.
+    prng = np.random.default_rng(seed + 99)
+    pts_plain = 0
+    t1 = time.perf_counter()
+    for _ in range(plain_requests):
+        hp, hw, ho = host[int(prng.integers(0, pool))]
+        coupe_amd.Rcb(int(6 + np.floor(7 * prng.random())), 0.01).partition(ho.numpy().view(np.uint64),
+                                                                             (hp.numpy(), hw.numpy()))
+        pts_plain += hp.shape[0]
+    plain_s = time.perf_counter() - t1
+    plain = {"requests": plain_requests, "Mpts_s": round(pts_plain / plain_s / 1e6, 1),
+             "req_s": round(plain_requests / plain_s, 1)}
+    log(f"plain coupe_rcb one request at a time: {plain['Mpts_s']} Mpts/s")
+    out = dict(main)
+    out.update({
+        "workload": f"Poisson {rate:.0f} req/s, sizes log-uniform in [1e4, 1e7] 3D points, weights 1..100, "
+                    f"iter_count 6..12, tolerance 0.01, new requests from {pool} pinned host meshes, 30% "
+                    f"re-partitions of {resident} meshes resident in HBM through mesh handles (N(0,0.01) drift "
+                    f"and 5% of the weights resampled on the device, then the working copy updated), seed {seed}",
+        "saturation": sweep,
+        "capacity": ({"req_s": capacity["achieved_req_s"], "Mpts_s": capacity["Mpts_s"], "p99_ms": capacity["p99_ms"]}
+                     if capacity else None),
+        "peak_Mpts_s": peak["Mpts_s"],
+        "per_instance_plain": plain,
+        "peak_vs_per_instance_plain": round(peak["Mpts_s"] / plain["Mpts_s"], 2) if plain["Mpts_s"] else None,
+        "peak_device_bytes": st["peak_device_bytes"], "memory_budget": st["memory_budget"],
+    })
+    return out
 
 
 if __name__ == "__main__":

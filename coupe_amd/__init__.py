@@ -28,7 +28,7 @@ from . import _native
 from ._native import strerror
 
 __all__ = ["Rcb", "Rib", "Grid", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
-           "partition_batch", "Server"]
+           "partition_batch", "partition_batch_mixed", "Server", "Mesh"]
 
 
 class Error(Exception):
@@ -291,14 +291,57 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
     return None
 
 
+def partition_batch_mixed(part_ids, points, weights, instances, *, stream=None, context: DeviceContext = None):
+    """Batched RCB of instances of dimension 2 and 3 in one call
+    (coupe_hip_rcb_batch_mixed, include/coupe_hip.h).
+
+    ``instances`` is a sequence of ``(n, dimension, iter_count, tolerance)``.
+    ``points`` is a flat float64 GPU tensor: instance q's n * dimension
+    coordinates follow instance q-1's; ``weights`` and ``part_ids`` hold one
+    entry per point in the same order.  Instance q's ids are those
+    ``Rcb(iter_count, tolerance)`` gives it alone.
+    """
+    import torch
+
+    specs = [(int(n), int(d), int(it), float(tol)) for n, d, it, tol in instances]
+    total = sum(n for n, _, _, _ in specs)
+    coords = sum(n * d for n, d, _, _ in specs)
+    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
+    for name, t, want in (("part_ids", part_ids, total), ("weights", weights, total)):
+        if t.shape[0] != want:
+            raise InputLenMismatch(want, t.shape[0])
+        if not t.is_contiguous():
+            raise TypeError(f"{name} must be contiguous")
+    if points.dtype != torch.float64 or not points.is_contiguous() or points.numel() != coords:
+        raise TypeError(f"points must be a contiguous float64 tensor of {coords} coordinates")
+    arr = (_native.InstanceND * max(len(specs), 1))()
+    for q, (n, d, it, tol) in enumerate(specs):
+        arr[q].n, arr[q].dimension, arr[q].iter_count, arr[q].tolerance = n, d, it, tol
+    ctx = context or _context_for(dev)
+    if ctx.device != dev:
+        raise TypeError(f"the context is on device {ctx.device}, the tensors on device {dev}")
+    st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
+    err = _native.lib().coupe_hip_rcb_batch_mixed(ctx.handle, C.c_void_p(st), part_ids.data_ptr(),
+                                                  points.data_ptr(), _weight_type(weights.dtype),
+                                                  weights.data_ptr(), arr, len(specs))
+    if err != _native.OK:
+        raise CoupeError(err)
+    return None
+
+
 class Server:
-    """Request server with continuous batching (coupe_hip_server_*, include/coupe_hip.h).
+    """Request server with continuous batching and resident meshes
+    (coupe_hip_server_*, coupe_hip_mesh_*, include/coupe_hip.h).
 
     ``submit(part_ids, points, weights, iter_count, tolerance)`` takes numpy
-    arrays (host memory) or torch tensors on the GPU (a mesh kept resident in
-    HBM) and returns a ticket; ``wait(ticket)`` blocks until the ids are
-    written and returns the request's latency figures.  The arrays are kept
-    alive until the request is waited for.
+    arrays (host memory) or torch tensors on the GPU and returns a ticket;
+    ``wait(ticket)`` blocks until the ids are written and returns the
+    request's latency figures; ``poll`` and ``cancel`` look at or withdraw a
+    queued request.  ``create_mesh(points, weights)`` keeps a mesh resident in
+    HBM; ``submit_mesh(mesh, part_ids, iter_count, tolerance)`` partitions it
+    in place.  The arrays are kept alive until the request is waited for.
+    Device tensors are read after the work queued on the current torch
+    stream at submit time.
     """
 
     def __init__(self, device: int = 0, slots: int = 4, memory_budget: int = 64 << 30,
@@ -307,14 +350,24 @@ class Server:
         self._s = _native.lib().coupe_hip_server_create(C.byref(cfg))
         if not self._s:
             raise RuntimeError(f"cannot start a server on device {device}")
+        self.device = device
         self._keep = {}
+
+    @staticmethod
+    def _stream_of(t):
+        import torch
+        return torch.cuda.current_stream(t.device).cuda_stream
 
     def submit(self, part_ids, points, weights, iter_count: int, tolerance: float) -> int:
         dev_in = _is_device_tensor(points)
         dev_out = _is_device_tensor(part_ids)
         if dev_in != _is_device_tensor(weights):
             raise TypeError("points and weights must both be host arrays or both device tensors")
-        if not dev_in:
+        if dev_in:
+            _common_device(points=points, weights=weights)
+            if not points.is_contiguous() or not weights.is_contiguous():
+                raise TypeError("points and weights must be contiguous")
+        else:
             points = np.ascontiguousarray(points, dtype=np.float64)
             weights = np.ascontiguousarray(weights)
         n = points.shape[0]
@@ -324,20 +377,66 @@ class Server:
             raise InputLenMismatch(part_ids.shape[0], n)
         ptr = (lambda a: a.data_ptr()) if dev_in else (lambda a: a.ctypes.data)
         optr = part_ids.data_ptr() if dev_out else part_ids.ctypes.data
+        stream = self._stream_of(points) if dev_in else None
         req = _native.Request(ptr(points), ptr(weights), _weight_type(weights.dtype), n,
                               points.shape[1] if points.ndim == 2 else 0, int(iter_count), float(tolerance),
-                              optr, 1 if dev_in else 0, 1 if dev_out else 0)
+                              optr, 1 if dev_in else 0, 1 if dev_out else 0, stream, 0)
+        return self._submit(req, (part_ids, points, weights))
+
+    def _submit(self, req, keep) -> int:
         t = C.c_uint64()
         err = _native.lib().coupe_hip_server_submit(self._s, C.byref(req), C.byref(t))
         if err != _native.OK:
             raise CoupeError(err)
-        self._keep[t.value] = (part_ids, points, weights)
+        self._keep[t.value] = keep
         return t.value
+
+    def create_mesh(self, points, weights) -> "Mesh":
+        """A mesh resident in HBM (its working copy: f32 coordinates and the
+        weights), from numpy arrays or GPU tensors."""
+        return Mesh(self, points, weights)
+
+    def submit_mesh(self, mesh: "Mesh", part_ids, iter_count: int, tolerance: float) -> int:
+        """Partitions a resident mesh; ``part_ids`` (n ids, numpy or GPU
+        tensor, or None to keep them on the mesh only) receives its ids."""
+        if part_ids is not None and part_ids.shape[0] != mesh.n:
+            raise InputLenMismatch(mesh.n, part_ids.shape[0])
+        dev_out = part_ids is not None and _is_device_tensor(part_ids)
+        optr = None if part_ids is None else (part_ids.data_ptr() if dev_out else part_ids.ctypes.data)
+        stream = self._stream_of(part_ids) if dev_out else None
+        req = _native.Request(None, None, 0, 0, 0, int(iter_count), float(tolerance), optr, 0,
+                              1 if dev_out else 0, stream, mesh.handle)
+        return self._submit(req, (part_ids,))
 
     def wait(self, ticket: int) -> dict:
         st = _native.RequestStats()
         err = _native.lib().coupe_hip_server_wait(self._s, int(ticket), C.byref(st))
         self._keep.pop(ticket, None)
+        if err != _native.OK:
+            raise CoupeError(err)
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def poll(self, ticket: int) -> bool:
+        done = C.c_int()
+        err = _native.lib().coupe_hip_server_poll(self._s, int(ticket), C.byref(done))
+        if err != _native.OK:
+            raise CoupeError(err)
+        return bool(done.value)
+
+    def cancel(self, ticket: int) -> bool:
+        """True when the request was still queued and is withdrawn (its
+        ticket is then unknown); False when it started (wait for it)."""
+        c = C.c_int()
+        err = _native.lib().coupe_hip_server_cancel(self._s, int(ticket), C.byref(c))
+        if err != _native.OK:
+            raise CoupeError(err)
+        if c.value:
+            self._keep.pop(ticket, None)
+        return bool(c.value)
+
+    def stats(self) -> dict:
+        st = _native.ServerStats()
+        err = _native.lib().coupe_hip_server_stats(self._s, C.byref(st))
         if err != _native.OK:
             raise CoupeError(err)
         return {k: getattr(st, k) for k, _ in st._fields_}
@@ -353,6 +452,92 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Mesh:
+    """A mesh resident in HBM on a Server (coupe_hip_mesh_*)."""
+
+    def __init__(self, server: Server, points, weights):
+        dev_in = _is_device_tensor(points)
+        if dev_in != _is_device_tensor(weights):
+            raise TypeError("points and weights must both be host arrays or both device tensors")
+        if dev_in:
+            _common_device(points=points, weights=weights)
+            points, weights = points.contiguous(), weights.contiguous()
+            ptr = lambda a: a.data_ptr()
+            stream = Server._stream_of(points)
+        else:
+            points = np.ascontiguousarray(points, dtype=np.float64)
+            weights = np.ascontiguousarray(weights)
+            ptr = lambda a: a.ctypes.data
+            stream = None
+        n = points.shape[0]
+        if weights.shape[0] != n:
+            raise InputLenMismatch(n, weights.shape[0])
+        self.server, self.n = server, n
+        self.dimension = points.shape[1] if points.ndim == 2 else 0
+        self.weight_type = _weight_type(weights.dtype)
+        h = C.c_uint64()
+        err = _native.lib().coupe_hip_mesh_create(server._s, ptr(points), ptr(weights), self.weight_type, n,
+                                                  self.dimension, 1 if dev_in else 0, stream, C.byref(h))
+        if err != _native.OK:
+            raise CoupeError(err)
+        self.handle = h.value
+
+    def update(self, points=None, weights=None, first: int = 0):
+        """Replaces points [first, first + len) and/or their weights (numpy or
+        GPU tensors, after the mesh's queued requests)."""
+        ref = points if points is not None else weights
+        if ref is None:
+            return
+        dev_in = _is_device_tensor(ref)
+        count = ref.shape[0]
+        for name, a in (("points", points), ("weights", weights)):
+            if a is not None and a.shape[0] != count:
+                raise InputLenMismatch(count, a.shape[0])
+            if a is not None and _is_device_tensor(a) != dev_in:
+                raise TypeError("points and weights must both be host arrays or both device tensors")
+        if dev_in:
+            conv = lambda a: None if a is None else a.contiguous()
+            ptr = lambda a: None if a is None else a.data_ptr()
+            stream = Server._stream_of(ref)
+        else:
+            conv = lambda a: None if a is None else np.ascontiguousarray(a)
+            ptr = lambda a: None if a is None else a.ctypes.data
+            stream = None
+        points, weights = conv(points), conv(weights)
+        if points is not None and (points.dtype != (np.float64 if not dev_in else points.dtype)
+                                   or points.shape[1] != self.dimension):
+            raise TypeError(f"points must be float64 of shape (k, {self.dimension})")
+        if weights is not None and _weight_type(weights.dtype) != self.weight_type:
+            raise TypeError("weights must keep the mesh's weight type")
+        err = _native.lib().coupe_hip_mesh_update(self.server._s, self.handle, int(first), count, ptr(points),
+                                                  ptr(weights), 1 if dev_in else 0, stream)
+        if err != _native.OK:
+            raise CoupeError(err)
+
+    def parts(self):
+        """(perm, offsets): the point indices of its last partition grouped by
+        part (numpy uint32 / uint64)."""
+        k = C.c_uint64()
+        L = _native.lib()
+        err = L.coupe_hip_mesh_parts(self.server._s, self.handle, None, None, 0, C.byref(k), 0)
+        if err != _native.OK:
+            raise CoupeError(err)
+        perm = np.empty(self.n, dtype=np.uint32)
+        off = np.empty(k.value + 1, dtype=np.uint64)
+        err = L.coupe_hip_mesh_parts(self.server._s, self.handle, perm.ctypes.data, off.ctypes.data, off.shape[0],
+                                     C.byref(k), 0)
+        if err != _native.OK:
+            raise CoupeError(err)
+        return perm, off
+
+    def destroy(self):
+        if self.handle:
+            err = _native.lib().coupe_hip_mesh_destroy(self.server._s, self.handle)
+            self.handle = 0
+            if err != _native.OK:
+                raise CoupeError(err)
 
 
 class Grid:

@@ -25,10 +25,12 @@ EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb", "coupe_rib",
     "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_device_noalloc",
-    "coupe_hip_rcb_batch",
+    "coupe_hip_rcb_batch", "coupe_hip_rcb_batch_mixed",
     "coupe_hip_rib_device", "coupe_hip_last_rib", "coupe_hip_grid_rcb",
     "coupe_hip_barycentres", "coupe_hip_parts_load", "coupe_hip_imbalance",
     "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
+    "coupe_hip_server_poll", "coupe_hip_server_cancel", "coupe_hip_server_stats", "coupe_hip_mesh_create",
+    "coupe_hip_mesh_update", "coupe_hip_mesh_parts", "coupe_hip_mesh_destroy",
     "coupe_hip_last_stats", "coupe_hip_set_profiling", "coupe_hip_kernel_times",
 ]
 
@@ -42,12 +44,18 @@ class Request(C.Structure):
     _fields_ = [("points", C.c_void_p), ("weights", C.c_void_p), ("weight_type", C.c_int),
                 ("n", C.c_size_t), ("dimension", C.c_size_t), ("iter_count", C.c_size_t),
                 ("tolerance", C.c_double), ("partition", C.c_void_p), ("device_input", C.c_int),
-                ("device_output", C.c_int)]
+                ("device_output", C.c_int), ("stream", C.c_void_p), ("mesh", C.c_uint64)]
 
 
 class RequestStats(C.Structure):
     _fields_ = [("queue_ms", C.c_double), ("run_ms", C.c_double), ("total_ms", C.c_double),
                 ("batch_requests", C.c_uint32), ("batch_points", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("device_bytes", C.c_uint64), ("mesh_bytes", C.c_uint64),
+                ("peak_device_bytes", C.c_uint64), ("queued", C.c_uint32), ("running", C.c_uint32),
+                ("meshes", C.c_uint32), ("completed", C.c_uint64), ("cancelled", C.c_uint64)]
 
 
 class RibInfo(C.Structure):
@@ -64,6 +72,10 @@ class RibInfo(C.Structure):
 
 class Instance(C.Structure):
     _fields_ = [("n", C.c_size_t), ("iter_count", C.c_size_t), ("tolerance", C.c_double)]
+
+
+class InstanceND(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("dimension", C.c_size_t), ("iter_count", C.c_size_t), ("tolerance", C.c_double)]
 
 KERNELS = ["prologue", "hist", "decide", "partition", "ids", "other", "moments"]
 
@@ -89,6 +101,7 @@ class Stats(C.Structure):
         ("first_read", C.c_uint64),
         ("forwarded", C.c_uint32),
         ("full_first", C.c_uint32),
+        ("graph", C.c_uint32),
     ]
 
     def as_dict(self):
@@ -142,6 +155,9 @@ def lib():
         L.coupe_hip_rcb_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                           C.c_void_p, C.POINTER(Instance), C.c_size_t]
         L.coupe_hip_rcb_batch.restype = C.c_int
+        L.coupe_hip_rcb_batch_mixed.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                C.POINTER(InstanceND), C.c_size_t]
+        L.coupe_hip_rcb_batch_mixed.restype = C.c_int
         L.coupe_hip_grid_rcb.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int,
                                          C.c_void_p, C.c_size_t, C.c_uint32, C.c_void_p]
         L.coupe_hip_grid_rcb.restype = C.c_int
@@ -162,6 +178,23 @@ def lib():
         L.coupe_hip_server_submit.restype = C.c_int
         L.coupe_hip_server_wait.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(RequestStats)]
         L.coupe_hip_server_wait.restype = C.c_int
+        L.coupe_hip_server_poll.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]
+        L.coupe_hip_server_poll.restype = C.c_int
+        L.coupe_hip_server_cancel.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]
+        L.coupe_hip_server_cancel.restype = C.c_int
+        L.coupe_hip_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        L.coupe_hip_server_stats.restype = C.c_int
+        L.coupe_hip_mesh_create.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_size_t, C.c_size_t,
+                                            C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]
+        L.coupe_hip_mesh_create.restype = C.c_int
+        L.coupe_hip_mesh_update.argtypes = [C.c_void_p, C.c_uint64, C.c_size_t, C.c_size_t, C.c_void_p, C.c_void_p,
+                                            C.c_int, C.c_void_p]
+        L.coupe_hip_mesh_update.restype = C.c_int
+        L.coupe_hip_mesh_parts.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_size_t,
+                                           C.POINTER(C.c_uint64), C.c_int]
+        L.coupe_hip_mesh_parts.restype = C.c_int
+        L.coupe_hip_mesh_destroy.argtypes = [C.c_void_p, C.c_uint64]
+        L.coupe_hip_mesh_destroy.restype = C.c_int
         L.coupe_hip_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
         L.coupe_hip_last_stats.restype = C.c_int
         L.coupe_hip_set_profiling.argtypes = [C.c_void_p, C.c_int]

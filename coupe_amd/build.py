@@ -24,7 +24,8 @@ LIB = os.path.join(HERE, "libcoupe_hip.so")
 WORKLOADS = os.path.join(HERE, "libcoupe_workloads.so")
 PROBE = os.path.join(HERE, "libcoupe_probe.so")
 
-HIP_SOURCES = ["rcb_engine.hip", "rib.hip", "imbalance.hip", "mesh.hip", "grid.hip", "coupe_api.cpp", "serve.cpp"]
+HIP_SOURCES = ["rcb_engine.hip", "rib.hip", "imbalance.hip", "mesh.hip", "grid.hip", "mesh_parts.hip", "coupe_api.cpp",
+               "serve.cpp"]
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared",
     "--offload-arch=gfx950",

@@ -64,6 +64,8 @@ struct Instance {
     uint32_t tile_begin;   // its first tile at level 0
     double tol;
     uint64_t heap_base;    // its dense split heap: heap_split[heap_base + h]
+    uint64_t src;          // index of its first point in the caller's weights and ids
+    uint64_t src_pt;       // offset of its first point's first coordinate in the caller's points
     // filled on the device
     uint64_t bmin[kMaxDim];     // ordered keys of the f64 bounding box
     uint64_t bmax[kMaxDim];

@@ -66,6 +66,53 @@ struct Profiler {
     void release();
 };
 
+// A repartition's starting point: the resident working copy of a mesh (SoA
+// f32 coordinates of its points in input order), which replaces the
+// prologue's AoS -> SoA conversion; the weights are the call's d_weights.
+struct ResidentSource {
+    const float *x[3];
+};
+
+// What a context learns from a call about the next call of the same shape
+// (the same instance table and dimension): the weight mode, and per level
+// the decisions the split search needed and the parts still searching after
+// the first decision.
+struct Hints {
+    uint64_t shape = 0;                 // hash of the instance table and the dimension (0: none)
+    int weight_hint = -1;               // weight mode (0 u16, 1 i32, 2 i64, 3 f64)
+    int passes_hint = 0;                // most decisions one level's search needed
+    std::vector<uint8_t> level_passes;  // decisions each level's search needed
+    std::vector<uint32_t> level_open;   // parts of each level still searching after the first decision
+};
+
+constexpr size_t kHintShapes = 16;      // shapes a context remembers
+
+// What a call queued and must check once the stream has run it: the
+// readback of its counters (split-search passes per level, errors) and of
+// its prologue flags (the weight mode the data needs).
+struct PendingCall {
+    size_t levels = 0;
+    int spec = 0;                       // search passes queued per level (0: host-checked)
+    std::vector<int> depth;             // queued passes of each level
+    uint64_t shape = 0;
+    double ws_bytes = 0.0;              // bytes of one stored weight
+    int mode = -1;                      // weight mode the call ran in
+    int wkind = 0;                      // caller's weight type
+};
+
+// A captured call: the graph of every launch and copy of one call of a
+// shape, replayed while the context's buffers, the call's arrays and the
+// shape's hints are those it was captured with.
+struct GraphEntry {
+    uint64_t key = 0;                   // shape, arrays, hints and switches
+    uint64_t layout = 0;                // the context's buffers when captured
+    hipGraphExec_t exec = nullptr;
+    PendingCall pending;
+    coupe_hip_stats stats{};            // the counters queuing the call set
+};
+
+constexpr size_t kGraphs = 8;           // graphs a context keeps
+
 struct Context {
     int device = 0;
     std::mutex lock;                 // one call at a time per context
@@ -90,15 +137,34 @@ struct Context {
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
     uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
-    int weight_hint = -1;            // weight mode of the last call (0 u16, 1 i32, 2 i64, 3 f64)
-    int passes_hint = 0;             // most decisions one level's search needed in the last call
-    // decisions each level's search needed in the last call, and that call's
-    // shape (points, levels, instances, dimension): a call of the same shape
-    // queues per level as many passes as that level needed
-    std::vector<uint8_t> level_passes;
-    // parts of each level still searching after their first decision in that call
-    std::vector<uint32_t> level_open;
-    uint64_t level_shape = 0;
+    bool graphs = true;              // replay a call of a known shape as one captured graph
+    bool capturing = false;          // the running call is being captured
+    PendingCall pending;             // what the running call queued
+    std::vector<GraphEntry> graph_cache;
+    hipStream_t gstream = nullptr;   // where graphs are captured and launched
+    hipEvent_t gevent = nullptr;     // orders gstream after the caller's stream
+    uint64_t layout() ;              // hash of every buffer's address and size
+    void drop_graphs();
+    int weight_hint = -1;            // weight mode of the last call, for a shape not seen before
+    std::vector<Hints> hints;        // per shape, the most recent last
+    const Hints *hints_for(uint64_t shape) const {
+        for (const Hints &h : hints)
+            if (h.shape == shape) return &h;
+        return nullptr;
+    }
+    Hints &remember(uint64_t shape) {   // the shape's entry, made the most recent
+        for (size_t i = 0; i < hints.size(); i++)
+            if (hints[i].shape == shape) {
+                Hints h = std::move(hints[i]);
+                hints.erase(hints.begin() + (long)i);
+                hints.push_back(std::move(h));
+                return hints.back();
+            }
+        if (hints.size() >= kHintShapes) hints.erase(hints.begin());
+        hints.emplace_back();
+        hints.back().shape = shape;
+        return hints.back();
+    }
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
     DevBuf gparts[2], gaxis, gchunk, gcount, gctr;   // Grid::rcb: subgrids, axis sums, chunk offsets
@@ -116,6 +182,17 @@ struct Context {
     coupe_hip_stats stats{};
     Profiler prof;
     bool borrowed_host = false;      // host_scratch is not the context's own pinned allocation
+    const float *slot0[3] = {nullptr, nullptr, nullptr};   // resident working copy of the running call, if any
+    const float *coords0(int k) const { return slot0[0] ? slot0[k] : (const float *)cx[0][k].ptr; }
+    // the hints of the context's most recent call (a resident mesh keeps its own)
+    void save_hints(Hints &h) const {
+        if (!hints.empty()) h = hints.back();
+    }
+    void load_hints(const Hints &h) {
+        if (!h.shape) return;
+        remember(h.shape) = h;
+        weight_hint = h.weight_hint;
+    }
     uint64_t device_bytes() const;
     void release_all();
     void forget_all();               // every buffer dropped without freeing (arena contexts)
@@ -153,6 +230,13 @@ inline bool full_first_from_env() {
     return !(v && v[0] == '0');
 }
 
+// COUPE_HIP_GRAPHS=0: never capture calls into graphs (every call launches
+// its kernels one by one).
+inline bool graphs_from_env() {
+    const char *v = getenv("COUPE_HIP_GRAPHS");
+    return !(v && v[0] == '0');
+}
+
 // COUPE_HIP_SCAN_DIRECT=b: a scan of more than b blocks scans the block
 // totals instead of re-summing them per block (default 2048, at least 1).
 inline uint64_t scan_direct_from_env() {
@@ -180,6 +264,7 @@ inline void configure_from_env(Context &c) {
     c.weights16 = weights16_from_env();
     c.full_first = full_first_from_env();
     c.scan_direct = scan_direct_from_env();
+    c.graphs = graphs_from_env();
 }
 
 // RCB of a batch of independent instances packed in input order (see
@@ -188,7 +273,17 @@ inline void configure_from_env(Context &c) {
 // then adds to the profile the caller started instead of starting one.
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
-                         const double *tols, size_t n_inst, const Rotation *rot = nullptr);
+                         const double *tols, size_t n_inst, const Rotation *rot = nullptr,
+                         const ResidentSource *src = nullptr, const uint64_t *src_idx = nullptr,
+                         const uint64_t *src_pt = nullptr);
+
+// A batch of instances of dimension 2 and 3 in one call: dims[q] per
+// instance, the caller's arrays packed in the given order (instance q's
+// points are counts[q] * dims[q] doubles).  The instances of each dimension
+// run as one batch, both on `st`.
+enum coupe_err rcb_batch_mixed(Context &cx, hipStream_t st, uint64_t *d_out, const double *d_points, int wkind,
+                               const void *d_weights, const uint64_t *counts, const size_t *dims, const size_t *levels,
+                               const double *tols, size_t n_inst);
 
 // Exclusive scan of n_dev (read on the device, at most n_ub; n_ub itself when
 // n_dev is null) uint64 values in place; `total` (device, may be null)
@@ -212,6 +307,13 @@ enum coupe_err parts_load(Context &cx, hipStream_t st, const uint64_t *ids, uint
                           uint64_t num_parts, void *host_loads);
 enum coupe_err imbalance(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,
                          uint64_t num_parts, double *out);
+
+// The working copy of a resident mesh, points [first, first + count): f32
+// SoA coordinates x[k] and weights mw (of kind wkind) from f64 AoS points and
+// weights given for that range (device pointers; either may be null).
+// Queued on `st`; the caller checks hipGetLastError and synchronises.
+void store_working_copy(hipStream_t st, size_t dim, int wkind, const double *pts, const void *w, uint64_t first,
+                        uint64_t count, float *const x[3], void *mw);
 
 // RCB on device-resident inputs (see coupe_hip_rcb_device).
 enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,

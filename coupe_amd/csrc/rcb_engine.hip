@@ -29,12 +29,15 @@ enum coupe_err map_hip_error(hipError_t e) {
 }
 
 static thread_local Arena *t_arena = nullptr;
+static thread_local bool t_capturing = false;   // a stream capture is running on this thread
 
 ArenaScope::ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
 ArenaScope::~ArenaScope() { t_arena = prev; }
 
 void DevBuf::ensure(size_t need) {
     if (need <= bytes) return;
+    // a capture records no allocation: the caller gives it up and runs eagerly
+    if (t_capturing) throw HipFailure{hipErrorStreamCaptureUnsupported};
     const size_t sz = need < 256 ? 256 : need;
     if (t_arena) {
         // carve: 256-byte aligned, never freed (the old carve stays unused)
@@ -118,6 +121,7 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 }
 
 // Small device counters shared by the kernels of one call.
+
 constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
 struct Counters {
     uint32_t n_parts[3];
@@ -135,6 +139,41 @@ struct Counters {
     uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
     uint32_t end;                               // end of the call's readback
 };
+
+// The call's counters, read back into host_scratch once the stream has run
+// it: errors, the bytes its search passes read, and per level the decisions
+// its search needed (speculative searches), which become the shape's hints.
+// Returns true when a speculative search did not end in its queued passes.
+static bool finish_levels(Context &cx) {
+    const PendingCall &P = cx.pending;
+    const size_t rb_off = offsetof(Counters, swept);
+    const char *h_rb = (const char *)cx.host_scratch + 256;
+    const Counters *h = (const Counters *)(h_rb - rb_off);   // only the copied fields are read
+    if (h->error) throw HipFailure{hipErrorUnknown};
+    cx.swept_bytes = (double)h->swept * (4.0 + P.ws_bytes) + (double)h->first_read * (8.0 + P.ws_bytes);
+    cx.stats.first_read = h->first_read;
+    cx.stats.forwarded = h->forwarded;
+    cx.bins_clean = true;
+    if (P.spec == 0) return false;
+    // passes per level: the first decision plus every pass whose gate was open
+    std::vector<uint8_t> used(P.levels, 0);
+    std::vector<uint32_t> open1(P.levels, 0);
+    for (size_t l = 0; l < P.levels; l++) {
+        uint32_t passes = 1;
+        for (int p = 1; p <= P.depth[l]; p++)
+            if (h->spec[l * (P.spec + 1) + p - 1]) passes++;
+        cx.stats.passes += passes;
+        cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
+        used[l] = (uint8_t)passes;
+        open1[l] = h->spec[l * (P.spec + 1)];
+    }
+    if (h->unfinished == 0) {   // (an unfinished run keeps the last call's figures for its rerun)
+        Hints &e = cx.remember(P.shape);
+        e.level_passes = used;
+        e.level_open = open1;
+    }
+    return h->unfinished != 0;
+}
 
 static inline unsigned grid_for(uint64_t items, unsigned per_block) {
     uint64_t g = (items + per_block - 1) / per_block;
@@ -213,12 +252,25 @@ struct BatchPlan {
     uint64_t tiles_cap = 1;
     uint64_t heap_total = 0;      // dense split heap entries
     bool dense = true;            // every instance has iter_count <= kDenseLevels
+    uint64_t shape = 0;           // hash of the instance table and the dimension (hints)
 };
 
-static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const double *tols, size_t n_inst) {
+// The instances in the order given, packed one after the other in the
+// working arrays.  src / src_pt (optional): where each instance's points
+// start in the caller's weights and ids, and in the caller's points (doubles);
+// by default the caller packs them the same way.
+static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const double *tols, size_t n_inst,
+                            size_t dim, const uint64_t *src = nullptr, const uint64_t *src_pt = nullptr) {
     BatchPlan b;
     b.inst.resize(n_inst);
     uint64_t begin = 0, tile = 0;
+    uint64_t h = 0xcbf29ce484222325ull ^ (uint64_t)dim;   // FNV-1a over the table
+    auto mix = [&](uint64_t v) {
+        for (int k = 0; k < 8; k++) {
+            h ^= (v >> (8 * k)) & 0xff;
+            h *= 0x100000001b3ull;
+        }
+    };
     for (size_t q = 0; q < n_inst; q++) {
         Instance &I = b.inst[q];
         memset(&I, 0, sizeof I);
@@ -227,6 +279,11 @@ static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const 
         I.levels = (uint32_t)std::min<size_t>(levels[q], 0xffffffffu);
         I.tile_begin = (uint32_t)tile;
         I.tol = tols[q];
+        I.src = src ? src[q] : begin;
+        I.src_pt = src_pt ? src_pt[q] : begin * dim;
+        mix(I.count);
+        mix(I.levels);
+        mix(__builtin_bit_cast(uint64_t, I.tol));
         begin += counts[q];
         tile += (counts[q] + kTile - 1) / kTile;
         if (counts[q] > 0) b.max_levels = std::max<uint32_t>(b.max_levels, I.levels);
@@ -254,6 +311,7 @@ static BatchPlan plan_batch(const uint64_t *counts, const size_t *levels, const 
             b.heap_total += (I.levels ? (1ull << I.levels) - 1 : 0);
         }
     }
+    b.shape = h ? h : 1;
     return b;
 }
 
@@ -279,13 +337,16 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // had this shape, as many as that level's search needed then (at least 1;
     // a search that needs more is closed by its last queued decision and the
     // batch runs again with host checks)
-    const uint64_t shape = (n * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)L << 48) ^ ((uint64_t)n_inst << 8) ^ (uint64_t)D;
+    const uint64_t shape = B.shape;
+    const Hints *seen = cx.hints_for(shape);
     std::vector<int> depth(L, spec_passes);
-    if (spec_passes > 0 && cx.level_shape == shape && cx.level_passes.size() == L)
-        for (size_t l = 0; l < L; l++) depth[l] = std::max(1, std::min(spec_passes, (int)cx.level_passes[l]));
+    if (spec_passes > 0 && seen && seen->level_passes.size() == L)
+        for (size_t l = 0; l < L; l++) depth[l] = std::max(1, std::min(spec_passes, (int)seen->level_passes[l]));
     std::vector<uint8_t> used(L, 0);   // decisions per level in this call
     std::vector<uint32_t> open1(L, 0);  // parts still searching after each level's first decision
-    const bool same_shape = cx.level_shape == shape && cx.level_open.size() == L;
+    const std::vector<uint32_t> last_open = seen && seen->level_open.size() == L ? seen->level_open
+                                                                                  : std::vector<uint32_t>();
+    const bool same_shape = !last_open.empty();
 
     // Two levels per move (dense trees, prefix partition): three part / tile
     // tables (a level, its children searched in place, its grandchildren).
@@ -348,7 +409,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // the entries of the next level's tiles and k_tile_left clears what it
     // reads, but the last level's counts are never read (no partition), so
     // a call cannot rely on the previous one leaving them clear.
-    if (!pairs) check(hipMemsetAsync(cx.below.ptr, 0, tiles_cap * sizeof(uint32_t), st));
+    if (!pairs) k_fill_u32<<<grid_for(tiles_cap, kBlock), kBlock, 0, st>>>((uint32_t *)cx.below.ptr, tiles_cap, 0u);
     // Bins: parts x bins of the widest pass of any level.
     uint64_t bins = 1;
     for (size_t l = 0; l < L; l++)
@@ -359,7 +420,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     cx.bmin.ensure(bins * 4);
     if (fresh_bins || !cx.bins_clean) {
         // k_decide leaves bins zeroed after reading them; only new or dirty
-        // buffers need a clear.
+        // buffers need a clear (never in a captured call: it starts clean).
+        if (cx.capturing) throw HipFailure{hipErrorStreamCaptureUnsupported};
         check(hipMemsetAsync(cx.bcnt.ptr, 0, cx.bcnt.bytes, st));
         check(hipMemsetAsync(cx.bw.ptr, 0, cx.bw.bytes, st));
         check(hipMemsetAsync(cx.bmin.ptr, 0xff, cx.bmin.bytes, st));
@@ -375,7 +437,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     float *cs[3][3];
     WS *ws[2];
     for (int b = 0; b < 3; b++)
-        for (int k = 0; k < 3; k++) cs[b][k] = k < D ? (float *)cx.cx[b][k].ptr : nullptr;
+        for (int k = 0; k < 3; k++)
+            cs[b][k] = k < D ? (b == 0 ? (float *)cx.coords0(k) : (float *)cx.cx[b][k].ptr) : nullptr;
     ws[0] = w_first;
     ws[1] = (WS *)cx.w[1].ptr;
     auto cslot = [](size_t level) { return level == 0 ? 0 : (level % 2 == 1 ? 1 : 2); };
@@ -383,11 +446,14 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
     const int real_mode = std::is_same<Acc, double>::value;
     const double payload = 4.0 * D + sizeof(WS);   // bytes of one point in the working arrays
-    check(hipMemsetAsync(ctr, 0, sizeof(Counters), st));
+    static_assert(sizeof(Counters) % 4 == 0, "counters are cleared as 32-bit words");
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
+        k_fill_u32<<<grid_for(sizeof(Counters) / 4, kBlock), kBlock, 0, st>>>((uint32_t *)ctr, sizeof(Counters) / 4,
+                                                                               0u);
         k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(
             insts, n_inst, (Part *)cx.parts[0].ptr, D, real_mode, root_fwd ? (uint32_t *)cx.fwd_cnt.ptr : nullptr,
-            (uint64_t *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr);
+            (uint64_t *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr, &ctr->n_parts[0], &ctr->n_tiles[0],
+            (uint32_t)B.tiles0);
     });
 
     cx.stats.levels = 0;
@@ -480,11 +546,6 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
     // Level 0: the root parts' tables come from k_init_roots (in prologue());
     // register sweep of the roots, then their split search.
-    check(hipMemcpyAsync(&ctr->n_parts[0], &n_inst, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    {
-        const uint32_t t0 = (uint32_t)B.tiles0;
-        check(hipMemcpyAsync(&ctr->n_tiles[0], &t0, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    }
     uint64_t parts_ub = n_inst;
     uint64_t tiles_ub = std::max<uint64_t>(B.tiles0, 1);
     if (L > 0 && root_fwd) {
@@ -571,7 +632,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             // pass with every candidate over all the parents' tiles instead.
             const uint64_t pub0 = std::max<uint64_t>(B.parts_ub[level], 1);
             const bool full_first = cx.full_first && same_shape && level + 2 == L &&
-                                    2 * (uint64_t)cx.level_open[level] > pub0;
+                                    2 * (uint64_t)last_open[level] > pub0;
             // the move that made this level forwarded its children's first pass
             const bool fwd_in = cx.forward && (level >= 2 || root_fwd) && level + 1 < L && !full_first;
             uint64_t *pl = (uint64_t *)cx.pleft[par].ptr;
@@ -751,44 +812,62 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     static_assert(offsetof(Counters, end) - offsetof(Counters, swept) <= kHostScratch - 256, "readback");
     char *h_rb = (char *)cx.host_scratch + 256;
     check(hipMemcpyAsync(h_rb, (char *)ctr + rb_off, rb_len, hipMemcpyDeviceToHost, st));
+    cx.pending.levels = L;
+    cx.pending.spec = spec_passes;
+    cx.pending.depth = depth;
+    cx.pending.shape = shape;
+    cx.pending.ws_bytes = sizeof(WS);
+    if (cx.capturing) return false;   // checked once the graph has run
     check(hipStreamSynchronize(st));
-    const Counters *h = (const Counters *)(h_rb - rb_off);   // only the copied fields are read
-    if (h->error) throw HipFailure{hipErrorUnknown};
-    cx.swept_bytes = (double)h->swept * (4.0 + sizeof(WS)) + (double)h->first_read * (8.0 + sizeof(WS));
-    cx.stats.first_read = h->first_read;
-    cx.stats.forwarded = h->forwarded;
-    cx.bins_clean = true;
-    if (spec_passes > 0) {
-        // passes per level: the first decision plus every pass whose gate was open
-        for (size_t l = 0; l < L; l++) {
-            uint32_t passes = 1;
-            for (int p = 1; p <= depth[l]; p++)
-                if (h->spec[l * (spec_passes + 1) + p - 1]) passes++;
-            cx.stats.passes += passes;
-            cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
-            used[l] = (uint8_t)passes;
-            open1[l] = h->spec[l * (spec_passes + 1)];
-        }
+    // the host-checked search counted its passes itself
+    if (spec_passes == 0) {
+        Hints &e = cx.remember(shape);
+        e.level_passes = used;
+        e.level_open = open1;
     }
-    if (h->unfinished == 0) {   // (an unfinished run keeps the last call's figures for its rerun)
-        cx.level_passes = used;
-        cx.level_open = open1;
-        cx.level_shape = shape;
-    }
-    return h->unfinished != 0;
+    return finish_levels(cx);
+}
+
+// Weight mode of a call's data from its prologue flags.  Integer kinds and
+// integer-valued doubles whose partial sums stay exact in f64 (|sum| <= 2^53)
+// use exact int64 sums: the same bits as coupe's sums in any order; they are
+// stored in 16 bits when all are in [0, 65535], else in 32 or 64.  Other
+// doubles use f64 sums.  Modes: 0 u16, 1 i32, 2 i64, 3 f64.
+static int weight_mode_of(const Prologue &p, int wkind, bool weights16) {
+    const bool integral = wkind != COUPE_DOUBLE || (!p.non_integral && p.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
+    if (!integral) return 3;
+    if (weights16 && !p.not_u16) return 0;
+    return p.not_i32 ? 2 : 1;
+}
+
+static Prologue *prologue_readback(Context &cx) {
+    static_assert(kHostScratch >= 8192 + sizeof(Prologue), "prologue readback");
+    return (Prologue *)((char *)cx.host_scratch + 8192);
+}
+
+// Once the call has run: the weight mode its data needs becomes the hint;
+// false when the mode the call ran in cannot hold that data (an integer
+// mode holds any narrower one; f64 and the integer modes differ), and the
+// call must run again.
+static bool mode_held(Context &cx) {
+    const int best = weight_mode_of(*prologue_readback(cx), cx.pending.wkind, cx.weights16);
+    cx.weight_hint = best;
+    cx.remember(cx.pending.shape).weight_hint = best;
+    const int mode = cx.pending.mode;
+    return (mode == 3) == (best == 3) && mode >= best;
 }
 
 template <int D, typename WIN>
 static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const double *pts, const WIN *win,
                      Prologue *h, bool wait, int hint) {
     const uint32_t n_inst = (uint32_t)B.inst.size();
-    check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
-    // bounding boxes start empty: min key = max, max key = 0
-    Prologue init;
-    memset(&init, 0, sizeof init);
-    check(hipMemcpyAsync(cx.prologue.ptr, &init, sizeof init, hipMemcpyHostToDevice, st));
+    // the instance table (a captured call's is copied before each launch)
+    if (!cx.capturing)
+        check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_reset_instances<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>((Instance *)cx.instances.ptr, n_inst);
+        // bounding boxes start empty (min key = max, max key = 0), flags clear
+        k_reset_instances<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>((Instance *)cx.instances.ptr, n_inst,
+                                                                        (Prologue *)cx.prologue.ptr);
         // blocks (instance, share of its tiles): about 4 tiles per thread
         uint64_t max_nt = 1;
         for (const Instance &I : B.inst) max_nt = std::max<uint64_t>(max_nt, (I.count + kTile - 1) / kTile);
@@ -822,8 +901,21 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
                                              (Prologue *)cx.prologue.ptr, rot);
             });
         };
-        if (cx.rot) launch(k_prologue<D, WIN, true>);
-        else launch(k_prologue<D, WIN, false>);
+        if (cx.slot0[0]) {
+            // a resident working copy: read its f32 coordinates, write none
+            const double sbytes = (double)B.n * (4.0 * D + sizeof(WIN) + (w32 ? 4.0 : 0.0) + (w16 ? 2.0 : 0.0));
+            const unsigned g = persistent_grid(cx, k_prologue_soa<D, WIN>, nt);
+            timed(cx, st, COUPE_HIP_K_PROLOGUE, sbytes, [&] {
+                k_prologue_soa<D, WIN><<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, cx.coords0(0),
+                                                             cx.coords0(1), D > 2 ? cx.coords0(2) : nullptr, win,
+                                                             w32, w16, (Instance *)cx.instances.ptr,
+                                                             (Prologue *)cx.prologue.ptr);
+            });
+        } else if (cx.rot) {
+            launch(k_prologue<D, WIN, true>);
+        } else {
+            launch(k_prologue<D, WIN, false>);
+        }
     }
     // without `wait`, h is pinned memory read once the call's last sync is done
     check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
@@ -835,7 +927,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
                       const WIN *win, int wkind, int spec_passes) {
     const uint64_t n = B.n;
     // Coordinate slots: 0 (prologue output, input order), 1 and 2 (levels).
-    for (int b = 0; b < 3; b++)
+    for (int b = cx.slot0[0] ? 1 : 0; b < 3; b++)
         for (int k = 0; k < D; k++) cx.cx[b][k].ensure(std::max<uint64_t>(n, 1) * sizeof(float));
     cx.w[0].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
     cx.prologue.ensure(sizeof(Prologue));
@@ -850,17 +942,9 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     // are in [0, 65535], else in 32 or 64.  Other doubles use f64 sums.
     // Modes: 0 u16, 1 i32, 2 i64, 3 f64.
     const bool wrap32 = wkind == COUPE_INT;
-    auto best_mode = [&](const Prologue &p) {
-        const bool integral = wkind != COUPE_DOUBLE ||
-                              (!p.non_integral && p.sum_abs <= 9007199254740992.0 * (1.0 - 1e-6));
-        if (!integral) return 3;
-        if (cx.weights16 && !p.not_u16) return 0;
-        return p.not_i32 ? 2 : 1;
-    };
-    // an integer mode holds any narrower one; f64 and the integer modes differ
-    auto holds = [](int mode, int best) { return (mode == 3) == (best == 3) && mode >= best; };
+    auto best_mode = [&](const Prologue &p) { return weight_mode_of(p, wkind, cx.weights16); };
     auto weight_bytes = [](int mode) { return mode == 0 ? 2u : (mode == 1 ? 4u : 8u); };
-    const unsigned grid = std::min<unsigned>(grid_for(n, kBlock), 4096);
+    const unsigned tgrid = (unsigned)std::min<uint64_t>(std::max<uint64_t>(B.tiles0, 1), 4096);
     auto run_mode = [&](int mode) {
         cx.stats.weight_mode = mode == 3 ? 1 : 0;
         cx.stats.weight_bytes = weight_bytes(mode);
@@ -871,12 +955,16 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
             return run_levels<D, int32_t, uint64_t>(cx, st, B, wrap32, d_out, (int32_t *)cx.w[0].ptr, spec_passes);
         if (mode == 2) {
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-                k_store_weights<WIN, int64_t><<<grid, kBlock, 0, st>>>(win, (int64_t *)cx.w[0].ptr, n);
+                k_store_weights<WIN, int64_t><<<tgrid, kBlock, 0, st>>>(
+                    (const Tile *)cx.tiles0.ptr, (uint32_t)B.tiles0, (const Instance *)cx.instances.ptr, win,
+                    (int64_t *)cx.w[0].ptr);
             });
             return run_levels<D, int64_t, uint64_t>(cx, st, B, wrap32, d_out, (int64_t *)cx.w[0].ptr, spec_passes);
         }
         timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-            k_store_weights<WIN, double><<<grid, kBlock, 0, st>>>(win, (double *)cx.w[0].ptr, n);
+            k_store_weights<WIN, double><<<tgrid, kBlock, 0, st>>>(
+                (const Tile *)cx.tiles0.ptr, (uint32_t)B.tiles0, (const Instance *)cx.instances.ptr, win,
+                (double *)cx.w[0].ptr);
         });
         return run_levels<D, double, double>(cx, st, B, 0, d_out, (double *)cx.w[0].ptr, spec_passes);
     };
@@ -884,10 +972,10 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     // prologue's flags are then read with the call's last counters, without a
     // wait after the prologue, and a call whose data does not fit that mode
     // runs again in its own.  A new context waits for the flags first.
-    static_assert(kHostScratch >= 8192 + sizeof(Prologue), "prologue readback");
-    Prologue *hp = (Prologue *)((char *)cx.host_scratch + 8192);
+    Prologue *hp = prologue_readback(cx);
     for (;;) {
-        const int hint = cx.weight_hint;
+        const Hints *seen = cx.hints_for(B.shape);
+        const int hint = seen ? seen->weight_hint : cx.weight_hint;
         int mode = hint;
         // buffers for the expected mode before anything is queued (a buffer
         // that grows is freed and allocated again)
@@ -898,10 +986,11 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
             mode = best_mode(*hp);
             if (mode >= 2) cx.w[0].ensure(std::max<uint64_t>(n, 1) * 8u);
         }
+        cx.pending.mode = mode;
+        cx.pending.wkind = wkind;
         const bool unfinished = run_mode(mode);
-        const int best = best_mode(*hp);   // complete: run_levels waited for the stream
-        cx.weight_hint = best;
-        if (!holds(mode, best)) continue;
+        if (cx.capturing) return false;   // checked once the graph has run
+        if (!mode_held(cx)) continue;     // complete: run_levels waited for the stream
         return unfinished;
     }
 }
@@ -917,21 +1006,160 @@ static bool run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPla
     }
 }
 
+uint64_t Context::layout() {
+    uint64_t h = 0xcbf29ce484222325ull;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 0x100000001b3ull;
+        h ^= h >> 29;
+    };
+    for_each_buf([&](DevBuf &d) {
+        mix((uint64_t)(uintptr_t)d.ptr);
+        mix(d.bytes);
+    });
+    mix((uint64_t)(uintptr_t)host_scratch);
+    return h;
+}
+
+void Context::drop_graphs() {
+    for (GraphEntry &g : graph_cache)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    graph_cache.clear();
+    if (gstream) (void)hipStreamDestroy(gstream);
+    if (gevent) (void)hipEventDestroy(gevent);
+    gstream = nullptr;
+    gevent = nullptr;
+}
+
+// The graph key of a call: its shape, arrays, the launch sequence its hints
+// select (queued passes per level, the last pair's first pass) and the
+// context's switches.
+static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int wkind, const void *out,
+                          const void *pts, const void *w, int spec) {
+    uint64_t h = 0x84222325cbf29ce4ull;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 0x100000001b3ull;
+        h ^= h >> 31;
+    };
+    mix(B.shape);
+    mix(dim);
+    mix((uint64_t)wkind);
+    mix((uint64_t)(uintptr_t)out);
+    mix((uint64_t)(uintptr_t)pts);
+    mix((uint64_t)(uintptr_t)w);
+    for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)cx.slot0[k]);
+    mix((uint64_t)spec);
+    mix((uint64_t)cx.pairs | (uint64_t)cx.forward << 1 | (uint64_t)cx.weights16 << 2 | (uint64_t)cx.full_first << 3);
+    mix(cx.scan_direct);
+    const Hints *e = cx.hints_for(B.shape);
+    mix((uint64_t)(int64_t)(e ? e->weight_hint : cx.weight_hint));
+    const size_t L = B.max_levels;
+    for (size_t l = 0; l < L; l++) {
+        const int d = e && e->level_passes.size() == L ? std::max(1, std::min(spec, (int)e->level_passes[l])) : spec;
+        const bool ff = e && e->level_open.size() == L &&
+                        2 * (uint64_t)e->level_open[l] > std::max<uint64_t>(B.parts_ub[l], 1);
+        mix((uint64_t)d | (uint64_t)ff << 16);
+    }
+    return h;
+}
+
+// One call of a known shape as a graph: captured on its first such call
+// (every launch and readback of the eager path, none of its waits), then
+// replayed.  The instance table is copied before each launch.  Returns 0
+// when the results stand, 1 when a search outran its queued passes (rerun
+// host-checked), 2 when the graph could not be made or run, or the data
+// needs another weight mode (rerun eagerly).
+template <typename Run>
+static int run_captured(Context &cx, hipStream_t caller, const BatchPlan &B, size_t dim, int wkind, const void *out,
+                        const void *pts, const void *w, int spec, Run &&run) {
+    // graphs are captured and launched on the context's own stream (a
+    // capture cannot use the null stream), after the caller's work so far
+    if (!cx.gstream) {
+        check(hipStreamCreateWithFlags(&cx.gstream, hipStreamNonBlocking));
+        check(hipEventCreateWithFlags(&cx.gevent, hipEventDisableTiming));
+    }
+    hipStream_t st = cx.gstream;
+    check(hipEventRecord(cx.gevent, caller));
+    check(hipStreamWaitEvent(st, cx.gevent, 0));
+    const uint64_t key = graph_key(cx, B, dim, wkind, out, pts, w, spec);
+    const uint64_t layout = cx.layout();
+    GraphEntry *G = nullptr;
+    for (GraphEntry &g : cx.graph_cache)
+        if (g.key == key && g.layout == layout) G = &g;
+    if (!G) {
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return 2;
+        }
+        cx.capturing = t_capturing = true;
+        bool queued = true;
+        try {
+            run(spec, st);
+        } catch (const HipFailure &) {
+            queued = false;
+        }
+        cx.capturing = t_capturing = false;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        hipGraphExec_t exec = nullptr;
+        const bool made = queued && e == hipSuccess && graph &&
+                          hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!made) {
+            (void)hipGetLastError();
+            return 2;
+        }
+        if (cx.graph_cache.size() >= kGraphs) {
+            (void)hipGraphExecDestroy(cx.graph_cache.front().exec);
+            cx.graph_cache.erase(cx.graph_cache.begin());
+        }
+        GraphEntry g;
+        g.key = key;
+        g.layout = layout;
+        g.exec = exec;
+        g.pending = cx.pending;
+        g.stats = cx.stats;   // what queuing the call set (levels, bins, weight mode, ...)
+        cx.graph_cache.push_back(std::move(g));
+        G = &cx.graph_cache.back();
+        cx.stats.graph = 2;
+    } else {
+        cx.stats = G->stats;
+        cx.stats.graph = 1;
+    }
+    cx.pending = G->pending;
+    check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), B.inst.size() * sizeof(Instance), hipMemcpyHostToDevice,
+                         st));
+    check(hipGraphLaunch(G->exec, st));
+    check(hipStreamSynchronize(st));
+    const bool unfinished = finish_levels(cx);
+    if (!mode_held(cx)) return 2;
+    return unfinished ? 1 : 0;
+}
+
 enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t dim, const double *d_points,
                          int wkind, const void *d_weights, const uint64_t *counts, const size_t *levels,
-                         const double *tols, size_t n_inst, const Rotation *rot) {
+                         const double *tols, size_t n_inst, const Rotation *rot, const ResidentSource *src,
+                         const uint64_t *src_idx, const uint64_t *src_pt) {
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
+    if (rot && src) return COUPE_ERR_CRASH;   // a rotation needs the points themselves
     memset(&cx.stats, 0, sizeof cx.stats);
     if (!rot) cx.prof.reset();
-    struct RotScope {
+    struct CallScope {
         Context &c;
-        RotScope(Context &c_, const Rotation *r) : c(c_) { c.rot = r; }
-        ~RotScope() { c.rot = nullptr; }
-    } rot_scope(cx, rot);
+        CallScope(Context &c_, const Rotation *r, const ResidentSource *s) : c(c_) {
+            c.rot = r;
+            for (int k = 0; k < 3; k++) c.slot0[k] = s ? s->x[k] : nullptr;
+        }
+        ~CallScope() {
+            c.rot = nullptr;
+            for (int k = 0; k < 3; k++) c.slot0[k] = nullptr;
+        }
+    } call_scope(cx, rot, src);
     cx.swept_bytes = 0.0;
     try {
-        BatchPlan B = plan_batch(counts, levels, tols, n_inst);
+        BatchPlan B = plan_batch(counts, levels, tols, n_inst, dim, src_idx, src_pt);
         cx.stats.points = B.n;
         if (B.n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
         if (B.n > 0xffffffffull) return COUPE_ERR_ALLOC;
@@ -942,25 +1170,35 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
             check(hipDeviceGetAttribute(&cx.cu_count, hipDeviceAttributeMultiprocessorCount, cx.device));
             if (cx.cu_count <= 0) cx.cu_count = 1;
         }
-        auto run = [&](int spec) {
-            return dim == 2 ? run_dim<2>(cx, st, d_out, B, d_points, wkind, d_weights, spec)
-                            : run_dim<3>(cx, st, d_out, B, d_points, wkind, d_weights, spec);
+        auto run = [&](int spec, hipStream_t s) {
+            return dim == 2 ? run_dim<2>(cx, s, d_out, B, d_points, wkind, d_weights, spec)
+                            : run_dim<3>(cx, s, d_out, B, d_points, wkind, d_weights, spec);
         };
         // Passes queued per level: the configured depth, or fewer (never
         // under 2) when this context's last call needed fewer (its deepest
         // search's decisions); each queued pass costs two launches even when
         // its gate is closed.
         int spec = cx.search_passes;
-        if (cx.passes_hint > 0 && spec > 2) spec = std::max(2, std::min(spec, cx.passes_hint));
-        if (run(spec)) {
+        const Hints *seen = cx.hints_for(B.shape);
+        const int passes_hint = seen ? seen->passes_hint : 0;
+        if (passes_hint > 0 && spec > 2) spec = std::max(2, std::min(spec, passes_hint));
+        // A shape this context has run (hints known, buffers sized) runs as
+        // one captured graph: 0 done, 1 a search outran its queued passes,
+        // 2 not run that way (or its data needs another weight mode).
+        int how = 2;
+        if (cx.graphs && !rot && !cx.prof.enabled && !t_arena && spec > 0 && cx.bins_clean && seen &&
+            seen->weight_hint >= 0 && seen->level_passes.size() == B.max_levels)
+            how = run_captured(cx, st, B, dim, wkind, d_out, d_points, d_weights, spec, run);
+        const bool unfinished = how == 1 || (how == 2 && run(spec, st));
+        if (unfinished) {
             // a split search needed more passes than were queued: again,
             // checking every pass on the host
             cx.stats = coupe_hip_stats{};
             cx.stats.points = B.n;
             cx.stats.reruns = 1;
-            run(0);
+            run(0, st);
         }
-        cx.passes_hint = (int)cx.stats.max_passes;
+        cx.remember(B.shape).passes_hint = (int)cx.stats.max_passes;
         cx.prof.collect();
         cx.prof.out[COUPE_HIP_K_HIST].algo_bytes += cx.swept_bytes;   // first passes + standalone passes
     } catch (const HipFailure &f) {
@@ -975,6 +1213,37 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         return COUPE_ERR_ALLOC;
     }
     cx.stats.device_bytes = cx.device_bytes();
+    return COUPE_ERR_OK;
+}
+
+enum coupe_err rcb_batch_mixed(Context &cx, hipStream_t st, uint64_t *d_out, const double *d_points, int wkind,
+                               const void *d_weights, const uint64_t *counts, const size_t *dims, const size_t *levels,
+                               const double *tols, size_t n_inst) {
+    std::vector<uint64_t> cnt[2], src[2], src_pt[2];
+    std::vector<size_t> lev[2];
+    std::vector<double> tol[2];
+    uint64_t idx = 0, pt = 0;
+    for (size_t q = 0; q < n_inst; q++) {
+        if (dims[q] != 2 && dims[q] != 3) return COUPE_ERR_BAD_DIMENSION;
+        // a group of one instance may be deep (node-record descent); it then
+        // writes its ids at its working positions, so only the whole call may be
+        if (n_inst > 1 && levels[q] > (size_t)kDenseLevels) return COUPE_ERR_BAD_TYPE;
+        const int g = (int)dims[q] - 2;
+        cnt[g].push_back(counts[q]);
+        lev[g].push_back(levels[q]);
+        tol[g].push_back(tols[q]);
+        src[g].push_back(idx);
+        src_pt[g].push_back(pt);
+        idx += counts[q];
+        pt += counts[q] * dims[q];
+    }
+    for (int g = 0; g < 2; g++) {
+        if (cnt[g].empty()) continue;
+        const enum coupe_err e = rcb_batch(cx, st, d_out, (size_t)g + 2, d_points, wkind, d_weights, cnt[g].data(),
+                                           lev[g].data(), tol[g].data(), cnt[g].size(), nullptr, nullptr,
+                                           src[g].data(), src_pt[g].data());
+        if (e != COUPE_ERR_OK) return e;
+    }
     return COUPE_ERR_OK;
 }
 
@@ -996,6 +1265,22 @@ void Context::for_each_buf(F &&f) {
     for (DevBuf *d : all) f(*d);
 }
 
+void store_working_copy(hipStream_t st, size_t dim, int wkind, const double *pts, const void *w, uint64_t first,
+                        uint64_t count, float *const x[3], void *mw) {
+    if (count == 0) return;
+    const unsigned g = (unsigned)std::min<uint64_t>((count + kBlock - 1) / kBlock, 4096);
+    auto launch = [&](auto wtag) {
+        using WT = decltype(wtag);
+        if (dim == 2)
+            k_mesh_store<2, WT><<<g, kBlock, 0, st>>>(pts, (const WT *)w, first, count, x[0], x[1], x[2], (WT *)mw);
+        else
+            k_mesh_store<3, WT><<<g, kBlock, 0, st>>>(pts, (const WT *)w, first, count, x[0], x[1], x[2], (WT *)mw);
+    };
+    if (wkind == COUPE_INT) launch(int32_t{});
+    else if (wkind == COUPE_INT64) launch(int64_t{});
+    else launch(double{});
+}
+
 uint64_t Context::device_bytes() const {
     uint64_t b = 0;
     const_cast<Context *>(this)->for_each_buf([&](DevBuf &d) { b += d.bytes; });
@@ -1003,6 +1288,7 @@ uint64_t Context::device_bytes() const {
 }
 
 void Context::release_all() {
+    drop_graphs();
     for_each_buf([](DevBuf &d) { d.release(); });
     if (host_scratch && !borrowed_host) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

@@ -86,6 +86,74 @@ __device__ __forceinline__ void flush_instance(Instance *I, double (&mn)[D], dou
     sf = 0.0;
 }
 
+// Per-thread weight sums and the weight-kind flags of the prologue; w32 /
+// w16 (each optional) receive the weight as int32 and as a 16-bit unsigned
+// integer (used when every weight is an integer in [0, 65535]).
+struct WeightScan {
+    uint64_t su = 0;    // integer sum (two's complement, exact in any order)
+    double sf = 0.0;    // f64 sum
+    double sa = 0.0;    // sum of |w|
+    double ma = 0.0;    // max |w|
+    uint32_t nonint = 0, not32 = 0, not16 = 0;
+
+    template <typename WIN>
+    __device__ __forceinline__ void add(WIN w, uint64_t i, int32_t *__restrict__ w32, uint16_t *__restrict__ w16) {
+        if constexpr (std::is_same<WIN, double>::value) {
+            const double aw = fabs(w);
+            const bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
+            nonint |= !integral;
+            sf += w;
+            sa += aw;
+            ma = aw > ma ? aw : ma;
+            if (integral) {
+                su += (uint64_t)(int64_t)w;
+                const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
+                not32 |= !fits;
+                if (w32) w32[i] = fits ? (int32_t)w : 0;
+                const bool f16 = w >= 0.0 && w <= 65535.0;
+                not16 |= !f16;
+                if (w16) w16[i] = f16 ? (uint16_t)w : (uint16_t)0;
+            } else {
+                not32 = 1;
+                not16 = 1;
+            }
+        } else {
+            const int64_t wi = (int64_t)w;
+            su += (uint64_t)wi;
+            const double aw = fabs((double)wi);
+            sf += (double)wi;
+            sa += aw;
+            ma = aw > ma ? aw : ma;
+            const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
+            not32 |= !fits;
+            if (w32) w32[i] = (int32_t)wi;
+            const bool f16 = wi >= 0 && wi <= 65535;
+            not16 |= !f16;
+            if (w16) w16[i] = (uint16_t)wi;
+        }
+    }
+
+    // the flags and |w| statistics of the whole call (one atomic per wavefront)
+    __device__ __forceinline__ void flush_flags(Prologue *out) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sa += __shfl_xor(sa, o, 64);
+            const double m2 = __shfl_xor(ma, o, 64);
+            ma = m2 > ma ? m2 : ma;
+            nonint |= __shfl_xor(nonint, o, 64);
+            not32 |= __shfl_xor(not32, o, 64);
+            not16 |= __shfl_xor(not16, o, 64);
+        }
+        if (lane_id() == 0) {
+            atomicAdd(&out->sum_abs, sa);
+            atomicMax((unsigned long long *)&out->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, ma));
+            if (nonint) atomicOr(&out->non_integral, 1u);
+            if (not32) atomicOr(&out->not_i32, 1u);
+            if (not16) atomicOr(&out->not_u16, 1u);
+        }
+    }
+};
+
 // Workgroups stride over the level-0 tiles (each inside one instance) and
 // read the AoS points of a tile through LDS so every global load is
 // contiguous.
@@ -107,20 +175,26 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
     double mn[D], mx[D];
 #pragma unroll
     for (int k = 0; k < D; k++) { mn[k] = 1.7976931348623157e308; mx[k] = -1.7976931348623157e308; }
-    uint64_t su = 0;
-    double sf = 0.0, sa = 0.0, ma = 0.0;
-    uint32_t nonint = 0, not32 = 0, not16 = 0;
+    WeightScan ws;
     uint32_t cur = 0xffffffffu;
+    // the instance's points and weights in the caller's arrays
+    const double *ipts = pts;
+    const WIN *iwin = win;
+    uint64_t ibeg = 0;
     for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
-            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, su, sf, s_d, s_u);
+            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
             cur = T.part;
+            ipts = pts + insts[cur].src_pt;
+            iwin = win + insts[cur].src;
+            ibeg = insts[cur].begin;
         }
         for (uint32_t r0 = 0; r0 < T.count; r0 += kProPoints) {
             const uint64_t base = T.begin + r0;
             const uint32_t cnt = T.count - r0 < (uint32_t)kProPoints ? T.count - r0 : (uint32_t)kProPoints;
-            const double *src = pts + base * D;
+            const double *src = ipts + (base - ibeg) * D;
+            const WIN *wsrc = iwin + (base - ibeg);
             // every load of the round issued back to back: the AoS doubles
             // (clamped to the round's last one) and the points' weights
             constexpr int kDv = kProPoints * D / kBlock;
@@ -130,7 +204,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
 #pragma unroll
             for (int j = 0; j < kDv; j++) dv[j] = src[min((uint32_t)(j * kBlock + tid), nd - 1u)];
 #pragma unroll
-            for (int k = 0; k < 4; k++) wv[k] = win[base + min((uint32_t)(k * kBlock + tid), cnt - 1u)];
+            for (int k = 0; k < 4; k++) wv[k] = wsrc[min((uint32_t)(k * kBlock + tid), cnt - 1u)];
 #pragma unroll
             for (int j = 0; j < kDv; j++)
                 if ((uint32_t)(j * kBlock + tid) < nd) sp[j * kBlock + tid] = dv[j];
@@ -162,71 +236,88 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                         float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
                         cs[i] = (float)v;
                     }
-                    const WIN w = wv[k];
-                    if constexpr (std::is_same<WIN, double>::value) {
-                        const double aw = fabs(w);
-                        const bool integral = (w == trunc(w)) && aw <= 9007199254740992.0;
-                        nonint |= !integral;
-                        sf += w;
-                        sa += aw;
-                        ma = aw > ma ? aw : ma;
-                        if (integral) {
-                            su += (uint64_t)(int64_t)w;
-                            const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
-                            not32 |= !fits;
-                            if (w32) w32[i] = fits ? (int32_t)w : 0;
-                            const bool f16 = w >= 0.0 && w <= 65535.0;
-                            not16 |= !f16;
-                            if (w16) w16[i] = f16 ? (uint16_t)w : (uint16_t)0;
-                        } else {
-                            not32 = 1;
-                            not16 = 1;
-                        }
-                    } else {
-                        const int64_t wi = (int64_t)w;
-                        su += (uint64_t)wi;
-                        const double aw = fabs((double)wi);
-                        sf += (double)wi;
-                        sa += aw;
-                        ma = aw > ma ? aw : ma;
-                        const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
-                        not32 |= !fits;
-                        if (w32) w32[i] = (int32_t)wi;
-                        const bool f16 = wi >= 0 && wi <= 65535;
-                        not16 |= !f16;
-                        if (w16) w16[i] = (uint16_t)wi;
-                    }
+                    ws.add(wv[k], i, w32, w16);
                 }
             }
             __syncthreads();
         }
     }
-    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, su, sf, s_d, s_u);
+    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+    ws.flush_flags(out);
+}
+
+// The prologue of a repartition that starts from a resident working copy
+// (SoA f32 coordinates in input order, weights as given): the per-instance
+// bounding box of the f32 coordinates (the f32 cast is monotone, so its
+// extremes are the f32 casts of the f64 extremes coupe takes,
+// geometry.rs:42-76), the weight sums and flags, and the weight copies the
+// expected mode reads.  Nothing is written to the coordinates.
+template <int D, typename WIN>
+__global__ void __launch_bounds__(kBlock)
+k_prologue_soa(const Tile *__restrict__ tiles, uint32_t n_tiles, const float *__restrict__ c0,
+               const float *__restrict__ c1, const float *__restrict__ c2, const WIN *__restrict__ win,
+               int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out) {
+    __shared__ double s_d[kBlock / 64 * 8];
+    __shared__ uint64_t s_u[kBlock / 64];
+    double mn[D], mx[D];
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        sa += __shfl_xor(sa, o, 64);
-        const double m2 = __shfl_xor(ma, o, 64);
-        ma = m2 > ma ? m2 : ma;
-        nonint |= __shfl_xor(nonint, o, 64);
-        not32 |= __shfl_xor(not32, o, 64);
-        not16 |= __shfl_xor(not16, o, 64);
+    for (int k = 0; k < D; k++) { mn[k] = 1.7976931348623157e308; mx[k] = -1.7976931348623157e308; }
+    WeightScan ws;
+    uint32_t cur = 0xffffffffu;
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const Tile T = tiles[t];
+        if (T.part != cur) {
+            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+            cur = T.part;
+        }
+        for (uint32_t p = threadIdx.x; p < T.count; p += kBlock) {
+            const uint64_t i = T.begin + p;
+            const float x[3] = {c0[i], c1[i], D > 2 ? c2[i] : 0.0f};
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                const double v = (double)x[a];
+                if (v < mn[a]) mn[a] = v;
+                if (mx[a] < v) mx[a] = v;
+            }
+            ws.add(win[i], i, w32, w16);
+        }
     }
-    if (lane_id() == 0) {
-        atomicAdd(&out->sum_abs, sa);
-        atomicMax((unsigned long long *)&out->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, ma));
-        if (nonint) atomicOr(&out->non_integral, 1u);
-        if (not32) atomicOr(&out->not_i32, 1u);
-        if (not16) atomicOr(&out->not_u16, 1u);
+    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+    ws.flush_flags(out);
+}
+
+// Working copy of a resident mesh: points [first, first + count) as f32 SoA
+// (`as f32`, :677) and their weights, from f64 AoS points and weights given
+// for that range.  Null `pts` or `w` leaves that array unchanged.
+template <int D, typename WIN>
+__global__ void __launch_bounds__(kBlock)
+k_mesh_store(const double *__restrict__ pts, const WIN *__restrict__ w, uint64_t first, uint64_t count,
+             float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2, WIN *__restrict__ mw) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < count; j += stride) {
+        const uint64_t i = first + j;
+        if (pts) {
+            c0[i] = (float)pts[j * D];
+            c1[i] = (float)pts[j * D + 1];
+            if (D > 2) c2[i] = (float)pts[j * D + 2];
+        }
+        if (w) mw[i] = w[j];
     }
 }
 
-// Weights stored as int64 (integer-valued, beyond int32) or kept as double.
+// Weights stored as int64 (integer-valued, beyond int32) or kept as double,
+// at the working positions of the level-0 tiles (from each instance's range
+// of the caller's weights).
 template <typename WIN, typename WS>
 __global__ void __launch_bounds__(kBlock)
-k_store_weights(const WIN *__restrict__ win, WS *__restrict__ ws, uint64_t n) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        ws[i] = (WS)win[i];
+k_store_weights(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance *__restrict__ insts,
+                const WIN *__restrict__ win, WS *__restrict__ ws) {
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const Tile T = tiles[t];
+        const Instance &I = insts[T.part];
+        const WIN *src = win + I.src + (T.begin - I.begin);
+        for (uint32_t p = threadIdx.x; p < T.count; p += kBlock) ws[T.begin + p] = (WS)src[p];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -878,8 +969,14 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
 }
 
 // Empty per-instance reductions before the prologue.
-__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst) {
+// Also empties the call's prologue flags (block 0).
+__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst, Prologue *pro) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q == 0) {
+        Prologue z;
+        memset(&z, 0, sizeof z);
+        *pro = z;
+    }
     if (q >= n_inst) return;
     Instance &I = insts[q];
     for (int k = 0; k < kMaxDim; k++) {
@@ -934,8 +1031,13 @@ k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restri
 // prologue's per-instance reductions; the id offset starts at "none".
 __global__ void __launch_bounds__(kBlock)
 k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts, int dim,
-                 int real_mode, uint32_t *fwd_cnt, uint64_t *fwd_w, uint32_t *fwd_mn) {
+                 int real_mode, uint32_t *fwd_cnt, uint64_t *fwd_w, uint32_t *fwd_mn, uint32_t *n_parts0,
+                 uint32_t *n_tiles0, uint32_t tiles0) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q == 0) {   // level 0's table sizes: one root per instance, the level-0 tiles
+        *n_parts0 = n_inst;
+        *n_tiles0 = tiles0;
+    }
     if (q >= n_inst) return;
     if (fwd_cnt) {   // the roots' forwarded sums start empty (any accumulator: zero bits)
         fwd_cnt[2 * q] = fwd_cnt[2 * q + 1] = 0;
@@ -1350,6 +1452,7 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
     __shared__ float cache[(1u << kDenseCached) - 1];
     uint32_t cur = 0xffffffffu, levels = 0, cached_levels = 0;
     uint64_t hb = 0, off = 0;
+    uint64_t src = 0, ibeg = 0;   // the instance's ids start at out[src]
     for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {             // level-0 tiles: part index = instance index
@@ -1359,6 +1462,8 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
             levels = I.levels;
             hb = I.heap_base;
             off = I.min_id;
+            src = I.src;
+            ibeg = I.begin;
             cached_levels = levels < (uint32_t)kDenseCached ? levels : (uint32_t)kDenseCached;
             const uint32_t cached = (1u << cached_levels) - 1;
             for (uint32_t j = threadIdx.x; j < cached; j += kBlock) cache[j] = heap_split[hb + j];
@@ -1408,9 +1513,16 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const uint32_t p = k * kBlock + threadIdx.x;
-            if (p < T.count) out[T.begin + p] = (uint64_t)h[k] - off;
+            if (p < T.count) out[src + (T.begin - ibeg) + p] = (uint64_t)h[k] - off;
         }
     }
+}
+
+// Fills n 32-bit words (clears that a captured graph can replay, where a
+// memset node is avoided).
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t *out, uint64_t n, uint32_t v) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = v;
 }
 
 // iter_count == 0: every point is in part 0.

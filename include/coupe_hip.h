@@ -75,8 +75,8 @@ enum coupe_err coupe_hip_rcb_device_noalloc(void *stream, uint64_t *d_partition,
  * d_partition, in the order given.  Instance q is partitioned exactly as
  * coupe_hip_rcb_device would partition it alone, with its own iter_count and
  * tolerance: its ids start at 0 and are written at its own offsets.
- * All instances share the dimension and the weight type (split a mixed batch
- * by dimension, one call each).  Every instance of a batch of more than one
+ * All instances share the dimension and the weight type (a batch of both
+ * dimensions: coupe_hip_rcb_batch_mixed).  Every instance of a batch of more than one
  * needs iter_count <= 24 (else COUPE_ERR_BAD_TYPE).
  * No counterpart in coupe: it replaces a loop of Rcb::partition calls
  * (coupe/src/algorithms/recursive_bisection.rs:805-811), one per instance.
@@ -89,6 +89,27 @@ struct coupe_hip_instance {
 enum coupe_err coupe_hip_rcb_batch(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,
 		uintptr_t dimension, const double *d_points, enum coupe_type weight_type,
 		const void *d_weights, const struct coupe_hip_instance *instances, uintptr_t n_instances);
+
+/*
+ * Batched RCB of instances of dimension 2 and 3 in one call: instance q has
+ * instances[q].n points of instances[q].dimension doubles each, packed one
+ * instance after the other in d_points (so the 2D and 3D instances may
+ * alternate), its weights and ids at the same point positions of d_weights
+ * and d_partition.  Each instance is partitioned exactly as
+ * coupe_hip_rcb_device would partition it alone.  A call of more than one
+ * instance needs every iter_count <= 24 (else COUPE_ERR_BAD_TYPE).
+ * No counterpart in coupe (a loop of Rcb::partition calls over instances of
+ * both dimensions).
+ */
+struct coupe_hip_instance_nd {
+	uintptr_t n;            /* points of this instance */
+	uintptr_t dimension;    /* 2 or 3 */
+	uintptr_t iter_count;   /* Rcb::iter_count */
+	double tolerance;       /* Rcb::tolerance */
+};
+enum coupe_err coupe_hip_rcb_batch_mixed(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,
+		const double *d_points, enum coupe_type weight_type, const void *d_weights,
+		const struct coupe_hip_instance_nd *instances, uintptr_t n_instances);
 
 /*
  * coupe::Rib on device-resident data: same arguments and errors as
@@ -169,24 +190,35 @@ enum coupe_err coupe_hip_imbalance(coupe_hip_ctx *ctx, void *stream, const uint6
 		double *imbalance);
 
 /*
- * Serving: a request server with continuous batching.
+ * Serving: a request server with continuous batching and resident meshes.
  *
  * `slots` workers each own a context and a stream.  A free worker takes the
- * queued requests (same dimension and weight type, in arrival order) that
- * fit its batch limits and the device-memory budget, packs them, runs them
- * as one batch, and copies every request's ids to its output; requests that
- * arrive while batches run are admitted as slots and memory free up, and
+ * oldest queued request that can run and, unless it must run alone, the
+ * queued requests of the same dimension and weight type (in arrival order)
+ * that fit its batch limits and the memory budget; it packs them, runs them
+ * as one batch, and copies every request's ids to its output.  Requests that
+ * arrive while batches run are admitted as workers and memory free up, and
  * batches of different workers overlap on the device.  A request's ids are
  * exactly those of coupe_hip_rcb_device on the same input.
- * Inputs may be host or device pointers (a mesh kept resident in HBM is
- * submitted with device_input = 1); they must stay valid until the request
- * is waited for.  No counterpart in coupe.
+ *
+ * Runs alone: a request on a resident mesh, and a request with
+ * iter_count > 24 (a batch of several instances needs iter_count <= 24).
+ *
+ * Device memory is admitted against `memory_budget` from live figures: the
+ * buffers every worker holds now plus the resident meshes; a batch that
+ * does not fit waits, and idle workers give their buffers back meanwhile.
+ *
+ * Inputs may be host or device pointers.  Device inputs written by work on
+ * a stream are ordered by passing that stream in the request: the worker
+ * waits for that stream's work up to the submit (NULL: the caller has
+ * synchronised).  Inputs and outputs must stay valid until the request is
+ * waited for.  No counterpart in coupe.
  */
 typedef struct coupe_hip_server coupe_hip_server;
 struct coupe_hip_server_config {
 	int device;
 	uint32_t slots;               /* concurrent batches (streams), >= 1 */
-	uint64_t memory_budget;       /* device bytes the workers may hold together */
+	uint64_t memory_budget;       /* device bytes the workers and the meshes may hold together */
 	uint64_t max_batch_points;    /* points packed into one batch */
 	uint32_t max_batch_requests;  /* requests packed into one batch */
 };
@@ -198,9 +230,12 @@ struct coupe_hip_request {
 	uintptr_t dimension;
 	uintptr_t iter_count;
 	double tolerance;
-	uint64_t *partition;          /* n ids */
+	uint64_t *partition;          /* n ids (may be NULL for a mesh request: ids kept on the mesh only) */
 	int device_input;             /* points and weights are device pointers */
 	int device_output;            /* partition is a device pointer */
+	void *stream;                 /* device inputs: the stream that produced them, or NULL */
+	uint64_t mesh;                /* 0, or a resident mesh: points, weights, n, dimension and
+	                                 weight_type are then the mesh's */
 };
 struct coupe_hip_request_stats {
 	double queue_ms;              /* submit -> batch start */
@@ -210,14 +245,61 @@ struct coupe_hip_request_stats {
 	uint64_t batch_points;        /* points in its batch */
 };
 coupe_hip_server *coupe_hip_server_create(const struct coupe_hip_server_config *config);
-/* Waits for every submitted request, then frees the server. */
+/* Runs every submitted request, waits for the waiters to leave, then frees the server. */
 void coupe_hip_server_destroy(coupe_hip_server *server);
-/* COUPE_ERR_ALLOC when the request alone exceeds the budget or a batch limit. */
+/* COUPE_ERR_ALLOC when the request alone cannot fit the budget next to the
+ * resident meshes or exceeds max_batch_points; COUPE_ERR_NOT_FOUND for an
+ * unknown mesh. */
 enum coupe_err coupe_hip_server_submit(coupe_hip_server *server, const struct coupe_hip_request *request,
 		uint64_t *ticket);
-/* Blocks until the request is done; returns its status (each ticket once). */
+/* Blocks until the request is done; returns its status.  Each ticket is
+ * waited for once: another wait on it returns COUPE_ERR_NOT_FOUND. */
 enum coupe_err coupe_hip_server_wait(coupe_hip_server *server, uint64_t ticket,
 		struct coupe_hip_request_stats *stats);
+/* *done = 1 when the request is done (it must still be waited for). */
+enum coupe_err coupe_hip_server_poll(coupe_hip_server *server, uint64_t ticket, int *done);
+/* A request still queued is removed (*cancelled = 1; its ticket is then
+ * unknown); one that started completes as usual (*cancelled = 0). */
+enum coupe_err coupe_hip_server_cancel(coupe_hip_server *server, uint64_t ticket, int *cancelled);
+struct coupe_hip_server_stats {
+	uint64_t memory_budget;
+	uint64_t device_bytes;        /* held now by the workers and the meshes */
+	uint64_t mesh_bytes;          /* held by the resident meshes */
+	uint64_t peak_device_bytes;   /* largest device_bytes seen */
+	uint32_t queued, running, meshes;
+	uint64_t completed, cancelled;
+};
+enum coupe_err coupe_hip_server_stats(coupe_hip_server *server, struct coupe_hip_server_stats *stats);
+
+/*
+ * Resident meshes.  A mesh keeps its working copy in HBM: the f32
+ * coordinates of its points (`as f32`, as coupe's rcb() takes them) and its
+ * weights, in the caller's point order, plus the ids of its last partition.
+ * A request with `mesh` set partitions that working copy with no input
+ * copy, starting from what the mesh's previous partition taught the engine
+ * (weight mode, split-search passes per level); its ids equal those of a
+ * fresh coupe_hip_rcb_device call on the mesh's points.  Updates apply after
+ * the mesh's queued and running requests and before later ones.
+ */
+/* A mesh of n points (f64 AoS) and weights, from host or device memory. */
+enum coupe_err coupe_hip_mesh_create(coupe_hip_server *server, const double *points, const void *weights,
+		enum coupe_type weight_type, uintptr_t n, uintptr_t dimension, int device_input, void *stream,
+		uint64_t *mesh);
+/* Points [first, first + count) and/or their weights replaced in place
+ * (either may be NULL). */
+enum coupe_err coupe_hip_mesh_update(coupe_hip_server *server, uint64_t mesh, uintptr_t first, uintptr_t count,
+		const double *points, const void *weights, int device_input, void *stream);
+/* The points of the mesh's last partition grouped by part (the order of
+ * coupe's permuted working set after rcb(), recursive_bisection.rs:122-181,
+ * made stable): perm (n entries) lists part 0's point indices in increasing
+ * order, then part 1's, ...; offsets (num_parts + 1 entries) bound each
+ * part's range.  Kept with the mesh until its next partition or update.
+ * perm / offsets may be NULL (then only *num_parts is returned);
+ * COUPE_ERR_LEN_MISMATCH when offsets_len < num_parts + 1. */
+enum coupe_err coupe_hip_mesh_parts(coupe_hip_server *server, uint64_t mesh, uint32_t *perm, uint64_t *offsets,
+		uintptr_t offsets_len, uint64_t *num_parts, int device_output);
+/* Waits for the mesh's requests, then frees it. */
+enum coupe_err coupe_hip_mesh_destroy(coupe_hip_server *server, uint64_t mesh);
 
 /*
  * Counters of the last call on a context (for benchmarks and tests).
@@ -236,6 +318,8 @@ struct coupe_hip_stats {
 	uint64_t first_read;      /* points read by the children's first passes (two levels per move) */
 	uint32_t forwarded;       /* parts whose children's first pass came with the move that filled them */
 	uint32_t full_first;      /* levels whose children's first pass took every candidate (last pair, adaptive) */
+	uint32_t graph;           /* 0: launched kernel by kernel; 1: one launch of the graph captured from an
+	                             earlier call of the same shape; 2: captured by this call, then launched */
 };
 enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out);
 

@@ -29,41 +29,55 @@ def oracle_ids(oracle, pts, w, iters, tol):
 
 def batch_workload(count=4096, seed=3):
     """The batched workload: sizes log-uniform in [10^3, 10^6], D in {2, 3},
-    a random box per instance, weights 1..16, iter_count 4..8."""
+    a random box per instance, weights 1..16, iter_count 4..8.  Returns the
+    instances in generator order with their points flat (each instance's
+    n * D coordinates after the previous instance's)."""
     n, dim, lo, ext, iters = workloads.batch_header(count, seed=seed, lg_lo=3.0, lg_hi=6.0, l_lo=4, l_hi=8)
-    groups = {}
-    for d in (2, 3):
-        idx = [q for q in range(count) if dim[q] == d]
-        pts, ws = [], []
-        for q in idx:
-            p, w = workloads.box(int(n[q]), d, lo[q, :d], ext[q, :d], workloads.instance_seed(seed, q), 16)
-            pts.append(p)
-            ws.append(w)
-        groups[d] = (idx, np.concatenate(pts), np.concatenate(ws))
-    return n, iters, groups
+    pts, ws = [], []
+    for q in range(count):
+        d = int(dim[q])
+        p, w = workloads.box(int(n[q]), d, lo[q, :d], ext[q, :d], workloads.instance_seed(seed, q), 16)
+        pts.append(p.ravel())
+        ws.append(w)
+    return n, dim, iters, np.concatenate(pts), np.concatenate(ws)
 
 
 @pytest.mark.slow
 def test_batched_workload_full(oracle_mod):
+    """The 4096 instances as one call of both dimensions, every instance
+    against the oracle; then one call per dimension gives the same ids."""
     torch = pytest.importorskip("torch")
     tol = 0.02
-    n, iters, groups = batch_workload()
-    for d, (idx, pts, w) in groups.items():
-        say(f"batched workload, D={d}: {len(idx)} instances, {pts.shape[0]} points")
-        out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
-        specs = [(int(n[q]), int(iters[q]), tol) for q in idx]
-        coupe_amd.partition_batch(out, torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(), specs)
+    n, dim, iters, flat, w = batch_workload()
+    specs = [(int(n[q]), int(dim[q]), int(iters[q]), tol) for q in range(len(n))]
+    say(f"batched workload: {len(specs)} instances of D 2 and 3, {w.shape[0]} points, one call")
+    out = torch.zeros(w.shape[0], dtype=torch.int64, device="cuda")
+    d_flat, d_w = torch.from_numpy(flat).cuda(), torch.from_numpy(w).cuda()
+    coupe_amd.partition_batch_mixed(out, d_flat, d_w, specs)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().astype(np.uint64)
+    off = poff = 0
+    starts = []
+    for q, (k, d, it, _) in enumerate(specs):
+        starts.append((off, poff))
+        p = flat[poff:poff + k * d].reshape(k, d)
+        exp = oracle_ids(oracle_mod, p, w[off:off + k], it, tol)
+        if not np.array_equal(got[off:off + k], exp):
+            bad = int(np.count_nonzero(got[off:off + k] != exp))
+            raise AssertionError(f"instance {q} (D={d}, n={k}, L={it}): {bad} ids differ")
+        off += k
+        poff += k * d
+    say("batched workload: every instance equals the oracle")
+    for d in (2, 3):
+        idx = [q for q in range(len(specs)) if specs[q][1] == d]
+        p = np.concatenate([flat[starts[q][1]:starts[q][1] + specs[q][0] * d] for q in idx]).reshape(-1, d)
+        ww = np.concatenate([w[starts[q][0]:starts[q][0] + specs[q][0]] for q in idx])
+        o = torch.zeros(ww.shape[0], dtype=torch.int64, device="cuda")
+        coupe_amd.partition_batch(o, torch.from_numpy(p).cuda(), torch.from_numpy(ww).cuda(),
+                                  [(specs[q][0], specs[q][2], tol) for q in idx])
         torch.cuda.synchronize()
-        got = out.cpu().numpy().astype(np.uint64)
-        del out
-        off = 0
-        for q, (k, it, _) in zip(idx, specs):
-            exp = oracle_ids(oracle_mod, pts[off:off + k], w[off:off + k], it, tol)
-            if not np.array_equal(got[off:off + k], exp):
-                bad = int(np.count_nonzero(got[off:off + k] != exp))
-                raise AssertionError(f"instance {q} (D={d}, n={k}, L={it}): {bad} ids differ")
-            off += k
-        say(f"batched workload, D={d}: every instance equals the oracle")
+        exp = np.concatenate([got[starts[q][0]:starts[q][0] + specs[q][0]] for q in idx])
+        assert np.array_equal(o.cpu().numpy().astype(np.uint64), exp), f"one call per dimension, D={d}"
 
 
 @pytest.mark.slow

@@ -1,0 +1,121 @@
+"""Calls replayed as captured graphs: a context captures a call of a shape it
+has run before (hints known) and replays the graph for later calls of that
+shape.  Every replay must give the eager path's ids -- the data is read at
+replay time, a weight mode the graph cannot hold or a search longer than its
+queued passes falls back to eager runs -- with eager work and
+synchronisations of the program between the replays."""
+import sys
+
+import numpy as np
+import pytest
+
+import coupe_amd
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _ids(oracle, pts, w, iters, tol):
+    err, ids, _ = oracle.rcb(pts, w, iters, tol)
+    assert err == 0
+    return ids
+
+
+def test_replays_read_the_current_data(oracle_mod):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(70)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        n, iters, tol = 300_000, 10, 0.01
+        d_pts = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+        d_w = torch.empty(n, dtype=torch.float64, device="cuda")
+        out = torch.empty(n, dtype=torch.int64, device="cuda")
+        seen = []
+        # data kinds: uniform, clustered (longer searches), weights past 16
+        # bits (another weight mode), then uniform again
+        kinds = ["uniform", "uniform", "uniform", "cluster", "uniform", "wide", "wide", "uniform", "cluster"]
+        for q, kind in enumerate(kinds):
+            pts = rng.random((n, 3)) if kind != "cluster" else rng.normal(0, 1, (n, 3)) ** 5
+            w = (rng.integers(1, 100, n) if kind != "wide" else rng.integers(1, 1 << 30, n)).astype(np.float64)
+            d_pts.copy_(torch.from_numpy(pts))
+            d_w.copy_(torch.from_numpy(w))
+            torch.cuda.synchronize()
+            out.fill_(-1)   # the program's own eager work between replays
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(out, (d_pts, d_w), context=ctx)
+            torch.cuda.synchronize()
+            st = ctx.stats()
+            seen.append(st["graph"])
+            say(f"graph replay call {q} ({kind}): graph {st['graph']} reruns {st['reruns']}")
+            got = out.cpu().numpy().astype(np.uint64)
+            assert np.array_equal(got, _ids(oracle_mod, pts, w, iters, tol)), f"call {q} ({kind})"
+        assert seen[0] == 0 and 2 in seen and 1 in seen, seen
+    finally:
+        ctx.close()
+
+
+def test_shapes_alternating_on_one_context(oracle_mod):
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(71)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        shapes = [(100_000, 2, 7, 0.05), (250_000, 3, 12, 0.01), (4_096, 3, 5, 0.0)]
+        data = []
+        for n, d, it, tol in shapes:
+            pts = rng.random((n, d))
+            w = rng.integers(1, 1000, n).astype(np.int64)
+            data.append((torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(),
+                         torch.empty(n, dtype=torch.int64, device="cuda"), _ids(oracle_mod, pts, w, it, tol)))
+        graphs = []
+        for rep in range(4):
+            for k, (n, d, it, tol) in enumerate(shapes):
+                dp, dw, out, exp = data[k]
+                out.fill_(-1)
+                coupe_amd.Rcb(iter_count=it, tolerance=tol).partition(out, (dp, dw), context=ctx)
+                torch.cuda.synchronize()
+                graphs.append(ctx.stats()["graph"])
+                assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"rep {rep} shape {k}"
+        assert graphs.count(1) >= 6, graphs
+    finally:
+        ctx.close()
+
+
+def test_batches_replayed(oracle_mod, monkeypatch):
+    """Batches of both dimensions, each call replayed; COUPE_HIP_GRAPHS=0
+    gives the same ids kernel by kernel."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(72)
+    specs, flat, ws = [], [], []
+    for q in range(80):
+        d = 2 + q % 2
+        n = int(10 ** rng.uniform(2, 4.5))
+        flat.append(rng.random((n, d)).ravel())
+        ws.append(rng.integers(1, 16, n).astype(np.float64))
+        specs.append((n, d, int(rng.integers(4, 9)), 0.02))
+    flat, w = np.concatenate(flat), np.concatenate(ws)
+    d_flat, d_w = torch.from_numpy(flat).cuda(), torch.from_numpy(w).cuda()
+    expect = []
+    off = poff = 0
+    for n, d, it, tol in specs:
+        expect.append(_ids(oracle_mod, flat[poff:poff + n * d].reshape(n, d), w[off:off + n], it, tol))
+        off += n
+        poff += n * d
+    expect = np.concatenate(expect)
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("COUPE_HIP_GRAPHS", graphs)
+        ctx = coupe_amd.DeviceContext(0)
+        try:
+            out = torch.empty(w.shape[0], dtype=torch.int64, device="cuda")
+            for rep in range(4):
+                out.fill_(-1)
+                coupe_amd.partition_batch_mixed(out, d_flat, d_w, specs, context=ctx)
+                torch.cuda.synchronize()
+                say(f"mixed batch, graphs {graphs}, call {rep}: graph {ctx.stats()['graph']}")
+                assert np.array_equal(out.cpu().numpy().astype(np.uint64), expect), f"graphs {graphs} call {rep}"
+                if graphs == "0":
+                    assert ctx.stats()["graph"] == 0
+        finally:
+            ctx.close()

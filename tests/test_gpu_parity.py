@@ -498,50 +498,28 @@ def test_batch_of_one_equals_plain_call():
     assert np.array_equal(out.cpu().numpy().astype(np.uint64), plain)
 
 
-# ---------------------------------------------------------------------------
-# Serving (coupe_hip_server_*): each result equals the lone call, in the mix.
-# ---------------------------------------------------------------------------
-def test_server_results_equal_lone_calls():
+@pytest.mark.parametrize("seed", [0, 1])
+def test_batch_of_both_dimensions(oracle_mod, seed):
+    """coupe_hip_rcb_batch_mixed: 2D and 3D instances alternating in one call."""
     torch = pytest.importorskip("torch")
-    rng = np.random.default_rng(21)
-    reqs = []
-    for q in range(60):
-        n = int(10 ** rng.uniform(2, 5.3))
-        dim = 3 if q % 4 else 2
-        pts = rng.random((n, dim))
-        w = rng.integers(1, 100, n).astype(np.float64)
-        reqs.append((pts, w, int(rng.integers(3, 11)), float(rng.choice([0.01, 0.05]))))
-    say(f"server: {len(reqs)} requests")
-    srv = coupe_amd.Server(slots=3, memory_budget=8 << 30, max_batch_requests=16)
-    try:
-        outs, tickets = [], []
-        for q, (pts, w, it, tol) in enumerate(reqs):
-            if q % 3 == 0:   # a mesh resident in HBM, ids to device memory
-                out = torch.zeros(pts.shape[0], dtype=torch.int64, device="cuda")
-                t = srv.submit(out, torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(), it, tol)
-            else:            # host buffers
-                out = np.zeros(pts.shape[0], dtype=np.uint64)
-                t = srv.submit(out, pts, w, it, tol)
-            outs.append(out)
-            tickets.append(t)
-        stats = [srv.wait(t) for t in tickets]
-    finally:
-        srv.close()
-    assert max(s["batch_requests"] for s in stats) > 1   # requests were batched
-    for q, ((pts, w, it, tol), out) in enumerate(zip(reqs, outs)):
-        got = out.cpu().numpy().astype(np.uint64) if hasattr(out, "cpu") else out
-        assert np.array_equal(got, gpu_rcb(pts, w, it, tol)), f"request {q}"
-
-
-def test_server_rejects_oversized_and_bad_requests():
-    srv = coupe_amd.Server(slots=1, memory_budget=1 << 20)
-    try:
-        pts = np.zeros((100_000, 3))
-        with pytest.raises(coupe_amd.CoupeError) as e:
-            srv.submit(np.zeros(100_000, dtype=np.uint64), pts, np.ones(100_000), 4, 0.05)
-        assert e.value.code == _native.ALLOC
-        with pytest.raises(coupe_amd.CoupeError) as e:
-            srv.submit(np.zeros(10, dtype=np.uint64), np.zeros((10, 4)), np.ones(10), 4, 0.05)
-        assert e.value.code == _native.BAD_DIMENSION
-    finally:
-        srv.close()
+    rng = np.random.default_rng(60 + seed)
+    specs, flat, ws = [], [], []
+    for q in range(50):
+        d = int(rng.integers(2, 4))
+        n = int(rng.choice([0, 1, 2, 17, 300, 2048, 2049, 5000, 40000]))
+        p = rng.random((n, d)) if q % 3 else rng.normal(0, 1, (n, d)) ** 3
+        flat.append(p.ravel())
+        ws.append(rng.integers(1, 50, n).astype([np.int64, np.float64, np.int32][seed]))
+        specs.append((n, d, int(rng.integers(0, 11)), float(rng.choice([0.0, 0.02, 0.05]))))
+    flat, w = np.concatenate(flat), np.concatenate(ws)
+    say(f"mixed batch seed {seed}: {len(specs)} instances, {w.shape[0]} points")
+    out = torch.full((w.shape[0],), -1, dtype=torch.int64, device="cuda")
+    coupe_amd.partition_batch_mixed(out, torch.from_numpy(flat).cuda(), torch.from_numpy(w).cuda(), specs)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().astype(np.uint64)
+    off = poff = 0
+    for q, (n, d, iters, tol) in enumerate(specs):
+        exp, _ = oracle_rcb(oracle_mod, flat[poff:poff + n * d].reshape(n, d), w[off:off + n], iters, tol)
+        assert np.array_equal(got[off:off + n], exp), f"instance {q}: n={n} D={d} L={iters} tol={tol}"
+        off += n
+        poff += n * d
