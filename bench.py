@@ -513,52 +513,52 @@ def bench_grid(torch, coupe_amd, with_cpu=True, side=10000, iters=12, threads=16
 
 
 def bench_batched(torch, coupe_amd, workloads, count=4096, seed=3):
-    """Batched RCB versus per-instance launches on the same inputs."""
+    """Batched RCB versus per-instance launches on the same inputs: the 4096
+    instances of both dimensions as one coupe_hip_rcb_batch_mixed call (after
+    the autotuner chose the switches for the batch's class; calls of a known
+    shape run as one captured graph) against one coupe_hip_rcb_device call
+    per instance."""
     t0 = time.time()
     n, dim, lo, ext, iters = workloads.batch_header(count, seed=seed, lg_lo=3.0, lg_hi=6.0, l_lo=4, l_hi=8)
     tol = 0.02
-    groups = {}
-    for d in (2, 3):
-        idx = [q for q in range(count) if dim[q] == d]
-        pts, ws = [], []
-        for q in idx:
-            p, w = workloads.box(int(n[q]), d, lo[q, :d], ext[q, :d], workloads.instance_seed(seed, q), 16)
-            pts.append(p)
-            ws.append(w)
-        groups[d] = (idx, np.concatenate(pts), np.concatenate(ws))
+    flat, ws = [], []
+    for q in range(count):
+        d = int(dim[q])
+        p, w = workloads.box(int(n[q]), d, lo[q, :d], ext[q, :d], workloads.instance_seed(seed, q), 16)
+        flat.append(p.ravel())
+        ws.append(w)
+    flat, ws = np.concatenate(flat), np.concatenate(ws)
+    specs = [(int(n[q]), int(dim[q]), int(iters[q]), tol) for q in range(count)]
     total = int(n.sum())
     log(f"batched: {count} instances, {total} points generated in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     ctx = coupe_amd.DeviceContext(0)
-    res = {}
-    dev_data = {}
-    for d, (idx, pts, ws) in groups.items():
-        dev_data[d] = (torch.from_numpy(pts).to(dev), torch.from_numpy(ws).to(dev),
-                       torch.zeros(pts.shape[0], dtype=torch.int64, device=dev),
-                       torch.zeros(pts.shape[0], dtype=torch.int64, device=dev))
+    d_flat, d_w = torch.from_numpy(flat).to(dev), torch.from_numpy(ws).to(dev)
+    out_b = torch.zeros(total, dtype=torch.int64, device=dev)
+    out_p = torch.zeros(total, dtype=torch.int64, device=dev)
     torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tuning = coupe_amd.autotune(out_b, d_flat, d_w, specs, trials=2, context=ctx, stream=stream.cuda_stream)
+    log(f"autotune: {tuning} in {time.perf_counter() - t1:.1f}s")
 
     def run_batched():
-        for d, (idx, _, _) in groups.items():
-            dp, dw, out, _ = dev_data[d]
-            specs = [(int(n[q]), int(iters[q]), tol) for q in idx]
-            coupe_amd.partition_batch(out, dp, dw, specs, context=ctx, stream=stream.cuda_stream)
+        coupe_amd.partition_batch_mixed(out_b, d_flat, d_w, specs, context=ctx, stream=stream.cuda_stream)
+
+    offs = np.concatenate([[0], np.cumsum([k * d for k, d, _, _ in specs])])
+    poffs = np.concatenate([[0], np.cumsum([k for k, _, _, _ in specs])])
 
     def run_plain():
-        for d, (idx, _, _) in groups.items():
-            dp, dw, _, out = dev_data[d]
-            off = 0
-            for q in idx:
-                k = int(n[q])
-                coupe_amd.Rcb(int(iters[q]), tol).partition(out[off:off + k], (dp[off:off + k], dw[off:off + k]),
-                                                            context=ctx, stream=stream.cuda_stream)
-                off += k
+        for q, (k, d, it, _) in enumerate(specs):
+            o, po = int(offs[q]), int(poffs[q])
+            coupe_amd.Rcb(it, tol).partition(out_p[po:po + k], (d_flat[o:o + k * d].view(k, d), d_w[po:po + k]),
+                                             context=ctx, stream=stream.cuda_stream)
 
-    for fn, name in ((run_batched, "batched"), (run_plain, "plain")):
-        fn()   # warm-up (buffer growth)
+    res = {}
+    for fn, name, reps in ((run_batched, "batched", 5), (run_plain, "plain", 1)):
+        for _ in range(2):
+            fn()   # warm-up: buffers, hints, the graph of the batch
         torch.cuda.synchronize()
-        reps = 3 if name == "batched" else 1
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t1 = time.perf_counter()
         e0.record(stream)
@@ -569,16 +569,21 @@ def bench_batched(torch, coupe_amd, workloads, count=4096, seed=3):
         wall_ms = (time.perf_counter() - t1) * 1e3 / reps
         res[name] = {"ms": round(max(e0.elapsed_time(e1) / reps, wall_ms), 3)}
         res[name]["Mpts_s"] = round(total / (res[name]["ms"] * 1e-3) / 1e6, 1)
+        if name == "batched":
+            res[name]["stats"] = ctx.stats()
         log(f"batched bench: {name} {res[name]['ms']:.1f} ms, {res[name]['Mpts_s']} Mpts/s")
-    same = all(torch.equal(dev_data[d][2], dev_data[d][3]) for d in groups)
+    same = bool(torch.equal(out_b, out_p))
+    st = res["batched"]["stats"]
     ctx.close()
     return {"workload": f"{count} instances, sizes log-uniform in [1e3, 1e6], D in {{2,3}}, weights 1..16, "
-                        f"iter_count 4..8, tolerance {tol}, seed {seed}",
+                        f"iter_count 4..8, tolerance {tol}, seed {seed}; one call for both dimensions",
             "points": total, "value": res["batched"]["Mpts_s"], "unit": "Mpts/s",
             "batched_ms": res["batched"]["ms"], "per_instance_ms": res["plain"]["ms"],
             "per_instance_Mpts_s": res["plain"]["Mpts_s"],
             "speedup_vs_per_instance": round(res["plain"]["ms"] / res["batched"]["ms"], 2),
-            "ids_equal_to_per_instance": bool(same)}
+            "ids_equal_to_per_instance": same,
+            "graph": {0: "kernel by kernel", 1: "one graph launch (replay)", 2: "captured"}[st["graph"]],
+            "tuning": {k: v for k, v in tuning.items()}, "tuned": bool(st["tuned"])}
 
 
 def bench_served(torch, coupe_amd, workloads, n_requests, seed=4, rate=200.0, resident=8, pool=24,

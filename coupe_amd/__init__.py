@@ -28,7 +28,7 @@ from . import _native
 from ._native import strerror
 
 __all__ = ["Rcb", "Rib", "Grid", "Partition", "Error", "InputLenMismatch", "CoupeError", "DeviceContext", "strerror",
-           "partition_batch", "partition_batch_mixed", "Server", "Mesh"]
+           "partition_batch", "partition_batch_mixed", "autotune", "Server", "Mesh"]
 
 
 class Error(Exception):
@@ -327,6 +327,32 @@ def partition_batch_mixed(part_ids, points, weights, instances, *, stream=None, 
     if err != _native.OK:
         raise CoupeError(err)
     return None
+
+
+def autotune(part_ids, points, weights, instances, *, trials: int = 3, stream=None,
+             context: DeviceContext = None) -> dict:
+    """Chooses the engine's switches for a batch's class (coupe_hip_autotune,
+    include/coupe_hip.h): same arguments as :func:`partition_batch_mixed`;
+    later calls of that class on the context run with the switches kept.
+    Returns them with the fastest mean call time (``best_ms``)."""
+    import torch
+
+    specs = [(int(n), int(d), int(it), float(tol)) for n, d, it, tol in instances]
+    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
+    arr = (_native.InstanceND * max(len(specs), 1))()
+    for q, (n, d, it, tol) in enumerate(specs):
+        arr[q].n, arr[q].dimension, arr[q].iter_count, arr[q].tolerance = n, d, it, tol
+    ctx = context or _context_for(dev)
+    st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
+    chosen, ms = _native.Tuning(), C.c_double()
+    err = _native.lib().coupe_hip_autotune(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), points.data_ptr(),
+                                           _weight_type(weights.dtype), weights.data_ptr(), arr, len(specs),
+                                           int(trials), C.byref(chosen), C.byref(ms))
+    if err != _native.OK:
+        raise CoupeError(err)
+    out = chosen.as_dict()
+    out["best_ms"] = ms.value
+    return out
 
 
 class Server:

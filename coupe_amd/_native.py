@@ -25,7 +25,7 @@ EXPORTS = [
     "coupe_strerror", "coupe_data_free", "coupe_data_array", "coupe_data_constant", "coupe_data_fn",
     "coupe_rcb", "coupe_rib",
     "coupe_hip_ctx_create", "coupe_hip_ctx_destroy", "coupe_hip_rcb_device", "coupe_hip_rcb_device_noalloc",
-    "coupe_hip_rcb_batch", "coupe_hip_rcb_batch_mixed",
+    "coupe_hip_rcb_batch", "coupe_hip_rcb_batch_mixed", "coupe_hip_autotune",
     "coupe_hip_rib_device", "coupe_hip_last_rib", "coupe_hip_grid_rcb",
     "coupe_hip_barycentres", "coupe_hip_parts_load", "coupe_hip_imbalance",
     "coupe_hip_server_create", "coupe_hip_server_destroy", "coupe_hip_server_submit", "coupe_hip_server_wait",
@@ -86,6 +86,14 @@ class KernelTime(C.Structure):
 ITH_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t)
 
 
+class Tuning(C.Structure):
+    _fields_ = [("bins", C.c_uint32), ("workgroups_per_cu", C.c_uint32), ("search_passes", C.c_uint32),
+                ("full_first", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("points", C.c_uint64),
@@ -102,10 +110,12 @@ class Stats(C.Structure):
         ("forwarded", C.c_uint32),
         ("full_first", C.c_uint32),
         ("graph", C.c_uint32),
+        ("tuned", C.c_uint32),
+        ("tuning", Tuning),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return {k: (getattr(self, k).as_dict() if k == "tuning" else getattr(self, k)) for k, _ in self._fields_}
 
 
 _lib = None
@@ -158,6 +168,10 @@ def lib():
         L.coupe_hip_rcb_batch_mixed.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                                 C.POINTER(InstanceND), C.c_size_t]
         L.coupe_hip_rcb_batch_mixed.restype = C.c_int
+        L.coupe_hip_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.POINTER(InstanceND), C.c_size_t, C.c_uint32, C.POINTER(Tuning),
+                                         C.POINTER(C.c_double)]
+        L.coupe_hip_autotune.restype = C.c_int
         L.coupe_hip_grid_rcb.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.c_int,
                                          C.c_void_p, C.c_size_t, C.c_uint32, C.c_void_p]
         L.coupe_hip_grid_rcb.restype = C.c_int

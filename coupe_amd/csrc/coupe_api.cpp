@@ -255,6 +255,35 @@ extern "C" enum coupe_err coupe_hip_rcb_batch_mixed(coupe_hip_ctx *ctx, void *st
     }
 }
 
+extern "C" enum coupe_err coupe_hip_autotune(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition,
+                                             const double *d_points, enum coupe_type weight_type,
+                                             const void *d_weights, const struct coupe_hip_instance_nd *instances,
+                                             uintptr_t n_instances, uint32_t trials, struct coupe_hip_tuning *chosen,
+                                             double *best_ms) {
+    if (!ctx) return COUPE_ERR_CRASH;
+    if (n_instances > 0 && !instances) return COUPE_ERR_CRASH;
+    if (weight_type != COUPE_INT && weight_type != COUPE_INT64 && weight_type != COUPE_DOUBLE)
+        return COUPE_ERR_BAD_TYPE;
+    Context *c = reinterpret_cast<Context *>(ctx);
+    std::lock_guard<std::mutex> g(c->lock);
+    try {
+        std::vector<uint64_t> counts(n_instances);
+        std::vector<size_t> dims(n_instances), levels(n_instances);
+        std::vector<double> tols(n_instances);
+        for (uintptr_t q = 0; q < n_instances; q++) {
+            counts[q] = instances[q].n;
+            dims[q] = instances[q].dimension;
+            levels[q] = instances[q].iter_count;
+            tols[q] = instances[q].tolerance;
+        }
+        return coupe_hip::autotune(*c, (hipStream_t)stream, d_partition, d_points, weight_type, d_weights,
+                                   counts.data(), dims.data(), levels.data(), tols.data(), n_instances, trials,
+                                   chosen, best_ms);
+    } catch (const std::bad_alloc &) {
+        return COUPE_ERR_ALLOC;
+    }
+}
+
 extern "C" enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out) {
     if (!ctx || !out) return COUPE_ERR_CRASH;
     *out = reinterpret_cast<const Context *>(ctx)->stats;

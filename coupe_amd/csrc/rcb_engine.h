@@ -10,6 +10,7 @@
 
 #include "../../include/coupe.h"
 #include "../../include/coupe_hip.h"
+#include "rcb_common.h"
 
 namespace coupe_hip {
 
@@ -138,11 +139,17 @@ struct Context {
     bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
     uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
     bool graphs = true;              // replay a call of a known shape as one captured graph
+    int max_bins = kMaxBins;         // bins per part of the later search passes (16..256, a power of two)
+    int wg_cap = 0;                  // workgroups per CU of the persistent kernels (0: as many as fit)
+    // tuned switches per batch class (tune_key), chosen by coupe_hip_autotune
+    std::vector<std::pair<uint64_t, coupe_hip_tuning>> tuned;
+    bool tuning_now = false;         // the autotuner runs the call: the table is not applied
     bool capturing = false;          // the running call is being captured
     PendingCall pending;             // what the running call queued
     std::vector<GraphEntry> graph_cache;
     hipStream_t gstream = nullptr;   // where graphs are captured and launched
     hipEvent_t gevent = nullptr;     // orders gstream after the caller's stream
+    uint64_t last_key = 0;           // graph key of the last call that could have run as a graph
     uint64_t layout() ;              // hash of every buffer's address and size
     void drop_graphs();
     int weight_hint = -1;            // weight mode of the last call, for a shape not seen before
@@ -237,6 +244,24 @@ inline bool graphs_from_env() {
     return !(v && v[0] == '0');
 }
 
+// COUPE_HIP_BINS=b: bins per part of the later search passes (16..256,
+// rounded up to a power of two; default 256).
+inline int bins_from_env() {
+    const char *v = getenv("COUPE_HIP_BINS");
+    int b = v && *v ? atoi(v) : 256;
+    int p = 16;
+    while (p < 256 && p < b) p *= 2;
+    return p;
+}
+
+// COUPE_HIP_WG_PER_CU=k: at most k workgroups per CU for the persistent
+// kernels (default 0: as many as fit).
+inline int wg_cap_from_env() {
+    const char *v = getenv("COUPE_HIP_WG_PER_CU");
+    const int k = v && *v ? atoi(v) : 0;
+    return k < 0 ? 0 : (k > 64 ? 64 : k);
+}
+
 // COUPE_HIP_SCAN_DIRECT=b: a scan of more than b blocks scans the block
 // totals instead of re-summing them per block (default 2048, at least 1).
 inline uint64_t scan_direct_from_env() {
@@ -265,7 +290,19 @@ inline void configure_from_env(Context &c) {
     c.full_first = full_first_from_env();
     c.scan_direct = scan_direct_from_env();
     c.graphs = graphs_from_env();
+    c.max_bins = bins_from_env();
+    c.wg_cap = wg_cap_from_env();
 }
+
+// The context's switches as a tuning record, and back.
+coupe_hip_tuning current_tuning(const Context &cx);
+void apply_tuning(Context &cx, const coupe_hip_tuning &t);
+
+// Autotuner of a batch of both dimensions (see coupe_hip_autotune).
+enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const double *d_points, int wkind,
+                        const void *d_weights, const uint64_t *counts, const size_t *dims, const size_t *levels,
+                        const double *tols, size_t n_inst, uint32_t trials, coupe_hip_tuning *chosen,
+                        double *best_ms);
 
 // RCB of a batch of independent instances packed in input order (see
 // coupe_hip_rcb_batch).  counts / levels / tols: one entry per instance.

@@ -215,9 +215,9 @@ void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev
 }
 
 // Bins per part of the standalone LDS passes, bounded by the bin buffers.
-static int choose_bins(uint64_t parts_ub) {
+static int choose_bins(uint64_t parts_ub, int max_bins) {
     const uint64_t cap = 1ull << 23;   // entries of each bin array
-    int nb = kMaxBins;
+    int nb = max_bins;
     while (nb > kFirstBins && parts_ub * nb > cap) nb >>= 1;
     return nb;
 }
@@ -236,6 +236,7 @@ static unsigned persistent_grid(Context &cx, K kernel, uint64_t work_items) {
             per_cu = 2;
         cx.occupancy.emplace_back(key, per_cu);
     }
+    if (cx.wg_cap > 0 && per_cu > cx.wg_cap) per_cu = cx.wg_cap;
     uint64_t g = (uint64_t)cx.cu_count * (uint64_t)per_cu;
     if (g > work_items) g = work_items;
     return (unsigned)(g < 1 ? 1 : g);
@@ -413,7 +414,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // Bins: parts x bins of the widest pass of any level.
     uint64_t bins = 1;
     for (size_t l = 0; l < L; l++)
-        bins = std::max<uint64_t>(bins, std::max<uint64_t>(B.parts_ub[l], 1) * (uint64_t)choose_bins(B.parts_ub[l]));
+        bins = std::max<uint64_t>(bins, std::max<uint64_t>(B.parts_ub[l], 1) * (uint64_t)choose_bins(B.parts_ub[l], cx.max_bins));
     const bool fresh_bins = cx.bcnt.bytes < bins * 4 || cx.bw.bytes < bins * 8 || cx.bmin.bytes < bins * 4;
     cx.bcnt.ensure(bins * 4);
     cx.bw.ensure(bins * 8);
@@ -471,7 +472,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                           uint32_t counts_first, uint32_t counts_hist, auto &&hist) {
         Part *parts = (Part *)cx.parts[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
-        const int nb = choose_bins(parts_ub);
+        const int nb = choose_bins(parts_ub, cx.max_bins);
         cx.stats.bins = std::max<uint32_t>(cx.stats.bins, (uint32_t)nb);
         if (spec_passes > 0) {
             // slot p counts the parts still searching after decision p; pass
@@ -650,7 +651,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 cx.stats.full_first++;
                 // every child searching: the pass reads both axes and the weight
                 // of every point (counted with the first passes by the decision)
-                const int nbf = choose_bins(pub1);
+                const int nbf = choose_bins(pub1, cx.max_bins);
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist_v(nbf, nullptr); });
                 search_gen(ib, level + 1, pub1, nbf, &ctr->first_read, kCountsBins, kCountsBins, hist_v);
             } else {
@@ -1006,6 +1007,27 @@ static bool run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPla
     }
 }
 
+coupe_hip_tuning current_tuning(const Context &cx) {
+    return coupe_hip_tuning{(uint32_t)cx.max_bins, (uint32_t)cx.wg_cap, (uint32_t)cx.search_passes,
+                            cx.full_first ? 1u : 0u};
+}
+
+void apply_tuning(Context &cx, const coupe_hip_tuning &t) {
+    int b = kFirstBins * 4;   // a power of two in [16, kMaxBins]
+    while (b < kMaxBins && (uint32_t)b < t.bins) b *= 2;
+    cx.max_bins = b;
+    cx.wg_cap = (int)std::min<uint32_t>(t.workgroups_per_cu, 64);
+    cx.search_passes = (int)std::max<uint32_t>(2, std::min<uint32_t>(t.search_passes, 8));
+    cx.full_first = t.full_first != 0;
+}
+
+static uint64_t tune_key(const BatchPlan &B, size_t dim) {
+    const uint64_t lg_n = B.n ? 63 - __builtin_clzll(B.n) : 0;
+    uint64_t lg4_inst = 0;
+    for (uint64_t k = B.inst.size(); k > 1; k >>= 2) lg4_inst++;
+    return (uint64_t)dim | lg_n << 8 | (uint64_t)B.max_levels << 16 | lg4_inst << 32;
+}
+
 uint64_t Context::layout() {
     uint64_t h = 0xcbf29ce484222325ull;
     auto mix = [&](uint64_t v) {
@@ -1051,6 +1073,7 @@ static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int
     for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)cx.slot0[k]);
     mix((uint64_t)spec);
     mix((uint64_t)cx.pairs | (uint64_t)cx.forward << 1 | (uint64_t)cx.weights16 << 2 | (uint64_t)cx.full_first << 3);
+    mix((uint64_t)cx.max_bins | (uint64_t)cx.wg_cap << 16);
     mix(cx.scan_direct);
     const Hints *e = cx.hints_for(B.shape);
     mix((uint64_t)(int64_t)(e ? e->weight_hint : cx.weight_hint));
@@ -1069,7 +1092,8 @@ static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int
 // replayed.  The instance table is copied before each launch.  Returns 0
 // when the results stand, 1 when a search outran its queued passes (rerun
 // host-checked), 2 when the graph could not be made or run, or the data
-// needs another weight mode (rerun eagerly).
+// needs another weight mode (rerun eagerly), 3 when the key is new (run
+// eagerly; captured if the next call has it too).
 template <typename Run>
 static int run_captured(Context &cx, hipStream_t caller, const BatchPlan &B, size_t dim, int wkind, const void *out,
                         const void *pts, const void *w, int spec, Run &&run) {
@@ -1087,6 +1111,12 @@ static int run_captured(Context &cx, hipStream_t caller, const BatchPlan &B, siz
     GraphEntry *G = nullptr;
     for (GraphEntry &g : cx.graph_cache)
         if (g.key == key && g.layout == layout) G = &g;
+    // a key is captured when it repeats (the same shape, arrays and hints
+    // twice in a row): calls whose hints keep changing stay eager
+    if (!G && key != cx.last_key) {
+        cx.last_key = key;
+        return 3;
+    }
     if (!G) {
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             (void)hipGetLastError();
@@ -1158,8 +1188,24 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         }
     } call_scope(cx, rot, src);
     cx.swept_bytes = 0.0;
+    // switches of the call: the autotuner's for its class, if any
+    struct TuneScope {
+        Context &c;
+        coupe_hip_tuning saved;
+        TuneScope(Context &c_) : c(c_), saved(current_tuning(c_)) {}
+        ~TuneScope() { apply_tuning(c, saved); }
+    } tune_scope(cx);
     try {
         BatchPlan B = plan_batch(counts, levels, tols, n_inst, dim, src_idx, src_pt);
+        if (!cx.tuning_now) {
+            const uint64_t tk = tune_key(B, dim);
+            for (const auto &e : cx.tuned)
+                if (e.first == tk) {
+                    apply_tuning(cx, e.second);
+                    cx.stats.tuned = 1;
+                }
+        }
+        cx.stats.tuning = current_tuning(cx);
         cx.stats.points = B.n;
         if (B.n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
         if (B.n > 0xffffffffull) return COUPE_ERR_ALLOC;
@@ -1178,10 +1224,12 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         // under 2) when this context's last call needed fewer (its deepest
         // search's decisions); each queued pass costs two launches even when
         // its gate is closed.
+        // A shape run before queues as many as its deepest search needed then
+        // (2..16; a level queues no more than it needed).
         int spec = cx.search_passes;
         const Hints *seen = cx.hints_for(B.shape);
         const int passes_hint = seen ? seen->passes_hint : 0;
-        if (passes_hint > 0 && spec > 2) spec = std::max(2, std::min(spec, passes_hint));
+        if (passes_hint > 0 && spec > 0) spec = std::max(2, std::min(16, passes_hint));
         // A shape this context has run (hints known, buffers sized) runs as
         // one captured graph: 0 done, 1 a search outran its queued passes,
         // 2 not run that way (or its data needs another weight mode).
@@ -1189,6 +1237,7 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         if (cx.graphs && !rot && !cx.prof.enabled && !t_arena && spec > 0 && cx.bins_clean && seen &&
             seen->weight_hint >= 0 && seen->level_passes.size() == B.max_levels)
             how = run_captured(cx, st, B, dim, wkind, d_out, d_points, d_weights, spec, run);
+        if (how == 3) how = 2;   // first sight of the graph key: eager
         const bool unfinished = how == 1 || (how == 2 && run(spec, st));
         if (unfinished) {
             // a split search needed more passes than were queued: again,
@@ -1245,6 +1294,95 @@ enum coupe_err rcb_batch_mixed(Context &cx, hipStream_t st, uint64_t *d_out, con
         if (e != COUPE_ERR_OK) return e;
     }
     return COUPE_ERR_OK;
+}
+
+enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const double *d_points, int wkind,
+                        const void *d_weights, const uint64_t *counts, const size_t *dims, const size_t *levels,
+                        const double *tols, size_t n_inst, uint32_t trials, coupe_hip_tuning *chosen,
+                        double *best_ms) {
+    if (trials < 1) trials = 1;
+    // the classes of the call's groups (one per dimension present)
+    std::vector<uint64_t> keys;
+    for (size_t d = 2; d <= 3; d++) {
+        std::vector<uint64_t> c;
+        std::vector<size_t> l;
+        std::vector<double> t;
+        for (size_t q = 0; q < n_inst; q++)
+            if (dims[q] == d) {
+                c.push_back(counts[q]);
+                l.push_back(levels[q]);
+                t.push_back(tols[q]);
+            }
+        if (!c.empty()) keys.push_back(tune_key(plan_batch(c.data(), l.data(), t.data(), c.size(), d), d));
+    }
+    struct Restore {
+        Context &c;
+        coupe_hip_tuning t;
+        ~Restore() {
+            apply_tuning(c, t);
+            c.tuning_now = false;
+        }
+    } restore{cx, current_tuning(cx)};
+    cx.tuning_now = true;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return COUPE_ERR_CRASH;
+    }
+    enum coupe_err err = COUPE_ERR_OK;
+    // mean ms of `trials` calls under t, after a warm-up call (hints, graph)
+    auto measure = [&](const coupe_hip_tuning &t, double &ms) {
+        apply_tuning(cx, t);
+        err = rcb_batch_mixed(cx, st, d_out, d_points, wkind, d_weights, counts, dims, levels, tols, n_inst);
+        if (err != COUPE_ERR_OK) return false;
+        if (hipEventRecord(e0, st) != hipSuccess) return false;
+        for (uint32_t k = 0; k < trials && err == COUPE_ERR_OK; k++)
+            err = rcb_batch_mixed(cx, st, d_out, d_points, wkind, d_weights, counts, dims, levels, tols, n_inst);
+        if (err != COUPE_ERR_OK || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return false;
+        float f = 0.f;
+        if (hipEventElapsedTime(&f, e0, e1) != hipSuccess) return false;
+        ms = f / trials;
+        return true;
+    };
+    coupe_hip_tuning best = current_tuning(cx);
+    double best_t = 0.0;
+    bool ok = measure(best, best_t);
+    // one switch at a time from the best so far; a candidate must win by 1%
+    auto try_set = [&](auto field, std::initializer_list<uint32_t> values) {
+        for (uint32_t v : values) {
+            if (!ok) return;
+            coupe_hip_tuning t = best;
+            t.*field = v;
+            if (t.*field == best.*field) continue;
+            double m = 0.0;
+            ok = measure(t, m);
+            if (ok && m < 0.99 * best_t) {
+                best = t;
+                best_t = m;
+            }
+        }
+    };
+    try_set(&coupe_hip_tuning::bins, {256, 128, 64, 32});
+    try_set(&coupe_hip_tuning::workgroups_per_cu, {0, 4, 2, 1});
+    try_set(&coupe_hip_tuning::full_first, {1, 0});
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) return err != COUPE_ERR_OK ? err : COUPE_ERR_CRASH;
+    for (uint64_t k : keys) {
+        bool found = false;
+        for (auto &e : cx.tuned)
+            if (e.first == k) {
+                e.second = best;
+                found = true;
+            }
+        if (!found) cx.tuned.emplace_back(k, best);
+    }
+    if (chosen) *chosen = best;
+    if (best_ms) *best_ms = best_t;
+    // the ids of the chosen switches (the last call ran a candidate)
+    cx.tuning_now = false;
+    return rcb_batch_mixed(cx, st, d_out, d_points, wkind, d_weights, counts, dims, levels, tols, n_inst);
 }
 
 enum coupe_err rcb_device(Context &cx, hipStream_t st, uint64_t *d_out, uint64_t n, size_t dim,

@@ -112,6 +112,37 @@ enum coupe_err coupe_hip_rcb_batch_mixed(coupe_hip_ctx *ctx, void *stream, uint6
 		const struct coupe_hip_instance_nd *instances, uintptr_t n_instances);
 
 /*
+ * The engine's switches a call runs with.  None changes the ids, only the
+ * work done: bins per part of the later split-search passes (16..256, a
+ * power of two: log2(bins) bisection steps per pass), workgroups per CU of
+ * the persistent kernels (0: as many as fit; fewer gives each workgroup
+ * more tiles), split-search passes queued per level before the host looks
+ * for a shape not run before (2..8), and the last pair of levels' first pass with every candidate where
+ * most searches outlast their first candidate (0/1).
+ */
+struct coupe_hip_tuning {
+	uint32_t bins;
+	uint32_t workgroups_per_cu;
+	uint32_t search_passes;
+	uint32_t full_first;
+};
+
+/*
+ * Autotuner: runs the batch of coupe_hip_rcb_batch_mixed (same arguments)
+ * under candidate switches, one switch at a time from the current ones
+ * (bins, then workgroups per CU, then the full first pass), each candidate timed with HIP events over `trials` calls after a
+ * warm-up, and keeps the fastest for the batch's class: its dimension, the
+ * log2 of its point count, its deepest iter_count and the log4 of its
+ * instance count.  Later calls of that class on the context run with those
+ * switches.  The ids are written as by coupe_hip_rcb_batch_mixed; `chosen`
+ * (may be NULL) receives the switches kept and `best_ms` the fastest mean
+ * time.  No counterpart in coupe.
+ */
+enum coupe_err coupe_hip_autotune(coupe_hip_ctx *ctx, void *stream, uint64_t *d_partition, const double *d_points,
+		enum coupe_type weight_type, const void *d_weights, const struct coupe_hip_instance_nd *instances,
+		uintptr_t n_instances, uint32_t trials, struct coupe_hip_tuning *chosen, double *best_ms);
+
+/*
  * coupe::Rib on device-resident data: same arguments and errors as
  * coupe_hip_rcb_device.  The points are rotated so that the first axis is
  * their inertia axis, then partitioned by RCB; the rotation is applied on the
@@ -320,6 +351,8 @@ struct coupe_hip_stats {
 	uint32_t full_first;      /* levels whose children's first pass took every candidate (last pair, adaptive) */
 	uint32_t graph;           /* 0: launched kernel by kernel; 1: one launch of the graph captured from an
 	                             earlier call of the same shape; 2: captured by this call, then launched */
+	uint32_t tuned;           /* 1 when the call ran with switches the autotuner chose for its class */
+	struct coupe_hip_tuning tuning;   /* the switches the call ran with */
 };
 enum coupe_err coupe_hip_last_stats(const coupe_hip_ctx *ctx, struct coupe_hip_stats *out);
 

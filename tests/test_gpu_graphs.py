@@ -52,7 +52,18 @@ def test_replays_read_the_current_data(oracle_mod):
             say(f"graph replay call {q} ({kind}): graph {st['graph']} reruns {st['reruns']}")
             got = out.cpu().numpy().astype(np.uint64)
             assert np.array_equal(got, _ids(oracle_mod, pts, w, iters, tol)), f"call {q} ({kind})"
-        assert seen[0] == 0 and 2 in seen and 1 in seen, seen
+        assert seen[0] == 0, seen
+        # the same data again and again: captured once its key repeats, then replayed
+        exp = _ids(oracle_mod, pts, w, iters, tol)
+        again = []
+        for rep in range(5):
+            out.fill_(-1)
+            coupe_amd.Rcb(iter_count=iters, tolerance=tol).partition(out, (d_pts, d_w), context=ctx)
+            torch.cuda.synchronize()
+            again.append(ctx.stats()["graph"])
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"repeat {rep}"
+        say(f"graph use over repeated calls: {again}")
+        assert 2 in again and again[-1] == 1 and again[-2] == 1, again
     finally:
         ctx.close()
 
@@ -70,7 +81,7 @@ def test_shapes_alternating_on_one_context(oracle_mod):
             data.append((torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda(),
                          torch.empty(n, dtype=torch.int64, device="cuda"), _ids(oracle_mod, pts, w, it, tol)))
         graphs = []
-        for rep in range(4):
+        for rep in range(5):
             for k, (n, d, it, tol) in enumerate(shapes):
                 dp, dw, out, exp = data[k]
                 out.fill_(-1)
@@ -78,7 +89,8 @@ def test_shapes_alternating_on_one_context(oracle_mod):
                 torch.cuda.synchronize()
                 graphs.append(ctx.stats()["graph"])
                 assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"rep {rep} shape {k}"
-        assert graphs.count(1) >= 6, graphs
+        say(f"graph use, shapes alternating: {graphs}")
+        assert graphs[-3:].count(1) >= 2, graphs
     finally:
         ctx.close()
 
@@ -119,3 +131,68 @@ def test_batches_replayed(oracle_mod, monkeypatch):
                     assert ctx.stats()["graph"] == 0
         finally:
             ctx.close()
+
+
+def test_autotuned_switches_keep_the_ids(oracle_mod):
+    """The autotuner's candidates (bins, workgroups per CU, full first pass)
+    change the work, never the ids; the switches it keeps apply to later
+    calls of the batch's class."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(73)
+    specs, flat, ws = [], [], []
+    for q in range(40):
+        d = 3 if q % 3 else 2
+        n = int(10 ** rng.uniform(3, 5))
+        flat.append((rng.normal(0, 1, (n, d)) ** 3).ravel())
+        ws.append(rng.integers(1, 100, n).astype(np.int64))
+        specs.append((n, d, int(rng.integers(4, 11)), 0.01))
+    flat, w = np.concatenate(flat), np.concatenate(ws)
+    expect = []
+    off = poff = 0
+    for n, d, it, tol in specs:
+        expect.append(_ids(oracle_mod, flat[poff:poff + n * d].reshape(n, d), w[off:off + n], it, tol))
+        off += n
+        poff += n * d
+    expect = np.concatenate(expect)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        d_flat, d_w = torch.from_numpy(flat).cuda(), torch.from_numpy(w).cuda()
+        out = torch.full((w.shape[0],), -1, dtype=torch.int64, device="cuda")
+        chosen = coupe_amd.autotune(out, d_flat, d_w, specs, trials=2, context=ctx)
+        torch.cuda.synchronize()
+        say(f"autotune: {chosen}")
+        assert np.array_equal(out.cpu().numpy().astype(np.uint64), expect)
+        out.fill_(-1)
+        coupe_amd.partition_batch_mixed(out, d_flat, d_w, specs, context=ctx)
+        torch.cuda.synchronize()
+        st = ctx.stats()
+        assert st["tuned"] == 1 and st["tuning"]["bins"] == chosen["bins"], st
+        assert np.array_equal(out.cpu().numpy().astype(np.uint64), expect)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("bins,wg", [(16, 1), (32, 2), (64, 0), (128, 4)])
+def test_switches_from_the_environment(oracle_mod, monkeypatch, bins, wg):
+    """COUPE_HIP_BINS / COUPE_HIP_WG_PER_CU: the values the tuner tries, on
+    clustered clouds whose searches take several passes."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_BINS", str(bins))
+    monkeypatch.setenv("COUPE_HIP_WG_PER_CU", str(wg))
+    rng = np.random.default_rng(74 + bins)
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        for q in range(3):
+            n = int(rng.integers(50_000, 400_000))
+            pts = rng.normal(0, 1, (n, 3)) ** 5
+            w = rng.integers(1, 300, n).astype(np.int64)
+            it, tol = 10, [0.0, 0.001, 0.01][q]
+            say(f"bins {bins} wg {wg} case {q}: n={n}")
+            out = torch.full((n,), -1, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=it, tolerance=tol).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            assert ctx.stats()["tuning"]["bins"] == bins
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), _ids(oracle_mod, pts, w, it, tol))
+    finally:
+        ctx.close()
