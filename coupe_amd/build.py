@@ -45,7 +45,9 @@ def _run(cmd):
 def build_hip(verbose: bool = True) -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs]
+    # roctx trace ranges (rocprofiler-sdk)
+    roctx = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-o", tmp, *srcs, *roctx]
     _run(cmd)
     os.replace(tmp, LIB)
     return LIB

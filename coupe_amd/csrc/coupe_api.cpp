@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "rcb_engine.h"
@@ -65,18 +66,122 @@ namespace {
 
 size_t type_size(enum coupe_type t) { return t == COUPE_INT ? sizeof(int) : 8; }
 
-// Host view of `n` elements of `elem` bytes: the caller's array when there is
-// one, else a materialised copy in `store`.
-const void *materialise(const coupe_data *d, size_t elem, std::vector<unsigned char> &store) {
-    if (d->kind == coupe_data::Array) return d->ptr;
-    store.resize(d->len * elem);
-    unsigned char *out = store.data();
-    if (d->kind == coupe_data::Constant) {
-        for (uintptr_t i = 0; i < d->len; i++) memcpy(out + i * elem, d->ptr, elem);
-    } else {
-        for (uintptr_t i = 0; i < d->len; i++) memcpy(out + i * elem, d->i_th(d->context, i), elem);
+// Host side of the host-memory entry: the data sets are produced (copied,
+// repeated or called for, on several threads, as coupe's rayon workers read
+// them, coupe-ffi/src/data.rs:231-256) into pinned chunks whose DMA to HBM
+// overlaps the production of the next chunk; the ids come back the same way.
+constexpr size_t kStageChunk = 32ull << 20;   // bytes per pinned chunk
+constexpr size_t kStageDirect = 4ull << 20;   // below this, one plain copy
+
+unsigned host_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return h ? std::min(h, 8u) : 4u;
+}
+
+// f(lo, hi) over [0, n), split across up to `nt` threads (the caller's included).
+template <typename F>
+void parallel_for(size_t n, unsigned nt, F &&f) {
+    if (nt <= 1 || n < (1u << 16)) {
+        f((size_t)0, n);
+        return;
     }
-    return out;
+    const size_t step = (n + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned k = 1; k < nt; k++) {
+        const size_t lo = k * step, hi = std::min(n, lo + step);
+        if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    f((size_t)0, std::min(n, step));
+    for (auto &t : th) t.join();
+}
+
+// Elements [first, first + count) of a data set (`elem` bytes each) into out.
+void produce(const coupe_data *d, size_t elem, uintptr_t first, uintptr_t count, unsigned char *out, unsigned nt) {
+    parallel_for(count, nt, [&](size_t lo, size_t hi) {
+        if (d->kind == coupe_data::Array) {
+            memcpy(out + lo * elem, (const unsigned char *)d->ptr + (first + lo) * elem, (hi - lo) * elem);
+        } else if (d->kind == coupe_data::Constant) {
+            for (size_t i = lo; i < hi; i++) memcpy(out + i * elem, d->ptr, elem);
+        } else {
+            for (size_t i = lo; i < hi; i++) memcpy(out + i * elem, d->i_th(d->context, first + i), elem);
+        }
+    });
+}
+
+// The context's pinned chunks and their events (made on first use, kept).
+void stage_init(Context &cx) {
+    if (cx.stage_buf[0]) return;
+    for (int b = 0; b < Context::kStageBufs; b++) {
+        hipError_t e = hipHostMalloc(&cx.stage_buf[b], kStageChunk, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&cx.stage_ev[b], hipEventDisableTiming);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+    }
+}
+
+// A data set of n elements into device memory at dst, on st.
+void stage_in(Context &cx, hipStream_t st, const coupe_data *d, size_t elem, uintptr_t n, void *dst, unsigned nt) {
+    const size_t bytes = n * elem;
+    if (bytes <= kStageDirect) {   // small: one copy (from the caller's array, or produced)
+        std::vector<unsigned char> tmp;
+        const void *src = d->ptr;
+        if (d->kind != coupe_data::Array) {
+            tmp.resize(bytes);
+            produce(d, elem, 0, n, tmp.data(), 1);
+            src = tmp.data();
+        }
+        const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) (void)hipStreamSynchronize(st);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+        return;
+    }
+    stage_init(cx);
+    const uintptr_t per = kStageChunk / elem;
+    for (uintptr_t c = 0, first = 0; first < n; c++, first += per) {
+        const int b = (int)(c % Context::kStageBufs);
+        const uintptr_t count = std::min<uintptr_t>(per, n - first);
+        // the chunk's previous copy to the device has run
+        hipError_t e = hipEventSynchronize(cx.stage_ev[b]);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+        produce(d, elem, first, count, (unsigned char *)cx.stage_buf[b], nt);
+        e = hipMemcpyAsync((unsigned char *)dst + first * elem, cx.stage_buf[b], count * elem, hipMemcpyHostToDevice,
+                           st);
+        if (e == hipSuccess) e = hipEventRecord(cx.stage_ev[b], st);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+    }
+}
+
+// n ids from device memory at src into the caller's array, on st.
+void stage_out(Context &cx, hipStream_t st, const uint64_t *src, uintptr_t n, uintptr_t *out, unsigned nt) {
+    const size_t bytes = n * sizeof(uint64_t);
+    if (bytes <= kStageDirect) {
+        hipError_t e = hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+        return;
+    }
+    stage_init(cx);
+    const uintptr_t per = kStageChunk / sizeof(uint64_t);
+    const uintptr_t chunks = (n + per - 1) / per;
+    auto copy_out = [&](uintptr_t c) {   // chunk c, once its copy from the device has run
+        const int b = (int)(c % Context::kStageBufs);
+        const hipError_t e = hipEventSynchronize(cx.stage_ev[b]);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+        const uintptr_t first = c * per, count = std::min<uintptr_t>(per, n - first);
+        parallel_for(count * sizeof(uint64_t), nt, [&](size_t lo, size_t hi) {
+            memcpy((unsigned char *)(out + first) + lo, (const unsigned char *)cx.stage_buf[b] + lo, hi - lo);
+        });
+    };
+    for (uintptr_t c = 0; c < chunks; c++) {
+        const int b = (int)(c % Context::kStageBufs);
+        const uintptr_t first = c * per, count = std::min<uintptr_t>(per, n - first);
+        if (c >= (uintptr_t)Context::kStageBufs) copy_out(c - Context::kStageBufs);   // frees chunk b
+        hipError_t e = hipMemcpyAsync(cx.stage_buf[b], src + first, count * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                      st);
+        if (e == hipSuccess) e = hipEventRecord(cx.stage_ev[b], st);
+        if (e != hipSuccess) throw coupe_hip::HipFailure{e};
+    }
+    for (uintptr_t c = chunks > (uintptr_t)Context::kStageBufs ? chunks - Context::kStageBufs : 0; c < chunks; c++)
+        copy_out(c);
 }
 
 std::mutex g_default_lock;
@@ -114,27 +219,26 @@ enum coupe_err host_partition(uintptr_t *partition, uintptr_t dimension, const c
     if (!cx) return COUPE_ERR_CRASH;
     std::lock_guard<std::mutex> g(cx->lock);
     try {
-        std::vector<unsigned char> pstore, wstore;
-        const void *hp = materialise(points, dimension * sizeof(double), pstore);
         const size_t wsz = type_size(weights->type);
-        const void *hw = materialise(weights, wsz, wstore);
         const size_t pbytes = n * dimension * sizeof(double), wbytes = n * wsz, obytes = n * sizeof(uint64_t);
         cx->in_points.ensure(pbytes);
         cx->in_weights.ensure(wbytes);
         cx->out_ids.ensure(obytes);
-        hipStream_t st = nullptr;
-        hipError_t e;
-        if ((e = hipMemcpy(cx->in_points.ptr, hp, pbytes, hipMemcpyHostToDevice)) != hipSuccess)
-            return coupe_hip::map_hip_error(e);
-        if ((e = hipMemcpy(cx->in_weights.ptr, hw, wbytes, hipMemcpyHostToDevice)) != hipSuccess)
-            return coupe_hip::map_hip_error(e);
+        if (!cx->stage_st && hipStreamCreateWithFlags(&cx->stage_st, hipStreamNonBlocking) != hipSuccess)
+            return COUPE_ERR_CRASH;
+        hipStream_t st = cx->stage_st;
+        const unsigned nt = host_threads();
+        stage_in(*cx, st, points, dimension * sizeof(double), n, cx->in_points.ptr, nt);
+        stage_in(*cx, st, weights, wsz, n, cx->in_weights.ptr, nt);
         auto run = rib ? coupe_hip::rib_device : coupe_hip::rcb_device;
         enum coupe_err r = run(*cx, st, (uint64_t *)cx->out_ids.ptr, n, dimension,
                                (const double *)cx->in_points.ptr, weights->type, cx->in_weights.ptr, iter_count,
                                tolerance);
         if (r != COUPE_ERR_OK) return r;
-        if ((e = hipMemcpy(partition, cx->out_ids.ptr, obytes, hipMemcpyDeviceToHost)) != hipSuccess)
-            return coupe_hip::map_hip_error(e);
+        stage_out(*cx, st, (const uint64_t *)cx->out_ids.ptr, n, partition, nt);
+    } catch (const coupe_hip::HipFailure &f) {
+        (void)hipGetLastError();
+        return coupe_hip::map_hip_error(f.err);
     } catch (const std::bad_alloc &) {
         return COUPE_ERR_ALLOC;
     } catch (...) {

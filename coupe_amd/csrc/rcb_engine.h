@@ -84,6 +84,7 @@ struct Hints {
     int passes_hint = 0;                // most decisions one level's search needed
     std::vector<uint8_t> level_passes;  // decisions each level's search needed
     std::vector<uint32_t> level_open;   // parts of each level still searching after the first decision
+    uint64_t graph_key = 0;             // graph key of its last call (a key is captured when it repeats)
 };
 
 constexpr size_t kHintShapes = 16;      // shapes a context remembers
@@ -149,7 +150,6 @@ struct Context {
     std::vector<GraphEntry> graph_cache;
     hipStream_t gstream = nullptr;   // where graphs are captured and launched
     hipEvent_t gevent = nullptr;     // orders gstream after the caller's stream
-    uint64_t last_key = 0;           // graph key of the last call that could have run as a graph
     uint64_t layout() ;              // hash of every buffer's address and size
     void drop_graphs();
     int weight_hint = -1;            // weight mode of the last call, for a shape not seen before
@@ -181,6 +181,10 @@ struct Context {
     const Rotation *rot = nullptr;   // change of basis for the running call
     DevBuf prologue, counters;
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
+    static constexpr int kStageBufs = 3;
+    void *stage_buf[kStageBufs] = {};        // pinned chunks of the host-memory ABI's copies
+    hipEvent_t stage_ev[kStageBufs] = {};    // each chunk's last copy
+    hipStream_t stage_st = nullptr;          // the host-memory ABI's stream
     void *host_scratch = nullptr;    // pinned, for small readbacks
     bool bins_clean = false;
     int cu_count = 0;                // compute units of the device

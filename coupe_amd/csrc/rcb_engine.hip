@@ -10,6 +10,8 @@
 #include <type_traits>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "rcb_engine.h"
 #include "rcb_kernels.h"
 #include "rcb_pairs.h"
@@ -111,9 +113,29 @@ void Profiler::release() {
     recs.clear();
 }
 
+// Trace ranges (roctx, seen by rocprofv3 --marker-trace): one per call, per
+// bisection level (coupe's spans per split, recursive_bisection.rs:467-468,
+// and per leaf pass, :590-595) and per kernel launch, named by kernel class.
+static const char *const kKernelRange[COUPE_HIP_K_COUNT] = {"prologue", "hist", "decide", "partition",
+                                                            "ids",      "other", "moments"};
+
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+};
+
+static const char *level_range(size_t level) {
+    static const char *const names[] = {"level 0",  "level 1",  "level 2",  "level 3",  "level 4",  "level 5",
+                                        "level 6",  "level 7",  "level 8",  "level 9",  "level 10", "level 11",
+                                        "level 12", "level 13", "level 14", "level 15", "level 16", "level 17",
+                                        "level 18", "level 19", "level 20", "level 21", "level 22", "level 23"};
+    return level < sizeof(names) / sizeof(names[0]) ? names[level] : "level 24+";
+}
+
 // Runs `launch` between two profiling events when profiling is on.
 template <typename F>
 static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launch) {
+    TraceRange range(kKernelRange[kind]);
     int r = cx.prof.begin(st, kind, bytes);
     launch();
     check(hipGetLastError());
@@ -615,6 +637,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             });
         };
         for (size_t level = 0; level < L; level += 2) {
+            TraceRange range(level_range(level));   // a pair of levels: this range covers both
             const int ib = (ia + 1) % 3, ic = (ia + 2) % 3;
             Part *pa = (Part *)cx.parts[ia].ptr;
             Part *pb = (Part *)cx.parts[ib].ptr;
@@ -720,6 +743,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         parts_ub = std::max<uint64_t>(B.parts_ub[L - 1], 1);
     } else {
         for (size_t level = 0; level < L; level++) {
+            TraceRange range(level_range(level));
             cx.stats.levels++;
             const int nxt = cur ^ 1;
             Part *parts = (Part *)cx.parts[cur].ptr;
@@ -1113,9 +1137,12 @@ static int run_captured(Context &cx, hipStream_t caller, const BatchPlan &B, siz
         if (g.key == key && g.layout == layout) G = &g;
     // a key is captured when it repeats (the same shape, arrays and hints
     // twice in a row): calls whose hints keep changing stay eager
-    if (!G && key != cx.last_key) {
-        cx.last_key = key;
-        return 3;
+    if (!G) {
+        Hints &e = cx.remember(B.shape);
+        if (key != e.graph_key) {
+            e.graph_key = key;
+            return 3;
+        }
     }
     if (!G) {
         if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -1174,6 +1201,7 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
     if (rot && src) return COUPE_ERR_CRASH;   // a rotation needs the points themselves
+    TraceRange range(n_inst == 1 ? "coupe_hip rcb" : "coupe_hip rcb batch");
     memset(&cx.stats, 0, sizeof cx.stats);
     if (!rot) cx.prof.reset();
     struct CallScope {
@@ -1197,15 +1225,15 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
     } tune_scope(cx);
     try {
         BatchPlan B = plan_batch(counts, levels, tols, n_inst, dim, src_idx, src_pt);
+        bool tuned = false;
         if (!cx.tuning_now) {
             const uint64_t tk = tune_key(B, dim);
             for (const auto &e : cx.tuned)
                 if (e.first == tk) {
                     apply_tuning(cx, e.second);
-                    cx.stats.tuned = 1;
+                    tuned = true;
                 }
         }
-        cx.stats.tuning = current_tuning(cx);
         cx.stats.points = B.n;
         if (B.n == 0) return COUPE_ERR_OK;            // BoundingBox::from_points is None (:685-688)
         if (B.n > 0xffffffffull) return COUPE_ERR_ALLOC;
@@ -1248,6 +1276,8 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
             run(0, st);
         }
         cx.remember(B.shape).passes_hint = (int)cx.stats.max_passes;
+        cx.stats.tuned = tuned ? 1u : 0u;
+        cx.stats.tuning = current_tuning(cx);
         cx.prof.collect();
         cx.prof.out[COUPE_HIP_K_HIST].algo_bytes += cx.swept_bytes;   // first passes + standalone passes
     } catch (const HipFailure &f) {
@@ -1427,6 +1457,14 @@ uint64_t Context::device_bytes() const {
 
 void Context::release_all() {
     drop_graphs();
+    for (int b = 0; b < kStageBufs; b++) {
+        if (stage_buf[b]) (void)hipHostFree(stage_buf[b]);
+        if (stage_ev[b]) (void)hipEventDestroy(stage_ev[b]);
+        stage_buf[b] = nullptr;
+        stage_ev[b] = nullptr;
+    }
+    if (stage_st) (void)hipStreamDestroy(stage_st);
+    stage_st = nullptr;
     for_each_buf([](DevBuf &d) { d.release(); });
     if (host_scratch && !borrowed_host) (void)hipHostFree(host_scratch);
     host_scratch = nullptr;

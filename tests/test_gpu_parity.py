@@ -92,6 +92,40 @@ def test_callback_data_sets():
     assert np.array_equal(part, gpu_rcb(pts, w, 5, 0.02))
 
 
+def test_host_entry_staged_in_chunks(oracle_mod):
+    """coupe_rcb on inputs of several pinned chunks: arrays, a constant data
+    set and a callback data set (called from several host threads), ids equal
+    to the oracle's."""
+    L = _native.lib()
+    rng = np.random.default_rng(33)
+    n = 3_000_000   # 72 MB of points: three chunks and a half
+    pts = rng.random((n, 3))
+    w = rng.integers(1, 1000, n).astype(np.int64)
+    part = np.zeros(n, dtype=np.uint64)
+    say("host entry, arrays in chunks")
+    dp = L.coupe_data_array(n, _native.COUPE_DOUBLE, pts.ctypes.data)
+    dw = L.coupe_data_array(n, _native.COUPE_INT64, w.ctypes.data)
+    assert L.coupe_rcb(part.ctypes.data, 3, dp, dw, 11, 0.01) == 0
+    L.coupe_data_free(dp)
+    L.coupe_data_free(dw)
+    exp, _ = oracle_rcb(oracle_mod, pts, w, 11, 0.01, parallel=True)
+    assert np.array_equal(part, exp)
+    # a callback for the points, a constant for the weights
+    m = 400_000
+    sub = np.ascontiguousarray(pts[:m])
+    p_cb = _native.ITH_FN(lambda ctx, i: sub.ctypes.data + 24 * i)
+    one = np.array([3], dtype=np.int32)
+    say("host entry, callback points and constant weights")
+    dp = L.coupe_data_fn(None, m, _native.COUPE_DOUBLE, p_cb)
+    dw = L.coupe_data_constant(m, _native.COUPE_INT, one.ctypes.data)
+    part = np.zeros(m, dtype=np.uint64)
+    assert L.coupe_rcb(part.ctypes.data, 3, dp, dw, 9, 0.02) == 0
+    L.coupe_data_free(dp)
+    L.coupe_data_free(dw)
+    exp, _ = oracle_rcb(oracle_mod, sub, np.full(m, 3, dtype=np.int32), 9, 0.02)
+    assert np.array_equal(part, exp)
+
+
 @pytest.mark.parametrize("seed", range(160))
 def test_random_small_cases(oracle_mod, seed):
     pts, w, iters, tol = cases.random_case(seed, n_max=300)
