@@ -1217,11 +1217,18 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
     } call_scope(cx, rot, src);
     cx.swept_bytes = 0.0;
     // switches of the call: the autotuner's for its class, if any
-    struct TuneScope {
+    struct TuneScope {   // the context's own switches come back after the call
         Context &c;
-        coupe_hip_tuning saved;
-        TuneScope(Context &c_) : c(c_), saved(current_tuning(c_)) {}
-        ~TuneScope() { apply_tuning(c, saved); }
+        int bins, wg, passes;
+        bool full;
+        TuneScope(Context &c_)
+            : c(c_), bins(c_.max_bins), wg(c_.wg_cap), passes(c_.search_passes), full(c_.full_first) {}
+        ~TuneScope() {
+            c.max_bins = bins;
+            c.wg_cap = wg;
+            c.search_passes = passes;
+            c.full_first = full;
+        }
     } tune_scope(cx);
     try {
         BatchPlan B = plan_batch(counts, levels, tols, n_inst, dim, src_idx, src_pt);
@@ -1345,14 +1352,18 @@ enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const doub
             }
         if (!c.empty()) keys.push_back(tune_key(plan_batch(c.data(), l.data(), t.data(), c.size(), d), d));
     }
-    struct Restore {
+    struct Restore {   // the context's own switches come back after the tuning
         Context &c;
-        coupe_hip_tuning t;
+        int bins, wg, passes;
+        bool full;
         ~Restore() {
-            apply_tuning(c, t);
+            c.max_bins = bins;
+            c.wg_cap = wg;
+            c.search_passes = passes;
+            c.full_first = full;
             c.tuning_now = false;
         }
-    } restore{cx, current_tuning(cx)};
+    } restore{cx, cx.max_bins, cx.wg_cap, cx.search_passes, cx.full_first};
     cx.tuning_now = true;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
