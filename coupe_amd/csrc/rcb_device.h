@@ -102,6 +102,24 @@ __device__ inline int bin_of(const float *cand, int nb, float x) {
         if (x >= cand[pos + step - 1]) pos += step;
     return pos;
 }
+
+// bin_of by an estimate and comparisons: the candidates of a pass lie close
+// to lo + (hi - lo) k / nb, so (x - lo) * nb / (hi - lo) is within a bin or
+// two of x's bin; the result is the bin where cand[pos - 1] <= x < cand[pos]
+// (the candidates are sorted), exactly bin_of's.  A search whose candidates
+// have collapsed (a few ulps wide) falls back to bin_of.  scale = nb / (hi - lo).
+__device__ inline int bin_near(const float *cand, int nb, float x, float lo, float scale) {
+    const float e = (x - lo) * scale;
+    int pos = e >= 0.0f ? (e < (float)(nb - 1) ? (int)e : nb - 1) : 0;   // NaN: 0
+#pragma unroll
+    for (int it = 0; it < 3; it++) {
+        const bool down = pos > 0 && x < cand[pos - 1];
+        const bool up = pos < nb - 1 && x >= cand[pos];
+        if (!down && !up) return pos;
+        pos += up ? 1 : -1;
+    }
+    return bin_of(cand, nb, x);
+}
 // Same for the 3 register candidates of the first pass.
 __device__ inline int bin4(float c0, float c1, float c2, float x) {
     int b = (x >= c1) ? 2 : 0;

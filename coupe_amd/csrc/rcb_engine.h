@@ -139,7 +139,7 @@ struct Context {
     bool weights16 = true;           // integer weights in [0, 65535] stored in 16 bits
     bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
     uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
-    bool graphs = true;              // replay a call of a known shape as one captured graph
+    bool graphs = false;             // replay a call of a known shape as one captured graph
     int max_bins = kMaxBins;         // bins per part of the later search passes (16..256, a power of two)
     int wg_cap = 0;                  // workgroups per CU of the persistent kernels (0: as many as fit)
     // tuned switches per batch class (tune_key), chosen by coupe_hip_autotune
@@ -241,11 +241,12 @@ inline bool full_first_from_env() {
     return !(v && v[0] == '0');
 }
 
-// COUPE_HIP_GRAPHS=0: never capture calls into graphs (every call launches
-// its kernels one by one).
+// COUPE_HIP_GRAPHS=1: calls of a shape seen before replay a captured graph
+// (default 0: kernel by kernel, unless the autotuner chose graphs for the
+// call's class).
 inline bool graphs_from_env() {
     const char *v = getenv("COUPE_HIP_GRAPHS");
-    return !(v && v[0] == '0');
+    return v && v[0] == '1';
 }
 
 // COUPE_HIP_BINS=b: bins per part of the later search passes (16..256,

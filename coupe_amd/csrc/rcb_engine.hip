@@ -1033,7 +1033,7 @@ static bool run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPla
 
 coupe_hip_tuning current_tuning(const Context &cx) {
     return coupe_hip_tuning{(uint32_t)cx.max_bins, (uint32_t)cx.wg_cap, (uint32_t)cx.search_passes,
-                            cx.full_first ? 1u : 0u};
+                            cx.full_first ? 1u : 0u, cx.graphs ? 1u : 0u};
 }
 
 void apply_tuning(Context &cx, const coupe_hip_tuning &t) {
@@ -1043,6 +1043,7 @@ void apply_tuning(Context &cx, const coupe_hip_tuning &t) {
     cx.wg_cap = (int)std::min<uint32_t>(t.workgroups_per_cu, 64);
     cx.search_passes = (int)std::max<uint32_t>(2, std::min<uint32_t>(t.search_passes, 8));
     cx.full_first = t.full_first != 0;
+    cx.graphs = t.graphs != 0;
 }
 
 static uint64_t tune_key(const BatchPlan &B, size_t dim) {
@@ -1220,14 +1221,16 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
     struct TuneScope {   // the context's own switches come back after the call
         Context &c;
         int bins, wg, passes;
-        bool full;
+        bool full, graphs;
         TuneScope(Context &c_)
-            : c(c_), bins(c_.max_bins), wg(c_.wg_cap), passes(c_.search_passes), full(c_.full_first) {}
+            : c(c_), bins(c_.max_bins), wg(c_.wg_cap), passes(c_.search_passes), full(c_.full_first),
+              graphs(c_.graphs) {}
         ~TuneScope() {
             c.max_bins = bins;
             c.wg_cap = wg;
             c.search_passes = passes;
             c.full_first = full;
+            c.graphs = graphs;
         }
     } tune_scope(cx);
     try {
@@ -1355,15 +1358,16 @@ enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const doub
     struct Restore {   // the context's own switches come back after the tuning
         Context &c;
         int bins, wg, passes;
-        bool full;
+        bool full, graphs;
         ~Restore() {
             c.max_bins = bins;
             c.wg_cap = wg;
             c.search_passes = passes;
             c.full_first = full;
+            c.graphs = graphs;
             c.tuning_now = false;
         }
-    } restore{cx, cx.max_bins, cx.wg_cap, cx.search_passes, cx.full_first};
+    } restore{cx, cx.max_bins, cx.wg_cap, cx.search_passes, cx.full_first, cx.graphs};
     cx.tuning_now = true;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1404,6 +1408,7 @@ enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const doub
             }
         }
     };
+    try_set(&coupe_hip_tuning::graphs, {1, 0});
     try_set(&coupe_hip_tuning::bins, {256, 128, 64, 32});
     try_set(&coupe_hip_tuning::workgroups_per_cu, {0, 4, 2, 1});
     try_set(&coupe_hip_tuning::full_first, {1, 0});

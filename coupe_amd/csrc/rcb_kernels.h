@@ -458,6 +458,7 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
     bool active = false;
     uint32_t before = 0;   // this thread's bin count of the part before the tile (tile_bins)
     const float *c = c0;
+    float lo = 0.0f, scale = 0.0f;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
@@ -471,17 +472,30 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
             if (active) {
                 for (int p = tid; p < nb - 1; p += kBlock) cand[p] = candidate(P.blo, P.bhi, p, nb);
                 for (int j = tid; j < nb; j += kBlock) { scnt[j] = 0; sw[j] = (Acc)0; smin[j] = 0xffffffffu; }
+                lo = P.blo;
+                scale = (float)nb / (P.bhi - P.blo);
             }
             c = P.axis == 0 ? c0 : (P.axis == 1 ? c1 : c2);
             __syncthreads();
         }
         if (!active) continue;
-        const uint64_t end = T.begin + T.count;
-        for (uint64_t i = T.begin + tid; i < end; i += kBlock) {
-            const float x = c[i];
-            const int b = bin_of(cand, nb, x);
+        // every load of the tile issued before the binning (clamped, unconditional)
+        const uint32_t cnt = T.count;
+        float xs[kItems];
+        WS wk[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t p = min((uint32_t)(k * kBlock + tid), cnt - 1u);
+            xs[k] = c[T.begin + p];
+            wk[k] = w[T.begin + p];
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            if ((uint32_t)(k * kBlock + tid) >= cnt) continue;
+            const float x = xs[k];
+            const int b = bin_near(cand, nb, x, lo, scale);
             atomicAdd(&scnt[b], 1u);
-            atomic_add_acc(&sw[b], AccOps<Acc>::of(w[i]));
+            atomic_add_acc(&sw[b], AccOps<Acc>::of(wk[k]));
             atomicMin(&smin[b], f32_key(x));
         }
         if (tile_bins) {

@@ -285,6 +285,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
     float split = 0.f;
     bool all_left = false;
     uint32_t before0 = 0, before1 = 0;
+    float clo[2] = {0.f, 0.f}, cscale[2] = {0.f, 0.f};
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
@@ -310,18 +311,33 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
                     const Part &C = children[ch[s]];
                     for (int p = tid; p < nb - 1; p += kBlock) cand[s][p] = candidate(C.blo, C.bhi, p, nb);
                     for (int j = tid; j < nb; j += kBlock) { scnt[s][j] = 0; sw[s][j] = (Acc)0; smin[s][j] = 0xffffffffu; }
+                    clo[s] = C.blo;
+                    cscale[s] = (float)nb / (C.bhi - C.blo);
                 }
             }
             __syncthreads();
         }
         if (!act[0] && !act[1]) continue;
-        for (uint64_t i = T.begin + tid; i < T.begin + T.count; i += kBlock) {
-            const int s = (!all_left && (ca[i] >= split)) ? 1 : 0;
+        // every load of the tile issued before the binning (clamped, unconditional)
+        const uint32_t cnt = T.count;
+        float xa[kItems], xb[kItems];
+        WS wk[kItems];
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const uint32_t p = min((uint32_t)(k * kBlock + tid), cnt - 1u);
+            xa[k] = ca[T.begin + p];
+            xb[k] = cb[T.begin + p];
+            wk[k] = w[T.begin + p];
+        }
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            if ((uint32_t)(k * kBlock + tid) >= cnt) continue;
+            const int s = (!all_left && (xa[k] >= split)) ? 1 : 0;
             if (!act[s]) continue;
-            const float x = cb[i];
-            const int b = bin_of(cand[s], nb, x);
+            const float x = xb[k];
+            const int b = bin_near(cand[s], nb, x, clo[s], cscale[s]);
             atomicAdd(&scnt[s][b], 1u);
-            atomic_add_acc(&sw[s][b], AccOps<Acc>::of(w[i]));
+            atomic_add_acc(&sw[s][b], AccOps<Acc>::of(wk[k]));
             atomicMin(&smin[s][b], f32_key(x));
         }
         __syncthreads();

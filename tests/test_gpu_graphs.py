@@ -1,6 +1,7 @@
-"""Calls replayed as captured graphs: a context captures a call of a shape it
-has run before (hints known) and replays the graph for later calls of that
-shape.  Every replay must give the eager path's ids -- the data is read at
+"""Calls replayed as captured graphs: with graphs on (COUPE_HIP_GRAPHS=1, or
+chosen by the autotuner), a context captures a call of a shape it has run
+before (hints known) once its graph key repeats, and replays the graph for
+later calls of that shape.  Every replay must give the eager path's ids -- the data is read at
 replay time, a weight mode the graph cannot hold or a search longer than its
 queued passes falls back to eager runs -- with eager work and
 synchronisations of the program between the replays."""
@@ -25,8 +26,9 @@ def _ids(oracle, pts, w, iters, tol):
     return ids
 
 
-def test_replays_read_the_current_data(oracle_mod):
+def test_replays_read_the_current_data(oracle_mod, monkeypatch):
     torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_GRAPHS", "1")
     rng = np.random.default_rng(70)
     ctx = coupe_amd.DeviceContext(0)
     try:
@@ -68,8 +70,9 @@ def test_replays_read_the_current_data(oracle_mod):
         ctx.close()
 
 
-def test_shapes_alternating_on_one_context(oracle_mod):
+def test_shapes_alternating_on_one_context(oracle_mod, monkeypatch):
     torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_GRAPHS", "1")
     rng = np.random.default_rng(71)
     ctx = coupe_amd.DeviceContext(0)
     try:
