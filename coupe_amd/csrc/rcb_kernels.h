@@ -429,13 +429,63 @@ k_sweep(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const ui
 // ---------------------------------------------------------------------------
 // Later passes: nb - 1 candidates (nb <= 256) binned in LDS.
 // ---------------------------------------------------------------------------
+// LDS bins of a search pass: count, weight and smallest key per bin.  With
+// 16-bit weights and integer sums, a part whose points a workgroup holds in
+// its bins stay under 2^24 (`packed`) keeps count and weight in one 64-bit
+// word (count << 40 | weight: at most 2^24 x 65535 < 2^40), one LDS atomic
+// instead of two.  The smallest key is taken with an atomic only when the
+// point's key is below the bin's current one (the bin's minimum only falls).
+constexpr int kPackShift = 40;
+
+template <typename WS, typename Acc>
+__device__ __forceinline__ constexpr bool packable() {
+    return std::is_same<WS, uint16_t>::value && std::is_same<Acc, uint64_t>::value;
+}
+
+// Whether a workgroup of a grid of `grid` workgroups striding over the tiles
+// of a part of `count` points can hold its share in packed bins: the part
+// spans at most count/kTile + 2 tiles, a workgroup takes at most
+// ceil(tiles/grid) of them, so it bins at most count/grid + 2 kTile points.
+__device__ __forceinline__ bool pack_fits(uint64_t count, uint32_t grid) {
+    return count < (uint64_t)grid * ((1u << 24) - 2 * kTile);
+}
+
+template <typename WS, typename Acc>
+__device__ __forceinline__ void bin_add(bool packed, uint32_t *scnt, Acc *sw, uint32_t *smin, int b, WS wv, float x) {
+    if constexpr (packable<WS, Acc>()) {
+        if (packed) {
+            atomicAdd((unsigned long long *)&sw[b], (1ull << kPackShift) | (unsigned long long)wv);
+        } else {
+            atomicAdd(&scnt[b], 1u);
+            atomic_add_acc(&sw[b], AccOps<Acc>::of(wv));
+        }
+    } else {
+        atomicAdd(&scnt[b], 1u);
+        atomic_add_acc(&sw[b], AccOps<Acc>::of(wv));
+    }
+    const uint32_t key = f32_key(x);
+    if (key < __atomic_load_n(&smin[b], __ATOMIC_RELAXED)) atomicMin(&smin[b], key);
+}
+
+template <typename Acc>
+__device__ __forceinline__ uint32_t bin_count(bool packed, const uint32_t *scnt, const Acc *sw, int j) {
+    if constexpr (std::is_same<Acc, uint64_t>::value)
+        if (packed) return (uint32_t)(sw[j] >> kPackShift);
+    return scnt[j];
+}
+
 template <typename Acc>
 __device__ inline void flush_lds_bins(uint64_t base, int nb, const uint32_t *scnt, const Acc *sw,
-                                      const uint32_t *smin, uint32_t *bcnt, Acc *bw, uint32_t *bmin) {
+                                      const uint32_t *smin, uint32_t *bcnt, Acc *bw, uint32_t *bmin,
+                                      bool packed = false) {
     for (int j = threadIdx.x; j < nb; j += kBlock) {
-        if (scnt[j]) {
-            atomicAdd(&bcnt[base + j], scnt[j]);
-            atomic_add_acc(&bw[base + j], sw[j]);
+        const uint32_t c = bin_count(packed, scnt, sw, j);
+        if (c) {
+            Acc wsum = sw[j];
+            if constexpr (std::is_same<Acc, uint64_t>::value)
+                if (packed) wsum &= (1ull << kPackShift) - 1;
+            atomicAdd(&bcnt[base + j], c);
+            atomic_add_acc(&bw[base + j], wsum);
             atomicMin(&bmin[base + j], smin[j]);
         }
     }
@@ -459,16 +509,18 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
     uint32_t before = 0;   // this thread's bin count of the part before the tile (tile_bins)
     const float *c = c0;
     float lo = 0.0f, scale = 0.0f;
+    bool packed = false;
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
             before = 0;
             __syncthreads();
-            if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin);
+            if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin, packed);
             __syncthreads();
             cur = T.part;
             const Part &P = parts[cur];
             active = P.status == kSearching;
+            packed = packable<WS, Acc>() && pack_fits(P.count, gridDim.x);
             if (active) {
                 for (int p = tid; p < nb - 1; p += kBlock) cand[p] = candidate(P.blo, P.bhi, p, nb);
                 for (int j = tid; j < nb; j += kBlock) { scnt[j] = 0; sw[j] = (Acc)0; smin[j] = 0xffffffffu; }
@@ -493,23 +545,20 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
         for (int k = 0; k < kItems; k++) {
             if ((uint32_t)(k * kBlock + tid) >= cnt) continue;
             const float x = xs[k];
-            const int b = bin_near(cand, nb, x, lo, scale);
-            atomicAdd(&scnt[b], 1u);
-            atomic_add_acc(&sw[b], AccOps<Acc>::of(wk[k]));
-            atomicMin(&smin[b], f32_key(x));
+            bin_add<WS, Acc>(packed, scnt, sw, smin, bin_near(cand, nb, x, lo, scale), wk[k], x);
         }
         if (tile_bins) {
             // each tile's own bin counts (the growth of the part's counts
             // over the tile) are kept for the partition's offsets
             __syncthreads();
-            const uint32_t after = tid < nb ? scnt[tid] : 0u;
+            const uint32_t after = tid < nb ? bin_count(packed, scnt, sw, tid) : 0u;
             if (tid < nb) tile_bins[(uint64_t)t * kMaxBins + tid] = (uint16_t)(after - before);
             before = after;
             __syncthreads();
         }
     }
     __syncthreads();
-    if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin);
+    if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin, packed);
 }
 
 // ---------------------------------------------------------------------------

@@ -286,6 +286,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
     bool all_left = false;
     uint32_t before0 = 0, before1 = 0;
     float clo[2] = {0.f, 0.f}, cscale[2] = {0.f, 0.f};
+    bool packed = false;   // both children's bins (the parent's points bound them)
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
@@ -294,9 +295,10 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
             before0 = before1 = 0;
             __syncthreads();
             for (int s = 0; s < 2; s++)
-                if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin);
+                if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin, packed);
             __syncthreads();
             cur = T.part;
+            packed = packable<WS, Acc>() && pack_fits(P.count, gridDim.x);
             {
                 const int ax = (int)P.axis, a2 = (int)((P.axis + 1) % D);
                 ca = ax == 0 ? c0 : (ax == 1 ? c1 : c2);
@@ -335,14 +337,11 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
             const int s = (!all_left && (xa[k] >= split)) ? 1 : 0;
             if (!act[s]) continue;
             const float x = xb[k];
-            const int b = bin_near(cand[s], nb, x, clo[s], cscale[s]);
-            atomicAdd(&scnt[s][b], 1u);
-            atomic_add_acc(&sw[s][b], AccOps<Acc>::of(wk[k]));
-            atomicMin(&smin[s][b], f32_key(x));
+            bin_add<WS, Acc>(packed, scnt[s], sw[s], smin[s], bin_near(cand[s], nb, x, clo[s], cscale[s]), wk[k], x);
         }
         __syncthreads();
         if (tid < nb) {
-            const uint32_t a0 = scnt[0][tid], a1 = scnt[1][tid];
+            const uint32_t a0 = bin_count(packed, scnt[0], sw[0], tid), a1 = bin_count(packed, scnt[1], sw[1], tid);
             uint16_t *tb = tile_bins2 + (uint64_t)t * 2 * kMaxBins;
             if (act[0]) tb[tid] = (uint16_t)(a0 - before0);
             if (act[1]) tb[kMaxBins + tid] = (uint16_t)(a1 - before1);
@@ -353,7 +352,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
     }
     __syncthreads();
     for (int s = 0; s < 2; s++)
-        if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin);
+        if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin, packed);
 }
 
 // Per tile of a parent, the points of each grandchild (q = 2 side + side of
@@ -607,6 +606,9 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const float *by = (a2 == 0 ? i0 : (a2 == 1 ? i1 : i2)) + T.begin;
             const float *bz = (a3 == 0 ? i0 : (a3 == 1 ? i1 : i2)) + T.begin;
             const WS *bwt = iw + T.begin;
+            // the wave issuing its loads goes ahead of waves busy classifying
+            // (keeps more reads in flight; measured 0.6% on the move)
+            __builtin_amdgcn_s_setprio(3);
 #pragma unroll
             for (int k = 0; k < kItems; k++) {
                 const uint32_t p = (uint32_t)(k * kBlock + tid);
@@ -620,6 +622,7 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 wv[k] = __builtin_nontemporal_load(&bwt[c]);
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         uint32_t pq[kItems];   // rank among the wavefront's points of the same slot | slot << 8
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
