@@ -18,6 +18,7 @@ constexpr int kItems = 8;            // points per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kMaxBins = 256;        // histogram bins per part per pass
 constexpr int kFirstBins = 4;        // bins of the register sweep (2 bisection steps)
+constexpr int kCopies = 8;           // copies of a search pass's bins, one per XCD
 
 // Part status during a level.
 enum : uint32_t {

@@ -69,6 +69,76 @@ __device__ inline uint32_t wave_min(uint32_t v) {
     return v;
 }
 
+// Wavefront totals without LDS: DPP row shifts leave each row's total in the
+// row's last lane (an inclusive scan within rows of 16), four readlanes
+// combine the rows.  Called with every lane of the wavefront active; the
+// result is uniform.
+template <int Ctrl, bool Bound>
+__device__ __forceinline__ uint32_t dpp_row(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, Ctrl, 0xf, 0xf, Bound);
+}
+template <int Ctrl>
+__device__ __forceinline__ uint64_t dpp_row64(uint64_t v) {   // shifted-in lanes read 0
+    return (uint64_t)dpp_row<Ctrl, true>(0u, (uint32_t)v) | (uint64_t)dpp_row<Ctrl, true>(0u, (uint32_t)(v >> 32)) << 32;
+}
+__device__ __forceinline__ uint32_t rows_last_lane(uint32_t v, int r) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 16 * r + 15);
+}
+__device__ __forceinline__ uint64_t rows_last_lane64(uint64_t v, int r) {
+    return (uint64_t)rows_last_lane((uint32_t)v, r) | (uint64_t)rows_last_lane((uint32_t)(v >> 32), r) << 32;
+}
+__device__ __forceinline__ uint32_t wave_total_min(uint32_t v) {
+    constexpr uint32_t kId = 0xffffffffu;
+    v = min(v, dpp_row<0x111, false>(kId, v));   // row_shr:1 (lanes shifted past the row's start keep kId)
+    v = min(v, dpp_row<0x112, false>(kId, v));   // row_shr:2
+    v = min(v, dpp_row<0x114, false>(kId, v));   // row_shr:4
+    v = min(v, dpp_row<0x118, false>(kId, v));   // row_shr:8
+    return min(min(rows_last_lane(v, 0), rows_last_lane(v, 1)), min(rows_last_lane(v, 2), rows_last_lane(v, 3)));
+}
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+    v += dpp_row<0x111, true>(0u, v);
+    v += dpp_row<0x112, true>(0u, v);
+    v += dpp_row<0x114, true>(0u, v);
+    v += dpp_row<0x118, true>(0u, v);
+    return rows_last_lane(v, 0) + rows_last_lane(v, 1) + rows_last_lane(v, 2) + rows_last_lane(v, 3);
+}
+__device__ __forceinline__ uint64_t wave_total(uint64_t v) {
+    v += dpp_row64<0x111>(v);
+    v += dpp_row64<0x112>(v);
+    v += dpp_row64<0x114>(v);
+    v += dpp_row64<0x118>(v);
+    return rows_last_lane64(v, 0) + rows_last_lane64(v, 1) + rows_last_lane64(v, 2) + rows_last_lane64(v, 3);
+}
+__device__ __forceinline__ double wave_total(double v) {   // 0 bits are 0.0
+    auto f = [](uint64_t b) { return __builtin_bit_cast(double, b); };
+    auto u = [](double d) { return __builtin_bit_cast(uint64_t, d); };
+    v += f(dpp_row64<0x111>(u(v)));
+    v += f(dpp_row64<0x112>(u(v)));
+    v += f(dpp_row64<0x114>(u(v)));
+    v += f(dpp_row64<0x118>(u(v)));
+    return (f(rows_last_lane64(u(v), 0)) + f(rows_last_lane64(u(v), 1))) +
+           (f(rows_last_lane64(u(v), 2)) + f(rows_last_lane64(u(v), 3)));
+}
+
+// Copies 1..7 of a search pass's bins (copy 0 is the bins themselves).
+// Workgroups go to the XCDs in turn (blockIdx mod 8), each XCD with its own
+// L2; with few parts, every workgroup of the grid flushes into the same few
+// lines, which then move between the L2s at each atomic.  The workgroups of
+// XCD c flush into copy c instead and the decision sums the copies (the
+// atomics stay device-wide: any placement is correct, only slower).
+// cnt == nullptr: no copies; parts: the copies' stride in parts.
+template <typename Acc>
+struct BinCopies {
+    uint32_t *cnt;
+    Acc *w;
+    uint32_t *mn;
+    uint32_t parts;
+};
+template <typename Acc>
+__device__ __forceinline__ uint64_t copy_index(const BinCopies<Acc> &xc, int c, uint32_t part, int j) {
+    return ((uint64_t)(c - 1) * xc.parts + part) * kMaxBins + j;
+}
+
 // v[a] for a uniform runtime axis a, as bit masks the compiler cannot turn
 // into an indexed (scratch) access to a register array.
 __device__ __forceinline__ float sel3(int a, float v0, float v1, float v2) {

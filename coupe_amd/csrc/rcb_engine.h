@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 #include <vector>
@@ -123,6 +124,7 @@ struct Context {
     DevBuf w16;                      // the prologue's 16-bit copy of the weights
     DevBuf parts[3], tiles[3];       // part / tile tables (two levels per move uses three)
     DevBuf bcnt, bw, bmin;           // per-part histogram bins
+    DevBuf xcnt, xw, xmin;           // per-XCD copies of the bins (levels of few parts)
     DevBuf nchild, ntiles;           // scan inputs / outputs (children, tiles per part)
     DevBuf bsum;                     // scan block sums
     DevBuf nodes, leaf;              // split tree of all levels, part ids of the leaves
@@ -140,6 +142,7 @@ struct Context {
     bool full_first = true;          // children's first pass with every candidate where forwarding mostly fails
     uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
     bool graphs = false;             // replay a call of a known shape as one captured graph
+    uint32_t copy_parts = 256;       // levels of at most this many parts flush into per-XCD copies
     int max_bins = kMaxBins;         // bins per part of the later search passes (16..256, a power of two)
     int wg_cap = 0;                  // workgroups per CU of the persistent kernels (0: as many as fit)
     // tuned switches per batch class (tune_key), chosen by coupe_hip_autotune
@@ -241,6 +244,15 @@ inline bool full_first_from_env() {
     return !(v && v[0] == '0');
 }
 
+// COUPE_HIP_BIN_COPIES=k: the search passes of levels of at most k parts
+// flush into one copy of the bins per XCD (default 256; 0: never).
+inline uint32_t copy_parts_from_env() {
+    const char *v = getenv("COUPE_HIP_BIN_COPIES");
+    if (!v || !*v) return 256;
+    const long long k = atoll(v);
+    return k < 0 ? 0u : (uint32_t)std::min<long long>(k, 1 << 20);
+}
+
 // COUPE_HIP_GRAPHS=1: calls of a shape seen before replay a captured graph
 // (default 0: kernel by kernel, unless the autotuner chose graphs for the
 // call's class).
@@ -295,6 +307,7 @@ inline void configure_from_env(Context &c) {
     c.full_first = full_first_from_env();
     c.scan_direct = scan_direct_from_env();
     c.graphs = graphs_from_env();
+    c.copy_parts = copy_parts_from_env();
     c.max_bins = bins_from_env();
     c.wg_cap = wg_cap_from_env();
 }

@@ -437,10 +437,23 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     uint64_t bins = 1;
     for (size_t l = 0; l < L; l++)
         bins = std::max<uint64_t>(bins, std::max<uint64_t>(B.parts_ub[l], 1) * (uint64_t)choose_bins(B.parts_ub[l], cx.max_bins));
-    const bool fresh_bins = cx.bcnt.bytes < bins * 4 || cx.bw.bytes < bins * 8 || cx.bmin.bytes < bins * 4;
+    // Copies 1..7 of the bins, one per XCD, for the levels of at most
+    // cx.copy_parts parts (the first levels of a call of few instances).
+    uint64_t xparts = 0;
+    for (size_t l = 0; l < L; l++)
+        if (B.parts_ub[l] <= cx.copy_parts) xparts = std::max<uint64_t>(xparts, B.parts_ub[l]);
+    const bool copies = xparts > 0;
+    const uint64_t xbins = (uint64_t)(kCopies - 1) * xparts * kMaxBins;
+    const bool fresh_bins = cx.bcnt.bytes < bins * 4 || cx.bw.bytes < bins * 8 || cx.bmin.bytes < bins * 4 ||
+                            cx.xcnt.bytes < xbins * 4 || cx.xw.bytes < xbins * 8 || cx.xmin.bytes < xbins * 4;
     cx.bcnt.ensure(bins * 4);
     cx.bw.ensure(bins * 8);
     cx.bmin.ensure(bins * 4);
+    if (copies) {
+        cx.xcnt.ensure(xbins * 4);
+        cx.xw.ensure(xbins * 8);
+        cx.xmin.ensure(xbins * 4);
+    }
     if (fresh_bins || !cx.bins_clean) {
         // k_decide leaves bins zeroed after reading them; only new or dirty
         // buffers need a clear (never in a captured call: it starts clean).
@@ -448,11 +461,21 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         check(hipMemsetAsync(cx.bcnt.ptr, 0, cx.bcnt.bytes, st));
         check(hipMemsetAsync(cx.bw.ptr, 0, cx.bw.bytes, st));
         check(hipMemsetAsync(cx.bmin.ptr, 0xff, cx.bmin.bytes, st));
+        if (cx.xcnt.bytes) {
+            check(hipMemsetAsync(cx.xcnt.ptr, 0, cx.xcnt.bytes, st));
+            check(hipMemsetAsync(cx.xw.ptr, 0, cx.xw.bytes, st));
+            check(hipMemsetAsync(cx.xmin.ptr, 0xff, cx.xmin.bytes, st));
+        }
     }
     cx.bins_clean = false;   // until this call completes
     uint32_t *bcnt = (uint32_t *)cx.bcnt.ptr;
     Acc *bw = (Acc *)cx.bw.ptr;
     uint32_t *bmin = (uint32_t *)cx.bmin.ptr;
+    // the copies a level of pub parts uses (none: nullptr)
+    auto copies_for = [&](uint64_t pub) {
+        if (!copies || pub > xparts) return BinCopies<Acc>{nullptr, nullptr, nullptr, 0u};
+        return BinCopies<Acc>{(uint32_t *)cx.xcnt.ptr, (Acc *)cx.xw.ptr, (uint32_t *)cx.xmin.ptr, (uint32_t)xparts};
+    };
 
     // Working arrays.  Coordinates: slot 0 is the prologue output in input
     // order and is kept for the final descent; levels 1, 3, ... use slot 1,
@@ -495,6 +518,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         Part *parts = (Part *)cx.parts[sl].ptr;
         uint32_t *np = &ctr->n_parts[sl];
         const int nb = choose_bins(parts_ub, cx.max_bins);
+        const BinCopies<Acc> xc = copies_for(parts_ub);
         cx.stats.bins = std::max<uint32_t>(cx.stats.bins, (uint32_t)nb);
         if (spec_passes > 0) {
             // slot p counts the parts still searching after decision p; pass
@@ -502,26 +526,26 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
             const int queued = depth[level];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
+                k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
                                                                &slot[0], swept_first, counts_first, nullptr,
-                                                               nullptr);
+                                                               nullptr, xc);
             });
             for (int p = 1; p <= queued; p++) {
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
                 // the last queued decision also closes the searches it leaves open
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                    k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(
+                    k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(
                         parts, np, bcnt, bw, bmin, nb, wrap32, &slot[p], &ctr->swept, counts_hist, &slot[p - 1],
-                        p == queued ? &ctr->unfinished : nullptr);
+                        p == queued ? &ctr->unfinished : nullptr, xc);
                 });
             }
             return;
         }
         check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-            k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
+            k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
                                                            &ctr->searching, swept_first, counts_first, nullptr,
-                                                           nullptr);
+                                                           nullptr, xc);
         });
         uint32_t passes = 1;
         for (;;) {
@@ -534,9 +558,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, nullptr); });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, 64, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
+                k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
                                                                &ctr->searching, &ctr->swept, counts_hist, nullptr,
-                                                               nullptr);
+                                                               nullptr, xc);
             });
             passes++;
         }
@@ -556,7 +580,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                    first_standalone ? &ctr->swept : &ctr->swept_fused, counts_first, hist_counts,
                    [&](int nb, const uint32_t *gate) {
                        k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt,
-                                                             bw, bmin, gate, tile_bins);
+                                                             bw, bmin, gate, tile_bins, copies_for(parts_ub));
                    });
     };
     auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, uint64_t n_parts_host,
@@ -668,7 +692,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             const unsigned gv = persistent_grid(cx, k_hist_virtual<D, WS, Acc>, tl_ub);
             auto hist_v = [&](int nb, const uint32_t *gate) {
                 k_hist_virtual<D, WS, Acc><<<gv, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2], wl, nb, bcnt,
-                                                                 bw, bmin, gate, tile_bins2);
+                                                                 bw, bmin, gate, tile_bins2, copies_for(pub1));
             };
             if (full_first) {
                 cx.stats.full_first++;
@@ -1097,7 +1121,8 @@ static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int
     mix((uint64_t)(uintptr_t)w);
     for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)cx.slot0[k]);
     mix((uint64_t)spec);
-    mix((uint64_t)cx.pairs | (uint64_t)cx.forward << 1 | (uint64_t)cx.weights16 << 2 | (uint64_t)cx.full_first << 3);
+    mix((uint64_t)cx.pairs | (uint64_t)cx.forward << 1 | (uint64_t)cx.weights16 << 2 | (uint64_t)cx.full_first << 3 |
+        (uint64_t)cx.copy_parts << 4);
     mix((uint64_t)cx.max_bins | (uint64_t)cx.wg_cap << 16);
     mix(cx.scan_direct);
     const Hints *e = cx.hints_for(B.shape);
@@ -1442,7 +1467,7 @@ void Context::for_each_buf(F &&f) {
     DevBuf *all[] = {&cx[0][0], &cx[0][1], &cx[0][2], &cx[1][0], &cx[1][1], &cx[1][2], &cx[2][0], &cx[2][1],
                      &cx[2][2], &w[0], &w[1], &w16, &parts[0], &parts[1], &parts[2], &tiles[0], &tiles[1],
                      &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn,
-                     &tile_bins2, &tile4, &bcnt, &bw, &bmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
+                     &tile_bins2, &tile4, &bcnt, &bw, &bmin, &xcnt, &xw, &xmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
                      &heap_split, &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins,
                      &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points,
                      &in_weights, &out_ids};

@@ -467,6 +467,71 @@ __device__ __forceinline__ void bin_add(bool packed, uint32_t *scnt, Acc *sw, ui
     if (key < __atomic_load_n(&smin[b], __ATOMIC_RELAXED)) atomicMin(&smin[b], key);
 }
 
+// The first and last bins of a pass hold every point below and above the
+// search window: in a refining pass most of them, so that the LDS atomics of
+// a wavefront would meet at two addresses.  Each lane sums its points of
+// those two bins in registers; once per tile the wavefront's totals go to
+// the bins, one atomic per bin (lane 0).
+template <typename WS, typename Acc>
+struct EdgeBins {
+    uint32_t c[2] = {0u, 0u};
+    Acc w[2] = {(Acc)0, (Acc)0};
+    uint32_t m[2] = {0xffffffffu, 0xffffffffu};
+
+    // Takes the point when bin b is the first or the last.
+    __device__ __forceinline__ bool take(int b, int nb, WS wv, float x) {
+        if (b != 0 && b != nb - 1) return false;
+        const uint32_t key = f32_key(x);
+        if (b == 0) { c[0]++; w[0] += AccOps<Acc>::of(wv); m[0] = min(m[0], key); }
+        else        { c[1]++; w[1] += AccOps<Acc>::of(wv); m[1] = min(m[1], key); }
+        return true;
+    }
+
+    // The whole wavefront, every lane active.
+    __device__ __forceinline__ void flush(bool packed, int nb, uint32_t *scnt, Acc *sw, uint32_t *smin) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            if (__ballot(c[e] != 0) == 0) continue;   // wave-uniform
+            const int b = e ? nb - 1 : 0;
+            const uint32_t mk = wave_total_min(m[e]);
+            bool done = false;
+            if constexpr (packable<WS, Acc>()) {
+                if (packed) {
+                    const uint64_t v = wave_total((uint64_t)c[e] << kPackShift | w[e]);
+                    if (lane_id() == 0) atomicAdd((unsigned long long *)&sw[b], (unsigned long long)v);
+                    done = true;
+                }
+            }
+            if (!done) {
+                const uint32_t cn = wave_total(c[e]);
+                const Acc ws = wave_total(w[e]);
+                if (lane_id() == 0) { atomicAdd(&scnt[b], cn); atomic_add_acc(&sw[b], ws); }
+            }
+            if (lane_id() == 0 && mk < smin[b]) atomicMin(&smin[b], mk);
+            c[e] = 0u;
+            w[e] = (Acc)0;
+            m[e] = 0xffffffffu;
+        }
+    }
+};
+
+// Where a workgroup flushes the bins of part `part`: the bins, or its XCD's
+// copy while the level uses copies.
+template <typename Acc>
+struct BinsDest {
+    uint32_t *cnt;
+    Acc *w;
+    uint32_t *mn;
+    uint64_t base;
+};
+template <typename Acc>
+__device__ __forceinline__ BinsDest<Acc> bins_dest(const BinCopies<Acc> &xc, uint32_t part, int nb, uint32_t *bcnt,
+                                                   Acc *bw, uint32_t *bmin) {
+    const int c = (int)(blockIdx.x % kCopies);
+    if (xc.cnt && c) return {xc.cnt, xc.w, xc.mn, copy_index(xc, c, part, 0)};
+    return {bcnt, bw, bmin, (uint64_t)part * nb};
+}
+
 template <typename Acc>
 __device__ __forceinline__ uint32_t bin_count(bool packed, const uint32_t *scnt, const Acc *sw, int j) {
     if constexpr (std::is_same<Acc, uint64_t>::value)
@@ -496,7 +561,7 @@ __global__ void __launch_bounds__(kBlock)
 k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-       uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins) {
+       uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins, BinCopies<Acc> xc) {
     if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
@@ -510,12 +575,16 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
     const float *c = c0;
     float lo = 0.0f, scale = 0.0f;
     bool packed = false;
+    auto flush = [&]() {
+        const BinsDest<Acc> d = bins_dest(xc, cur, nb, bcnt, bw, bmin);
+        flush_lds_bins(d.base, nb, scnt, sw, smin, d.cnt, d.w, d.mn, packed);
+    };
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
             before = 0;
             __syncthreads();
-            if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin, packed);
+            if (active) flush();
             __syncthreads();
             cur = T.part;
             const Part &P = parts[cur];
@@ -541,12 +610,15 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
             xs[k] = c[T.begin + p];
             wk[k] = w[T.begin + p];
         }
+        EdgeBins<WS, Acc> edge;
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             if ((uint32_t)(k * kBlock + tid) >= cnt) continue;
             const float x = xs[k];
-            bin_add<WS, Acc>(packed, scnt, sw, smin, bin_near(cand, nb, x, lo, scale), wk[k], x);
+            const int b = bin_near(cand, nb, x, lo, scale);
+            if (!edge.take(b, nb, wk[k], x)) bin_add<WS, Acc>(packed, scnt, sw, smin, b, wk[k], x);
         }
+        edge.flush(packed, nb, scnt, sw, smin);
         if (tile_bins) {
             // each tile's own bin counts (the growth of the part's counts
             // over the tile) are kept for the partition's offsets
@@ -558,18 +630,18 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
         }
     }
     __syncthreads();
-    if (active) flush_lds_bins((uint64_t)cur * nb, nb, scnt, sw, smin, bcnt, bw, bmin, packed);
+    if (active) flush();
 }
 
 // ---------------------------------------------------------------------------
 // Decision replay: one wavefront per searching part.
 // ---------------------------------------------------------------------------
 template <typename Acc>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(kBlock)
 k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
          Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
          uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
-         const uint32_t *gate, uint32_t *unfinished) {
+         const uint32_t *gate, uint32_t *unfinished, BinCopies<Acc> xc) {
     // unfinished (non-null): the last queued decision of a speculative
     // search; a part still searching after it is closed as "all left" (a
     // consistent tree for the kernels that follow) and the call is flagged,
@@ -578,16 +650,46 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
     __shared__ uint32_t sm[kMaxBins + 1];
+    __shared__ uint32_t gc[kMaxBins];   // the pass's bins, gathered by the workgroup
+    __shared__ Acc gw[kMaxBins];
+    __shared__ uint32_t gm[kMaxBins];
     const uint32_t s = blockIdx.x;
     if (s >= *n_parts) return;
     Part *P = &parts[s];
     if (P->status != kSearching) return;
+    {
+        // one bin per thread: the bins and their copies read (all loads in
+        // flight at once) and left cleared for the next pass
+        const int j = threadIdx.x;
+        if (j < nb) {
+            const uint64_t g = (uint64_t)s * nb + j;
+            uint32_t cn = bcnt[g];
+            Acc wt = bw[g];
+            uint32_t mn = bmin[g];
+            bcnt[g] = 0; bw[g] = (Acc)0; bmin[g] = 0xffffffffu;   // ready for the next pass
+            if (xc.cnt) {
+#pragma unroll
+                for (int cp = 1; cp < kCopies; cp++) {
+                    const uint64_t x = copy_index(xc, cp, s, j);
+                    cn += xc.cnt[x]; wt += xc.w[x]; mn = min(mn, xc.mn[x]);
+                }
+#pragma unroll
+                for (int cp = 1; cp < kCopies; cp++) {
+                    const uint64_t x = copy_index(xc, cp, s, j);
+                    xc.cnt[x] = 0; xc.w[x] = (Acc)0; xc.mn[x] = 0xffffffffu;
+                }
+            }
+            gc[j] = cn; gw[j] = wt; gm[j] = mn;
+        }
+        __syncthreads();
+    }
+    // the first wavefront scans the bins (the others hold no bins: j0 >= nb)
+    // and its lane 0 replays the decisions
     const int lane = threadIdx.x;
     if (lane == 0) atomicAdd(points_swept, (unsigned long long)P->count);   // swept by the last pass
     const int per = nb >= 64 ? nb / 64 : 1;
     const int j0 = lane * per;
     const bool mine = j0 < nb;
-    const uint64_t base = (uint64_t)s * nb;
     uint64_t c[4];
     Acc w[4];
     uint32_t m[4];
@@ -597,11 +699,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         c[k] = 0; w[k] = (Acc)0; m[k] = 0xffffffffu;
-        if (k < per && mine) {
-            uint64_t j = base + j0 + k;
-            c[k] = bcnt[j]; w[k] = bw[j]; m[k] = bmin[j];
-            bcnt[j] = 0; bw[j] = (Acc)0; bmin[j] = 0xffffffffu;   // ready for the next pass
-        }
+        if (k < per && mine) { c[k] = gc[j0 + k]; w[k] = gw[j0 + k]; m[k] = gm[j0 + k]; }
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) { lc += c[k]; lw += w[k]; }

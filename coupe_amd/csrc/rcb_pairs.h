@@ -270,7 +270,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
                const Part *__restrict__ children, const float *__restrict__ c0, const float *__restrict__ c1,
                const float *__restrict__ c2, const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt,
                Acc *__restrict__ bw, uint32_t *__restrict__ bmin, const uint32_t *gate,
-               uint16_t *__restrict__ tile_bins2) {
+               uint16_t *__restrict__ tile_bins2, BinCopies<Acc> xc) {
     if (gate && *gate == 0) return;
     __shared__ float cand[2][kMaxBins];
     __shared__ uint32_t scnt[2][kMaxBins];
@@ -287,6 +287,10 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
     uint32_t before0 = 0, before1 = 0;
     float clo[2] = {0.f, 0.f}, cscale[2] = {0.f, 0.f};
     bool packed = false;   // both children's bins (the parent's points bound them)
+    auto flush = [&](int s) {
+        const BinsDest<Acc> d = bins_dest(xc, (uint32_t)ch[s], nb, bcnt, bw, bmin);
+        flush_lds_bins(d.base, nb, scnt[s], sw[s], smin[s], d.cnt, d.w, d.mn, packed);
+    };
     for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
@@ -295,7 +299,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
             before0 = before1 = 0;
             __syncthreads();
             for (int s = 0; s < 2; s++)
-                if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin, packed);
+                if (act[s]) flush(s);
             __syncthreads();
             cur = T.part;
             packed = packable<WS, Acc>() && pack_fits(P.count, gridDim.x);
@@ -331,14 +335,19 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
             xb[k] = cb[T.begin + p];
             wk[k] = w[T.begin + p];
         }
+        EdgeBins<WS, Acc> edge0, edge1;
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             if ((uint32_t)(k * kBlock + tid) >= cnt) continue;
             const int s = (!all_left && (xa[k] >= split)) ? 1 : 0;
             if (!act[s]) continue;
             const float x = xb[k];
-            bin_add<WS, Acc>(packed, scnt[s], sw[s], smin[s], bin_near(cand[s], nb, x, clo[s], cscale[s]), wk[k], x);
+            const int b = bin_near(cand[s], nb, x, clo[s], cscale[s]);
+            const bool taken = s ? edge1.take(b, nb, wk[k], x) : edge0.take(b, nb, wk[k], x);
+            if (!taken) bin_add<WS, Acc>(packed, scnt[s], sw[s], smin[s], b, wk[k], x);
         }
+        edge0.flush(packed, nb, scnt[0], sw[0], smin[0]);
+        edge1.flush(packed, nb, scnt[1], sw[1], smin[1]);
         __syncthreads();
         if (tid < nb) {
             const uint32_t a0 = bin_count(packed, scnt[0], sw[0], tid), a1 = bin_count(packed, scnt[1], sw[1], tid);
@@ -352,7 +361,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
     }
     __syncthreads();
     for (int s = 0; s < 2; s++)
-        if (act[s]) flush_lds_bins((uint64_t)ch[s] * nb, nb, scnt[s], sw[s], smin[s], bcnt, bw, bmin, packed);
+        if (act[s]) flush(s);
 }
 
 // Per tile of a parent, the points of each grandchild (q = 2 side + side of
