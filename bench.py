@@ -69,7 +69,9 @@ def algo_bytes_per_point(dim: int, levels: int) -> int:
     return (12 * dim + 20) + 12 * levels + (levels - 1) * (8 * dim + 24) + 12
 
 
-TRAFFIC_PROFILES = {("jgrid512", "partition"): "profiles/rcb_jgrid512_hbm_traffic.md"}
+TRAFFIC_PROFILES = {("jgrid512", "partition"): "profiles/rcb_jgrid512_hbm_traffic.md",
+                    ("gauss3d", "hist"): "profiles/rcb_gauss3d_hbm_traffic.md",
+                    ("gauss3d", "partition"): "profiles/rcb_gauss3d_hbm_traffic.md"}
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -82,7 +84,10 @@ def measured_traffic(workload: str, kernel: str):
     text = open(os.path.join(ROOT, path)).read()
     block = text.split("```json", 1)[-1].split("```", 1)[0] if "```json" in text else ""
     try:
-        return int(json.loads(block)["traffic_bytes"])
+        data = json.loads(block)
+        if "classes" in data:   # per kernel class, averaged over the class's dispatches
+            return int(data["classes"][kernel]["traffic_bytes"])
+        return int(data["traffic_bytes"])
     except (ValueError, KeyError):
         return None
 
