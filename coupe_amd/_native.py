@@ -88,7 +88,7 @@ ITH_FN = C.CFUNCTYPE(C.c_void_p, C.c_void_p, C.c_size_t)
 
 class Tuning(C.Structure):
     _fields_ = [("bins", C.c_uint32), ("workgroups_per_cu", C.c_uint32), ("search_passes", C.c_uint32),
-                ("full_first", C.c_uint32), ("graphs", C.c_uint32)]
+                ("full_first", C.c_uint32), ("graphs", C.c_uint32), ("search_chunk", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

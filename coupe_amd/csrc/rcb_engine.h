@@ -143,6 +143,7 @@ struct Context {
     uint64_t scan_direct = 2048;     // scans of more blocks scan their block totals first
     bool graphs = false;             // replay a call of a known shape as one captured graph
     uint32_t copy_parts = 256;       // levels of at most this many parts flush into per-XCD copies
+    uint32_t search_chunk = 2;       // consecutive tiles a workgroup takes at a time in the search passes
     int max_bins = kMaxBins;         // bins per part of the later search passes (16..256, a power of two)
     int wg_cap = 0;                  // workgroups per CU of the persistent kernels (0: as many as fit)
     // tuned switches per batch class (tune_key), chosen by coupe_hip_autotune
@@ -253,6 +254,14 @@ inline uint32_t copy_parts_from_env() {
     return k < 0 ? 0u : (uint32_t)std::min<long long>(k, 1 << 20);
 }
 
+// COUPE_HIP_SEARCH_CHUNK=k: the later search passes hand k consecutive
+// tiles at a time to each workgroup (default 2, at most 64).
+inline uint32_t search_chunk_from_env() {
+    const char *v = getenv("COUPE_HIP_SEARCH_CHUNK");
+    const int k = v && *v ? atoi(v) : 2;
+    return (uint32_t)(k < 1 ? 1 : (k > 64 ? 64 : k));
+}
+
 // COUPE_HIP_GRAPHS=1: calls of a shape seen before replay a captured graph
 // (default 0: kernel by kernel, unless the autotuner chose graphs for the
 // call's class).
@@ -308,6 +317,7 @@ inline void configure_from_env(Context &c) {
     c.scan_direct = scan_direct_from_env();
     c.graphs = graphs_from_env();
     c.copy_parts = copy_parts_from_env();
+    c.search_chunk = search_chunk_from_env();
     c.max_bins = bins_from_env();
     c.wg_cap = wg_cap_from_env();
 }

@@ -580,7 +580,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                    first_standalone ? &ctr->swept : &ctr->swept_fused, counts_first, hist_counts,
                    [&](int nb, const uint32_t *gate) {
                        k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt,
-                                                             bw, bmin, gate, tile_bins, copies_for(parts_ub));
+                                                             bw, bmin, gate, tile_bins, copies_for(parts_ub),
+                                                             cx.search_chunk);
                    });
     };
     auto search = [&](int sl, size_t level, uint64_t parts_ub, uint64_t tiles_ub, uint64_t n_parts_host,
@@ -692,7 +693,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             const unsigned gv = persistent_grid(cx, k_hist_virtual<D, WS, Acc>, tl_ub);
             auto hist_v = [&](int nb, const uint32_t *gate) {
                 k_hist_virtual<D, WS, Acc><<<gv, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2], wl, nb, bcnt,
-                                                                 bw, bmin, gate, tile_bins2, copies_for(pub1));
+                                                                 bw, bmin, gate, tile_bins2, copies_for(pub1),
+                                                                 cx.search_chunk);
             };
             if (full_first) {
                 cx.stats.full_first++;
@@ -1057,7 +1059,7 @@ static bool run_dim(Context &cx, hipStream_t st, uint64_t *d_out, const BatchPla
 
 coupe_hip_tuning current_tuning(const Context &cx) {
     return coupe_hip_tuning{(uint32_t)cx.max_bins, (uint32_t)cx.wg_cap, (uint32_t)cx.search_passes,
-                            cx.full_first ? 1u : 0u, cx.graphs ? 1u : 0u};
+                            cx.full_first ? 1u : 0u, cx.graphs ? 1u : 0u, cx.search_chunk};
 }
 
 void apply_tuning(Context &cx, const coupe_hip_tuning &t) {
@@ -1068,6 +1070,7 @@ void apply_tuning(Context &cx, const coupe_hip_tuning &t) {
     cx.search_passes = (int)std::max<uint32_t>(2, std::min<uint32_t>(t.search_passes, 8));
     cx.full_first = t.full_first != 0;
     cx.graphs = t.graphs != 0;
+    cx.search_chunk = std::max<uint32_t>(1, std::min<uint32_t>(t.search_chunk, 64));
 }
 
 static uint64_t tune_key(const BatchPlan &B, size_t dim) {
@@ -1122,7 +1125,7 @@ static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int
     for (int k = 0; k < 3; k++) mix((uint64_t)(uintptr_t)cx.slot0[k]);
     mix((uint64_t)spec);
     mix((uint64_t)cx.pairs | (uint64_t)cx.forward << 1 | (uint64_t)cx.weights16 << 2 | (uint64_t)cx.full_first << 3 |
-        (uint64_t)cx.copy_parts << 4);
+        (uint64_t)cx.copy_parts << 4 | (uint64_t)cx.search_chunk << 40);
     mix((uint64_t)cx.max_bins | (uint64_t)cx.wg_cap << 16);
     mix(cx.scan_direct);
     const Hints *e = cx.hints_for(B.shape);
@@ -1247,15 +1250,17 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
         Context &c;
         int bins, wg, passes;
         bool full, graphs;
+        uint32_t chunk;
         TuneScope(Context &c_)
             : c(c_), bins(c_.max_bins), wg(c_.wg_cap), passes(c_.search_passes), full(c_.full_first),
-              graphs(c_.graphs) {}
+              graphs(c_.graphs), chunk(c_.search_chunk) {}
         ~TuneScope() {
             c.max_bins = bins;
             c.wg_cap = wg;
             c.search_passes = passes;
             c.full_first = full;
             c.graphs = graphs;
+            c.search_chunk = chunk;
         }
     } tune_scope(cx);
     try {
@@ -1384,15 +1389,17 @@ enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const doub
         Context &c;
         int bins, wg, passes;
         bool full, graphs;
+        uint32_t chunk;
         ~Restore() {
             c.max_bins = bins;
             c.wg_cap = wg;
             c.search_passes = passes;
             c.full_first = full;
             c.graphs = graphs;
+            c.search_chunk = chunk;
             c.tuning_now = false;
         }
-    } restore{cx, cx.max_bins, cx.wg_cap, cx.search_passes, cx.full_first, cx.graphs};
+    } restore{cx, cx.max_bins, cx.wg_cap, cx.search_passes, cx.full_first, cx.graphs, cx.search_chunk};
     cx.tuning_now = true;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1437,6 +1444,7 @@ enum coupe_err autotune(Context &cx, hipStream_t st, uint64_t *d_out, const doub
     try_set(&coupe_hip_tuning::bins, {256, 128, 64, 32});
     try_set(&coupe_hip_tuning::workgroups_per_cu, {0, 4, 2, 1});
     try_set(&coupe_hip_tuning::full_first, {1, 0});
+    try_set(&coupe_hip_tuning::search_chunk, {2, 1, 4});
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!ok) return err != COUPE_ERR_OK ? err : COUPE_ERR_CRASH;

@@ -561,7 +561,8 @@ __global__ void __launch_bounds__(kBlock)
 k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uint32_t *n_tiles,
        const float *__restrict__ c0, const float *__restrict__ c1, const float *__restrict__ c2,
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
-       uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins, BinCopies<Acc> xc) {
+       uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins, BinCopies<Acc> xc,
+       uint32_t chunk) {
     if (gate && *gate == 0) return;   // no part left searching after the previous decision
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
@@ -579,7 +580,10 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
         const BinsDest<Acc> d = bins_dest(xc, cur, nb, bcnt, bw, bmin);
         flush_lds_bins(d.base, nb, scnt, sw, smin, d.cnt, d.w, d.mn, packed);
     };
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    // chunks of `chunk` consecutive tiles go to the workgroups in turn: a
+    // workgroup stays on one part over a chunk (one bin setup and flush)
+    for (uint32_t t0 = blockIdx.x * chunk; t0 < nt; t0 += gridDim.x * chunk)
+    for (uint32_t t = t0; t < min(nt, t0 + chunk); t++) {
         const Tile T = tiles[t];
         if (T.part != cur) {
             before = 0;

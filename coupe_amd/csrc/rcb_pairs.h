@@ -270,7 +270,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
                const Part *__restrict__ children, const float *__restrict__ c0, const float *__restrict__ c1,
                const float *__restrict__ c2, const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt,
                Acc *__restrict__ bw, uint32_t *__restrict__ bmin, const uint32_t *gate,
-               uint16_t *__restrict__ tile_bins2, BinCopies<Acc> xc) {
+               uint16_t *__restrict__ tile_bins2, BinCopies<Acc> xc, uint32_t chunk) {
     if (gate && *gate == 0) return;
     __shared__ float cand[2][kMaxBins];
     __shared__ uint32_t scnt[2][kMaxBins];
@@ -291,7 +291,9 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
         const BinsDest<Acc> d = bins_dest(xc, (uint32_t)ch[s], nb, bcnt, bw, bmin);
         flush_lds_bins(d.base, nb, scnt[s], sw[s], smin[s], d.cnt, d.w, d.mn, packed);
     };
-    for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    // chunks of `chunk` consecutive tiles go to the workgroups in turn
+    for (uint32_t t0 = blockIdx.x * chunk; t0 < nt; t0 += gridDim.x * chunk)
+    for (uint32_t t = t0; t < min(nt, t0 + chunk); t++) {
         const Tile T = tiles[t];
         const Part &P = parts[T.part];
         if (!children_searched(P)) continue;

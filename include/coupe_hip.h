@@ -126,12 +126,14 @@ struct coupe_hip_tuning {
 	uint32_t search_passes;
 	uint32_t full_first;
 	uint32_t graphs;        /* 1: a call of a shape seen before replays a captured HIP graph */
+	uint32_t search_chunk;  /* consecutive tiles a workgroup takes at a time in the later search passes */
 };
 
 /*
  * Autotuner: runs the batch of coupe_hip_rcb_batch_mixed (same arguments)
  * under candidate switches, one switch at a time from the current ones
- * (graphs, bins, workgroups per CU, the full first pass), each candidate timed with HIP events over `trials` calls after a
+ * (graphs, bins, workgroups per CU, the full first pass, the search chunk), each candidate timed with HIP events over
+ * `trials` calls after a
  * warm-up, and keeps the fastest for the batch's class: its dimension, the
  * log2 of its point count, its deepest iter_count and the log4 of its
  * instance count.  Later calls of that class on the context run with those

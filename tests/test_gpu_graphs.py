@@ -170,18 +170,21 @@ def test_autotuned_switches_keep_the_ids(oracle_mod):
         torch.cuda.synchronize()
         st = ctx.stats()
         assert st["tuned"] == 1 and st["tuning"]["bins"] == chosen["bins"], st
+        assert st["tuning"]["search_chunk"] == chosen["search_chunk"], st
         assert np.array_equal(out.cpu().numpy().astype(np.uint64), expect)
     finally:
         ctx.close()
 
 
-@pytest.mark.parametrize("bins,wg", [(16, 1), (32, 2), (64, 0), (128, 4)])
-def test_switches_from_the_environment(oracle_mod, monkeypatch, bins, wg):
-    """COUPE_HIP_BINS / COUPE_HIP_WG_PER_CU: the values the tuner tries, on
-    clustered clouds whose searches take several passes."""
+@pytest.mark.parametrize("bins,wg,chunk", [(16, 1, 1), (32, 2, 4), (64, 0, 3), (128, 4, 2), (256, 0, 1)])
+def test_switches_from_the_environment(oracle_mod, monkeypatch, bins, wg, chunk):
+    """COUPE_HIP_BINS / COUPE_HIP_WG_PER_CU / COUPE_HIP_SEARCH_CHUNK: the
+    values the tuner tries, on clustered clouds whose searches take several
+    passes."""
     torch = pytest.importorskip("torch")
     monkeypatch.setenv("COUPE_HIP_BINS", str(bins))
     monkeypatch.setenv("COUPE_HIP_WG_PER_CU", str(wg))
+    monkeypatch.setenv("COUPE_HIP_SEARCH_CHUNK", str(chunk))
     rng = np.random.default_rng(74 + bins)
     ctx = coupe_amd.DeviceContext(0)
     try:
@@ -195,7 +198,7 @@ def test_switches_from_the_environment(oracle_mod, monkeypatch, bins, wg):
             coupe_amd.Rcb(iter_count=it, tolerance=tol).partition(
                 out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
             torch.cuda.synchronize()
-            assert ctx.stats()["tuning"]["bins"] == bins
+            assert ctx.stats()["tuning"]["bins"] == bins and ctx.stats()["tuning"]["search_chunk"] == chunk
             assert np.array_equal(out.cpu().numpy().astype(np.uint64), _ids(oracle_mod, pts, w, it, tol))
     finally:
         ctx.close()
