@@ -104,6 +104,16 @@ struct Rotation {
 constexpr uint32_t kCountsRead = 0, kCountsBelow = 1, kCountsBins = 2;
 
 // Results of the prologue reductions over the whole batch.
+// One copy of an instance's prologue reductions (ordered keys of the f64
+// box, the wrapping integer and the f64 weight sums); the prologue's
+// workgroups on XCD c reduce into copy c and the roots' set-up combines them.
+struct InstReduce {
+    uint64_t bmin[kMaxDim];
+    uint64_t bmax[kMaxDim];
+    uint64_t sum_u64;
+    double sum_f64;
+};
+
 struct Prologue {
     double sum_abs;             // sum of |w| (exactness guard)
     uint64_t max_abs_bits;      // bits of max |w| (non-negative doubles order as integers)

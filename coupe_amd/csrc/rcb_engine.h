@@ -184,6 +184,7 @@ struct Context {
     coupe_hip_rib_info rib{};        // Rib: rotation of the last call
     const Rotation *rot = nullptr;   // change of basis for the running call
     DevBuf prologue, counters;
+    DevBuf ired;                     // per-XCD copies of the prologue's instance reductions
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     static constexpr int kStageBufs = 3;
     void *stage_buf[kStageBufs] = {};        // pinned chunks of the host-memory ABI's copies

@@ -499,7 +499,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(
             insts, n_inst, (Part *)cx.parts[0].ptr, D, real_mode, root_fwd ? (uint32_t *)cx.fwd_cnt.ptr : nullptr,
             (uint64_t *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr, &ctr->n_parts[0], &ctr->n_tiles[0],
-            (uint32_t)B.tiles0);
+            (uint32_t)B.tiles0, (const InstReduce *)cx.ired.ptr);
     });
 
     cx.stats.levels = 0;
@@ -891,9 +891,23 @@ static int weight_mode_of(const Prologue &p, int wkind, bool weights16) {
     return p.not_i32 ? 2 : 1;
 }
 
-static Prologue *prologue_readback(Context &cx) {
-    static_assert(kHostScratch >= 8192 + sizeof(Prologue), "prologue readback");
+// The prologue's flags are kept in one copy per XCD; the host reads the
+// copies (into the context's readback area) and combines them.
+static Prologue *prologue_copies(Context &cx) {
+    static_assert(kHostScratch >= 8192 + kCopies * sizeof(Prologue), "prologue readback");
     return (Prologue *)((char *)cx.host_scratch + 8192);
+}
+static Prologue prologue_readback(Context &cx) {
+    const Prologue *c = prologue_copies(cx);
+    Prologue p = c[0];
+    for (int k = 1; k < kCopies; k++) {
+        p.sum_abs += c[k].sum_abs;
+        p.max_abs_bits = std::max(p.max_abs_bits, c[k].max_abs_bits);   // non-negative doubles order as integers
+        p.non_integral |= c[k].non_integral;
+        p.not_i32 |= c[k].not_i32;
+        p.not_u16 |= c[k].not_u16;
+    }
+    return p;
 }
 
 // Once the call has run: the weight mode its data needs becomes the hint;
@@ -901,7 +915,7 @@ static Prologue *prologue_readback(Context &cx) {
 // mode holds any narrower one; f64 and the integer modes differ), and the
 // call must run again.
 static bool mode_held(Context &cx) {
-    const int best = weight_mode_of(*prologue_readback(cx), cx.pending.wkind, cx.weights16);
+    const int best = weight_mode_of(prologue_readback(cx), cx.pending.wkind, cx.weights16);
     cx.weight_hint = best;
     cx.remember(cx.pending.shape).weight_hint = best;
     const int mode = cx.pending.mode;
@@ -917,8 +931,8 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
         check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
         // bounding boxes start empty (min key = max, max key = 0), flags clear
-        k_reset_instances<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>((Instance *)cx.instances.ptr, n_inst,
-                                                                        (Prologue *)cx.prologue.ptr);
+        k_reset_instances<<<grid_for(std::max<uint32_t>(n_inst, kCopies), kBlock), kBlock, 0, st>>>(
+            (Instance *)cx.instances.ptr, n_inst, (Prologue *)cx.prologue.ptr, (InstReduce *)cx.ired.ptr);
         // blocks (instance, share of its tiles): about 4 tiles per thread
         uint64_t max_nt = 1;
         for (const Instance &I : B.inst) max_nt = std::max<uint64_t>(max_nt, (I.count + kTile - 1) / kTile);
@@ -949,7 +963,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
                 kernel<<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr,
                                              (float *)cx.cx[0][1].ptr, (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
                                              w32, w16, (Instance *)cx.instances.ptr,
-                                             (Prologue *)cx.prologue.ptr, rot);
+                                             (Prologue *)cx.prologue.ptr, rot, (InstReduce *)cx.ired.ptr, n_inst);
             });
         };
         if (cx.slot0[0]) {
@@ -960,7 +974,8 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
                 k_prologue_soa<D, WIN><<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, cx.coords0(0),
                                                              cx.coords0(1), D > 2 ? cx.coords0(2) : nullptr, win,
                                                              w32, w16, (Instance *)cx.instances.ptr,
-                                                             (Prologue *)cx.prologue.ptr);
+                                                             (Prologue *)cx.prologue.ptr, (InstReduce *)cx.ired.ptr,
+                                                             n_inst);
             });
         } else if (cx.rot) {
             launch(k_prologue<D, WIN, true>);
@@ -969,7 +984,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
         }
     }
     // without `wait`, h is pinned memory read once the call's last sync is done
-    check(hipMemcpyAsync(h, cx.prologue.ptr, sizeof(Prologue), hipMemcpyDeviceToHost, st));
+    check(hipMemcpyAsync(h, cx.prologue.ptr, kCopies * sizeof(Prologue), hipMemcpyDeviceToHost, st));
     if (wait) check(hipStreamSynchronize(st));
 }
 
@@ -981,7 +996,8 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     for (int b = cx.slot0[0] ? 1 : 0; b < 3; b++)
         for (int k = 0; k < D; k++) cx.cx[b][k].ensure(std::max<uint64_t>(n, 1) * sizeof(float));
     cx.w[0].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
-    cx.prologue.ensure(sizeof(Prologue));
+    cx.prologue.ensure(kCopies * sizeof(Prologue));
+    cx.ired.ensure(kCopies * std::max<size_t>(B.inst.size(), 1) * sizeof(InstReduce));
     cx.counters.ensure(sizeof(Counters));
     cx.instances.ensure(std::max<size_t>(B.inst.size(), 1) * sizeof(Instance));
     cx.parts[0].ensure(std::max<uint64_t>(B.parts_cap, B.inst.size()) * sizeof(Part));
@@ -1023,7 +1039,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     // prologue's flags are then read with the call's last counters, without a
     // wait after the prologue, and a call whose data does not fit that mode
     // runs again in its own.  A new context waits for the flags first.
-    Prologue *hp = prologue_readback(cx);
+    Prologue *hp = prologue_copies(cx);
     for (;;) {
         const Hints *seen = cx.hints_for(B.shape);
         const int hint = seen ? seen->weight_hint : cx.weight_hint;
@@ -1034,7 +1050,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         if (hint >= 0) cx.w[1].ensure(std::max<uint64_t>(n, 1) * weight_bytes(hint));
         prologue<D, WIN>(cx, st, B, pts, win, hp, hint < 0, hint);
         if (hint < 0) {
-            mode = best_mode(*hp);
+            mode = best_mode(prologue_readback(cx));
             if (mode >= 2) cx.w[0].ensure(std::max<uint64_t>(n, 1) * 8u);
         }
         cx.pending.mode = mode;
@@ -1477,7 +1493,7 @@ void Context::for_each_buf(F &&f) {
                      &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn,
                      &tile_bins2, &tile4, &bcnt, &bw, &bmin, &xcnt, &xw, &xmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
                      &heap_split, &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins,
-                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &counters, &in_points,
+                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &ired, &counters, &in_points,
                      &in_weights, &out_ids};
     for (DevBuf *d : all) f(*d);
 }

@@ -38,7 +38,7 @@ constexpr int kProPoints = kBlock * 4;   // points per LDS transpose round
 // Per-instance bounding box and weight sums, reduced over the workgroup and
 // added to the instance with one atomic per field.
 template <int D>
-__device__ __forceinline__ void flush_instance(Instance *I, double (&mn)[D], double (&mx)[D], uint64_t &su,
+__device__ __forceinline__ void flush_instance(InstReduce *I, double (&mn)[D], double (&mx)[D], uint64_t &su,
                                                double &sf, double *s_d, uint64_t *s_u) {
     const int lane = lane_id(), wid = threadIdx.x >> 6;
 #pragma unroll
@@ -133,8 +133,10 @@ struct WeightScan {
         }
     }
 
-    // the flags and |w| statistics of the whole call (one atomic per wavefront)
-    __device__ __forceinline__ void flush_flags(Prologue *out) {
+    // the flags and |w| statistics of the whole call: the workgroup's, into
+    // its XCD's copy of the flags (blockIdx mod kCopies; the host combines
+    // the copies).  Every thread; s_d holds kBlock / 64 * 8 doubles.
+    __device__ __forceinline__ void flush_flags(Prologue *out, double *s_d) {
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
             sa += __shfl_xor(sa, o, 64);
@@ -144,12 +146,28 @@ struct WeightScan {
             not32 |= __shfl_xor(not32, o, 64);
             not16 |= __shfl_xor(not16, o, 64);
         }
+        __syncthreads();   // s_d free
+        const int wid = threadIdx.x >> 6;
         if (lane_id() == 0) {
-            atomicAdd(&out->sum_abs, sa);
-            atomicMax((unsigned long long *)&out->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, ma));
-            if (nonint) atomicOr(&out->non_integral, 1u);
-            if (not32) atomicOr(&out->not_i32, 1u);
-            if (not16) atomicOr(&out->not_u16, 1u);
+            s_d[wid * 4] = sa;
+            s_d[wid * 4 + 1] = ma;
+            s_d[wid * 4 + 2] = (double)(nonint | not32 << 1 | not16 << 2);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double a = 0.0, m = 0.0;
+            uint32_t f = 0;
+            for (int v = 0; v < kBlock / 64; v++) {
+                a += s_d[v * 4];
+                m = s_d[v * 4 + 1] > m ? s_d[v * 4 + 1] : m;
+                f |= (uint32_t)s_d[v * 4 + 2];
+            }
+            Prologue *c = &out[blockIdx.x % kCopies];
+            atomicAdd(&c->sum_abs, a);
+            atomicMax((unsigned long long *)&c->max_abs_bits, (unsigned long long)__builtin_bit_cast(uint64_t, m));
+            if (f & 1u) atomicOr(&c->non_integral, 1u);
+            if (f & 2u) atomicOr(&c->not_i32, 1u);
+            if (f & 4u) atomicOr(&c->not_u16, 1u);
         }
     }
 };
@@ -164,7 +182,7 @@ __global__ void __launch_bounds__(kBlock)
 k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__restrict__ pts,
            const WIN *__restrict__ win, float *__restrict__ c0, float *__restrict__ c1, float *__restrict__ c2,
            int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out,
-           Rotation rot) {
+           Rotation rot, InstReduce *red, uint32_t n_inst) {
     // w32, w16 (each optional): the weights as int32 and as 16-bit unsigned
     // integers; the latter are used when every weight is an integer in
     // [0, 65535] (not_u16 stays 0)
@@ -184,7 +202,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
     for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
-            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+            if (cur != 0xffffffffu) flush_instance<D>(&red[(blockIdx.x % kCopies) * n_inst + cur], mn, mx, ws.su, ws.sf, s_d, s_u);
             cur = T.part;
             ipts = pts + insts[cur].src_pt;
             iwin = win + insts[cur].src;
@@ -242,8 +260,8 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
             __syncthreads();
         }
     }
-    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
-    ws.flush_flags(out);
+    if (cur != 0xffffffffu) flush_instance<D>(&red[(blockIdx.x % kCopies) * n_inst + cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+    ws.flush_flags(out, s_d);
 }
 
 // The prologue of a repartition that starts from a resident working copy
@@ -256,7 +274,8 @@ template <int D, typename WIN>
 __global__ void __launch_bounds__(kBlock)
 k_prologue_soa(const Tile *__restrict__ tiles, uint32_t n_tiles, const float *__restrict__ c0,
                const float *__restrict__ c1, const float *__restrict__ c2, const WIN *__restrict__ win,
-               int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out) {
+               int32_t *__restrict__ w32, uint16_t *__restrict__ w16, Instance *__restrict__ insts, Prologue *out,
+               InstReduce *red, uint32_t n_inst) {
     __shared__ double s_d[kBlock / 64 * 8];
     __shared__ uint64_t s_u[kBlock / 64];
     double mn[D], mx[D];
@@ -267,7 +286,7 @@ k_prologue_soa(const Tile *__restrict__ tiles, uint32_t n_tiles, const float *__
     for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const Tile T = tiles[t];
         if (T.part != cur) {
-            if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+            if (cur != 0xffffffffu) flush_instance<D>(&red[(blockIdx.x % kCopies) * n_inst + cur], mn, mx, ws.su, ws.sf, s_d, s_u);
             cur = T.part;
         }
         for (uint32_t p = threadIdx.x; p < T.count; p += kBlock) {
@@ -282,8 +301,8 @@ k_prologue_soa(const Tile *__restrict__ tiles, uint32_t n_tiles, const float *__
             ws.add(win[i], i, w32, w16);
         }
     }
-    if (cur != 0xffffffffu) flush_instance<D>(&insts[cur], mn, mx, ws.su, ws.sf, s_d, s_u);
-    ws.flush_flags(out);
+    if (cur != 0xffffffffu) flush_instance<D>(&red[(blockIdx.x % kCopies) * n_inst + cur], mn, mx, ws.su, ws.sf, s_d, s_u);
+    ws.flush_flags(out, s_d);
 }
 
 // Working copy of a resident mesh: points [first, first + count) as f32 SoA
@@ -1135,22 +1154,25 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
 
 // Empty per-instance reductions before the prologue.
 // Also empties the call's prologue flags (block 0).
-__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst, Prologue *pro) {
+__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst, Prologue *pro,
+                                                            InstReduce *red) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q == 0) {
+    if (q < kCopies) {   // the flags' copies
         Prologue z;
         memset(&z, 0, sizeof z);
-        *pro = z;
+        pro[q] = z;
     }
     if (q >= n_inst) return;
-    Instance &I = insts[q];
-    for (int k = 0; k < kMaxDim; k++) {
-        I.bmin[k] = ~0ull;
-        I.bmax[k] = 0;
+    for (int c = 0; c < kCopies; c++) {   // the reductions' copies start empty
+        InstReduce &R = red[(uint64_t)c * n_inst + q];
+        for (int k = 0; k < kMaxDim; k++) {
+            R.bmin[k] = ~0ull;
+            R.bmax[k] = 0;
+        }
+        R.sum_u64 = 0;
+        R.sum_f64 = 0.0;
     }
-    I.sum_u64 = 0;
-    I.sum_f64 = 0.0;
-    I.min_id = ~0ull;
+    insts[q].min_id = ~0ull;
 }
 
 // Root parts, one per instance (part index = instance index), and their
@@ -1197,7 +1219,7 @@ k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restri
 __global__ void __launch_bounds__(kBlock)
 k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts, int dim,
                  int real_mode, uint32_t *fwd_cnt, uint64_t *fwd_w, uint32_t *fwd_mn, uint32_t *n_parts0,
-                 uint32_t *n_tiles0, uint32_t tiles0) {
+                 uint32_t *n_tiles0, uint32_t tiles0, const InstReduce *__restrict__ red) {
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q == 0) {   // level 0's table sizes: one root per instance, the level-0 tiles
         *n_parts0 = n_inst;
@@ -1210,6 +1232,24 @@ k_roots_finalize(Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict
         fwd_mn[2 * q] = fwd_mn[2 * q + 1] = 0xffffffffu;
     }
     Instance &I = insts[q];
+    {   // the prologue's copies of the reductions, combined in copy order
+        InstReduce r = red[q];
+        for (int c = 1; c < kCopies; c++) {
+            const InstReduce &x = red[(uint64_t)c * n_inst + q];
+            for (int k = 0; k < kMaxDim; k++) {
+                r.bmin[k] = x.bmin[k] < r.bmin[k] ? x.bmin[k] : r.bmin[k];
+                r.bmax[k] = x.bmax[k] > r.bmax[k] ? x.bmax[k] : r.bmax[k];
+            }
+            r.sum_u64 += x.sum_u64;
+            r.sum_f64 += x.sum_f64;
+        }
+        for (int k = 0; k < kMaxDim; k++) {
+            I.bmin[k] = r.bmin[k];
+            I.bmax[k] = r.bmax[k];
+        }
+        I.sum_u64 = r.sum_u64;
+        I.sum_f64 = r.sum_f64;
+    }
     Part &R = parts[q];
     R.sum = real_mode ? __builtin_bit_cast(uint64_t, I.sum_f64) : I.sum_u64;
     for (int k = 0; k < kMaxDim; k++) {
