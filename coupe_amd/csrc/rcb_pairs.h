@@ -523,39 +523,75 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
     // lanes 0..3 serve grandchild slot `lane`: its scan array
     const int js = lane & 3;
     const uint64_t *pj = js == 0 ? p0 : (js == 1 ? p1 : (js == 2 ? p2 : p3));
-    // wavefront wid's grandchild slot in the second phase: the grandchild,
-    // its first candidate t3 and its children's t4, and the sums of both
-    // first passes over the tiles of the current parent
-    int32_t gw_ = -1;
-    float t3w = 0.f, t4w = 0.f;
-    uint32_t gcnt = 0, gmn = 0xffffffffu;
-    Acc gws = (Acc)0;
-    Acc fws0 = (Acc)0, fws1 = (Acc)0;
-    uint32_t fmn0 = 0xffffffffu, fmn1 = 0xffffffffu, fcn0 = 0, fcn1 = 0;
-    auto flush = [&]() {   // wavefront-local: no workgroup barrier
-        if (gw_ < 0) return;
-        const uint64_t b = 2 * (uint64_t)gw_;
+    // the current parent's grandchild slots (LDS, threads 0..3 write slot
+    // tid at a part change): the grandchild, its first candidate t3 and its
+    // children's t4, and the sums of both first passes over the parent's
+    // tiles, to which every wavefront adds those of the positions it stores;
+    // threads 0..3 flush them after a barrier at the next part change
+    __shared__ int32_t s_g[4];
+    __shared__ float s_t3[4], s_t4[4];
+    __shared__ uint32_t s_gcnt[4], s_gmn[4], s_fcn[2][4], s_fmn[2][4];
+    __shared__ Acc s_gws[4], s_fws[2][4];
+    auto reset_slot = [&](int j) {
+        s_gcnt[j] = 0;
+        s_gmn[j] = 0xffffffffu;
+        s_gws[j] = (Acc)0;
+        for (int h = 0; h < 2; h++) {
+            s_fcn[h][j] = 0;
+            s_fmn[h][j] = 0xffffffffu;
+            s_fws[h][j] = (Acc)0;
+        }
+    };
+    auto flush = [&]() {   // threads 0..3 (slot tid), after a workgroup barrier
+        if (tid >= 4) return;
+        const int32_t g = s_g[tid];
+        if (g >= 0) {
+            const uint64_t b = 2 * (uint64_t)g;
+            if (s_gcnt[tid]) { atomicAdd(&bcnt[b], s_gcnt[tid]); atomic_add_acc(&bw[b], s_gws[tid]); }
+            if (s_gmn[tid] != 0xffffffffu) atomicMin(&bmin[b + 1], s_gmn[tid]);
+            if constexpr (FWD) {
+                for (int h = 0; h < 2; h++) {
+                    if (s_fcn[h][tid]) { atomicAdd(&fw.cnt[b + h], s_fcn[h][tid]); atomic_add_acc(&fw.w[b + h], s_fws[h][tid]); }
+                    if (s_fmn[h][tid] != 0xffffffffu) atomicMin(&fw.mn[b + h], s_fmn[h][tid]);
+                }
+            }
+        }
+        reset_slot(tid);
+    };
+    if (tid < 4) {   // visible after the first tile's barriers
+        s_g[tid] = -1;
+        reset_slot(tid);
+    }
+    // this wavefront's sums for the slot of its last segment (registers):
+    // added to the slot's LDS sums when its next segment is of another slot,
+    // and before a part change
+    int cur_j = -1;   // wave-uniform
+    uint32_t gcnt = 0, fcn0 = 0, fcn1 = 0;
+    Acc gws = (Acc)0, fws0 = (Acc)0, fws1 = (Acc)0;
+    uint32_t gmn = 0xffffffffu, fmn0 = 0xffffffffu, fmn1 = 0xffffffffu;
+    auto wave_flush = [&]() {
+        if (cur_j < 0) return;
+        const int j = cur_j;
         const Acc gx = wave_sum(gws);
         const uint32_t gm = wave_min(gmn);
         if (lane == 0) {
-            if (gcnt) { atomicAdd(&bcnt[b], gcnt); atomic_add_acc(&bw[b], gx); }
-            if (gm != 0xffffffffu) atomicMin(&bmin[b + 1], gm);
+            if (gcnt) { atomicAdd(&s_gcnt[j], gcnt); atomic_add_acc(&s_gws[j], gx); }
+            if (gm != 0xffffffffu) atomicMin(&s_gmn[j], gm);
         }
-        gws = (Acc)0;
-        gmn = 0xffffffffu;
-        gcnt = 0;
-        if (!FWD) return;
-        const Acc x0 = wave_sum(fws0), x1 = wave_sum(fws1);
-        const uint32_t m0 = wave_min(fmn0), m1 = wave_min(fmn1);
-        if (lane == 0) {
-            if (fcn0) { atomicAdd(&fw.cnt[b], fcn0); atomic_add_acc(&fw.w[b], x0); }
-            if (fcn1) { atomicAdd(&fw.cnt[b + 1], fcn1); atomic_add_acc(&fw.w[b + 1], x1); }
-            if (m0 != 0xffffffffu) atomicMin(&fw.mn[b], m0);
-            if (m1 != 0xffffffffu) atomicMin(&fw.mn[b + 1], m1);
+        if constexpr (FWD) {
+            const Acc x0 = wave_sum(fws0), x1 = wave_sum(fws1);
+            const uint32_t m0 = wave_min(fmn0), m1 = wave_min(fmn1);
+            if (lane == 0) {
+                if (fcn0) { atomicAdd(&s_fcn[0][j], fcn0); atomic_add_acc(&s_fws[0][j], x0); }
+                if (fcn1) { atomicAdd(&s_fcn[1][j], fcn1); atomic_add_acc(&s_fws[1][j], x1); }
+                if (m0 != 0xffffffffu) atomicMin(&s_fmn[0][j], m0);
+                if (m1 != 0xffffffffu) atomicMin(&s_fmn[1][j], m1);
+            }
         }
-        fws0 = fws1 = (Acc)0;
-        fmn0 = fmn1 = 0xffffffffu;
-        fcn0 = fcn1 = 0;
+        gcnt = fcn0 = fcn1 = 0;
+        gws = fws0 = fws1 = (Acc)0;
+        gmn = fmn0 = fmn1 = 0xffffffffu;
+        cur_j = -1;
     };
 
     uint32_t t_first, t_last;
@@ -569,6 +605,8 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             continue;
         }
         if (T.part != cur) {
+            wave_flush();
+            __syncthreads();   // every wavefront's stores and sums of the previous part are done
             if (cur != 0xffffffffu) flush();
             cur = T.part;
             ax = (int)P.axis;
@@ -593,12 +631,12 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                 s_gbeg[tid] = gj >= 0 ? grand[gj].begin : 0;
                 s_gtb[tid] = gj >= 0 ? grand[gj].tile_begin : 0;
                 s_gpj0[tid] = gj >= 0 ? pj[P.tile_begin] : 0;
-            }
-            gw_ = wid == 0 ? g[0] : (wid == 1 ? g[1] : (wid == 2 ? g[2] : g[3]));
-            if (gw_ >= 0) {
-                const Part &G = grand[gw_];
-                t3w = (G.blo + G.bhi) / 2.0f;   // the grandchild's first candidate (:472)
-                t4w = (sel3(a4, G.lo[0], G.lo[1], G.lo[2]) + sel3(a4, G.hi[0], G.hi[1], G.hi[2])) / 2.0f;
+                s_g[tid] = gj;
+                if (gj >= 0) {
+                    const Part &G = grand[gj];
+                    s_t3[tid] = (G.blo + G.bhi) / 2.0f;   // the grandchild's first candidate (:472)
+                    s_t4[tid] = (sel3(a4, G.lo[0], G.lo[1], G.lo[2]) + sel3(a4, G.hi[0], G.hi[1], G.hi[2])) / 2.0f;
+                }
             }
         }
         // this tile's scan values: loaded by every lane without a branch, so
@@ -660,13 +698,13 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             // keeps the four slot masks of every item in SGPRs until then
             asm volatile("" : "+v"(pq[k]));
         }
-        if (tid < 4) {
+        __syncthreads();
+        if (tid < 4) {   // after the barrier: another wavefront may still store the last tile's runs
             const uint64_t off = pv - s_gpj0[tid];   // this tile's first position in the grandchild
             s_dst[tid] = s_gbeg[tid] + off;
             s_off[tid] = (uint32_t)(off % kTile);
             s_tile[tid] = s_gtb[tid] + (uint32_t)(off / kTile);
         }
-        __syncthreads();
         if (tid < 64) {   // exclusive scans of the counts of each grandchild slot, item-major
             constexpr int kE = kItems * (kBlock / 64);
             static_assert(kE <= 64, "one wavefront scans the counts");
@@ -688,17 +726,13 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             const uint32_t q = (pq[k] >> 8) < 4 ? (pq[k] >> 8) : 0u;
             cp[k] = wtot[q][k * (kBlock / 64) + wid] + (pq[k] & 0xffu);   // includes the run start of q
         }
-        // this wavefront's run, read before the staging barrier: no barrier
-        // ends the tile, so wavefront 0 may write the next tile's run
-        // starts and destinations while another still stores this tile's
-        // (the staged arrays, the counts and the scan are rewritten only after
-        // the next tile's first barrier, which every wavefront reaches after
-        // its store phase)
-        const uint32_t rb = s_run[wid], re = s_run[wid + 1];
-        const uint64_t dv = s_dst[wid];
-        const uint32_t run_off = s_off[wid], run_tile = s_tile[wid];
-        // every array staged in slot order, then wavefront j stores run j
-        // (3D: a3 = rz, a4 = the parent's axis rx, a2 = ry; 2D: a3 = rx, a4 = ry)
+        // every array staged in slot order (3D: a3 = rz, a4 = the parent's
+        // axis rx, a2 = ry; 2D: a3 = rx, a4 = ry); then each wavefront stores
+        // a quarter of the staged tile, run segment by run segment: the runs'
+        // lengths follow the data (a tile whose points all go to one
+        // grandchild is one run), the quarters do not.  The run starts and
+        // destinations stay in LDS until the next tile's first barrier, which
+        // every wavefront reaches after its stores.
 #pragma unroll
         for (int k = 0; k < kItems; k++)
             if (k * kBlock + tid < (int)cnt) {
@@ -717,105 +751,126 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
             float *oa2 = a2 == 0 ? o0 : (a2 == 1 ? o1 : o2);
             float *oa3 = a3 == 0 ? o0 : (a3 == 1 ? o1 : o2);
             float *oa4 = a4 == 0 ? o0 : (a4 == 1 ? o1 : o2);
-            // wave-uniform: into SGPRs once, for the stores' bases
-            const uint64_t db = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(dv >> 32)) << 32) |
-                                __builtin_amdgcn_readfirstlane((uint32_t)dv);
-            uint32_t len = re - rb;
-            len = db + len <= n_total ? len : (db < n_total ? (uint32_t)(n_total - db) : 0u);   // never past the arrays
-            const uint32_t split_i = kTile - run_off;   // positions from here on are in the next tile
-            uint32_t c_left = 0, c_b0 = 0, c_b1 = 0;
-            // FWD: the same counts over the positions in the first destination
-            // tile (positions below split_i), taken where the run crosses it
-            uint32_t f_left = 0, f_b0 = 0, f_b1 = 0;
-            bool crossed = false;
-            // the run's destination as uniform bases with 32-bit offsets (a
-            // run lies in one tile: i < kTile + 128, so the mask below changes
-            // nothing and shows the compiler the offsets' range)
-            float *pa2 = oa2 + db, *pa3 = oa3 + db, *pa4 = oa4 + db;
             const bool keep2 = (uint32_t)a2 != skip_axis;   // wave-uniform
-            WS *pw = ow + db;
-            // two positions per lane per iteration, every lane runs every
-            // iteration: validity is in uniform lane masks, and the ballots
-            // take the comparisons alone
-            for (uint32_t base = 0; base < len; base += 128) {
-                float xa[2], xb[2], xc[2];
-                WS wq[2];
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
-                    const uint32_t c = rb + (i < len ? i : 0u);
-                    xa[u] = sfa[c];
-                    xb[u] = sfb[c];
-                    if constexpr (D == 3) xc[u] = sfc[c];
-                    wq[u] = sws[c];
+            const uint32_t total = s_run[4];
+            const uint32_t quarter = ((total + 255u) / 256u) * 64u;   // whole 64-position chunks
+            const uint32_t p0 = min((uint32_t)wid * quarter, total), p1 = min(p0 + quarter, total);
+            for (int j = 0; j < 4; j++) {   // wave-uniform: the runs this quarter meets
+                const uint32_t rbj = s_run[j];
+                const uint32_t sa = max(p0, rbj), sb = min(p1, s_run[j + 1]);
+                if (sa >= sb) continue;
+                const uint32_t roff = sa - rbj;   // the segment's first position in run j
+                const uint64_t dv = s_dst[j] + roff;
+                // wave-uniform: into SGPRs once, for the stores' bases
+                const uint64_t db = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(dv >> 32)) << 32) |
+                                    __builtin_amdgcn_readfirstlane((uint32_t)dv);
+                uint32_t len = sb - sa;
+                len = db + len <= n_total ? len : (db < n_total ? (uint32_t)(n_total - db) : 0u);   // never past the arrays
+                // positions of the segment from si on are in the destination
+                // tile after the one of run j's first position
+                const int32_t si = (int32_t)(kTile - s_off[j]) - (int32_t)roff;
+                const uint32_t run_tile = s_tile[j];
+                const float t3w = s_t3[j], t4w = s_t4[j];
+                if (j != cur_j) {   // the sums go on in registers while the slot stays
+                    wave_flush();
+                    cur_j = j;
                 }
+                uint32_t c_left = 0, c_b0 = 0, c_b1 = 0;
+                // FWD: the same counts over the positions in the first destination
+                // tile (positions below si), taken where the segment crosses it
+                uint32_t f_left = 0, f_b0 = 0, f_b1 = 0;
+                bool crossed = si <= 0;
+                // the segment's destination as uniform bases with 32-bit offsets
+                // (a segment lies in one run: i < kTile + 128, so the mask below
+                // changes nothing and shows the compiler the offsets' range)
+                float *pa2 = oa2 + db, *pa3 = oa3 + db, *pa4 = oa4 + db;
+                WS *pw = ow + db;
+                // two positions per lane per iteration, every lane runs every
+                // iteration: validity is in uniform lane masks, and the ballots
+                // take the comparisons alone
+                for (uint32_t base = 0; base < len; base += 128) {
+                    float xa[2], xb[2], xc[2];
+                    WS wq[2];
 #pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const uint32_t pos = base + (uint32_t)u * 64;   // wave-uniform
-                    const bool vl = pos + (uint32_t)lane < len;
-                    const uint64_t vm = __builtin_amdgcn_ballot_w64(vl);   // lanes holding a position of the run
-                    const uint32_t i = (pos + (uint32_t)lane) & (4 * kTile - 1);
-                    if (vl) {
-                        if constexpr (D == 3)
-                            if (keep2) __builtin_nontemporal_store(xc[u], &pa2[i]);
-                        __builtin_nontemporal_store(xa[u], &pa3[i]);
-                        __builtin_nontemporal_store(xb[u], &pa4[i]);
-                        __builtin_nontemporal_store(wq[u], &pw[i]);
+                    for (int u = 0; u < 2; u++) {
+                        const uint32_t i = base + (uint32_t)u * 64 + (uint32_t)lane;
+                        const uint32_t c = sa + (i < len ? i : 0u);
+                        xa[u] = sfa[c];
+                        xb[u] = sfb[c];
+                        if constexpr (D == 3) xc[u] = sfc[c];
+                        wq[u] = sws[c];
                     }
-                    const bool lt3 = !(xa[u] >= t3w);
-                    const uint64_t bb = __builtin_amdgcn_ballot_w64(lt3) & vm;
-                    const bool bel3 = vl && lt3;
-                    const Acc wa = AccOps<Acc>::of(wq[u]);
-                    // the grandchild's first pass
-                    c_left += __popcll(bb);
-                    gws += bel3 ? wa : (Acc)0;
-                    const uint32_t ka = (vl && !lt3) ? f32_key(xa[u]) : 0xffffffffu;
-                    gmn = ka < gmn ? ka : gmn;
-                    if constexpr (FWD) {
-                        const bool lt4 = !(xb[u] >= t4w);
-                        const uint64_t b4 = __builtin_amdgcn_ballot_w64(lt4) & vm;
-                        if (!crossed && split_i < pos + 64) {
-                            // the chunk that holds the destination tile's end (once
-                            // per run, uniform): the counts of the positions before it
-                            const uint64_t lo = (1ull << (split_i - pos)) - 1ull;   // split_i - pos in [0, 63]
-                            f_left = c_left - __popcll(bb) + __popcll(bb & lo);
-                            f_b0 = c_b0 + __popcll(bb & b4 & lo);
-                            f_b1 = c_b1 + __popcll(~bb & b4 & lo);
-                            crossed = true;
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        const uint32_t pos = base + (uint32_t)u * 64;   // wave-uniform
+                        const bool vl = pos + (uint32_t)lane < len;
+                        const uint64_t vm = __builtin_amdgcn_ballot_w64(vl);   // lanes holding a position of the segment
+                        const uint32_t i = (pos + (uint32_t)lane) & (4 * kTile - 1);
+                        if (vl) {
+                            if constexpr (D == 3)
+                                if (keep2) __builtin_nontemporal_store(xc[u], &pa2[i]);
+                            __builtin_nontemporal_store(xa[u], &pa3[i]);
+                            __builtin_nontemporal_store(xb[u], &pa4[i]);
+                            __builtin_nontemporal_store(wq[u], &pw[i]);
                         }
-                        c_b0 += __popcll(bb & b4);
-                        c_b1 += __popcll(~bb & b4);
-                        const uint32_t key = f32_key(xb[u]);
-                        const bool below4 = vl && lt4;
-                        fws0 += (below4 && lt3) ? wa : (Acc)0;
-                        fws1 += (below4 && !lt3) ? wa : (Acc)0;
-                        const bool above4 = vl && !lt4;
-                        const uint32_t k0 = (above4 && lt3) ? key : 0xffffffffu;
-                        const uint32_t k1 = (above4 && !lt3) ? key : 0xffffffffu;
-                        fmn0 = k0 < fmn0 ? k0 : fmn0;
-                        fmn1 = k1 < fmn1 ? k1 : fmn1;
+                        const bool lt3 = !(xa[u] >= t3w);
+                        const uint64_t bb = __builtin_amdgcn_ballot_w64(lt3) & vm;
+                        const bool bel3 = vl && lt3;
+                        const Acc wa = AccOps<Acc>::of(wq[u]);
+                        // the grandchild's first pass
+                        c_left += __popcll(bb);
+                        gws += bel3 ? wa : (Acc)0;
+                        const uint32_t ka = (vl && !lt3) ? f32_key(xa[u]) : 0xffffffffu;
+                        gmn = ka < gmn ? ka : gmn;
+                        if constexpr (FWD) {
+                            const bool lt4 = !(xb[u] >= t4w);
+                            const uint64_t b4 = __builtin_amdgcn_ballot_w64(lt4) & vm;
+                            if (!crossed && (uint32_t)si < pos + 64) {
+                                // the chunk that holds the destination tile's end (once
+                                // per segment, uniform): the counts of the positions before it
+                                const uint64_t lo = (1ull << ((uint32_t)si - pos)) - 1ull;   // si - pos in [0, 63]
+                                f_left = c_left - __popcll(bb) + __popcll(bb & lo);
+                                f_b0 = c_b0 + __popcll(bb & b4 & lo);
+                                f_b1 = c_b1 + __popcll(~bb & b4 & lo);
+                                crossed = true;
+                            }
+                            c_b0 += __popcll(bb & b4);
+                            c_b1 += __popcll(~bb & b4);
+                            const uint32_t key = f32_key(xb[u]);
+                            const bool below4 = vl && lt4;
+                            fws0 += (below4 && lt3) ? wa : (Acc)0;
+                            fws1 += (below4 && !lt3) ? wa : (Acc)0;
+                            const bool above4 = vl && !lt4;
+                            const uint32_t k0 = (above4 && lt3) ? key : 0xffffffffu;
+                            const uint32_t k1 = (above4 && !lt3) ? key : 0xffffffffu;
+                            fmn0 = k0 < fmn0 ? k0 : fmn0;
+                            fmn1 = k1 < fmn1 ? k1 : fmn1;
+                        }
                     }
                 }
-            }
-            gcnt += c_left;
-            if (FWD && len > 0) {
-                fcn0 += c_b0;
-                fcn1 += c_b1;
-                if (!crossed) { f_left = c_left; f_b0 = c_b0; f_b1 = c_b1; }
-                const uint32_t h_left = c_left - f_left, h_b0 = c_b0 - f_b0, h_b1 = c_b1 - f_b1;
-                if (lane == 0) {
-                    const uint64_t dt = run_tile;
-                    const uint32_t l0 = f_left, b0 = f_b0, b1 = f_b1;
-                    if (l0) atomicAdd((unsigned long long *)&fw.left[dt], (unsigned long long)l0);
-                    if (b0) atomicAdd(&fw.below2[2 * dt], b0);
-                    if (b1) atomicAdd(&fw.below2[2 * dt + 1], b1);
-                    if (h_left) atomicAdd((unsigned long long *)&fw.left[dt + 1], (unsigned long long)h_left);
-                    if (h_b0) atomicAdd(&fw.below2[2 * dt + 2], h_b0);
-                    if (h_b1) atomicAdd(&fw.below2[2 * dt + 3], h_b1);
+                gcnt += c_left;
+                if constexpr (FWD) {
+                    fcn0 += c_b0;
+                    fcn1 += c_b1;
+                    if (!crossed) { f_left = c_left; f_b0 = c_b0; f_b1 = c_b1; }
+                    const uint32_t h_left = c_left - f_left, h_b0 = c_b0 - f_b0, h_b1 = c_b1 - f_b1;
+                    if (lane == 0) {
+                        if (len > 0) {   // per destination tile
+                            const uint64_t dt = run_tile;
+                            if (f_left) atomicAdd((unsigned long long *)&fw.left[dt], (unsigned long long)f_left);
+                            if (f_b0) atomicAdd(&fw.below2[2 * dt], f_b0);
+                            if (f_b1) atomicAdd(&fw.below2[2 * dt + 1], f_b1);
+                            if (h_left) atomicAdd((unsigned long long *)&fw.left[dt + 1], (unsigned long long)h_left);
+                            if (h_b0) atomicAdd(&fw.below2[2 * dt + 2], h_b0);
+                            if (h_b1) atomicAdd(&fw.below2[2 * dt + 3], h_b1);
+                        }
+                    }
                 }
             }
         }
     }
+    wave_flush();
+    __syncthreads();   // every wavefront's stores and sums of the last part are done
     if (cur != 0xffffffffu) flush();
 }
 
