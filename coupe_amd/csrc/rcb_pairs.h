@@ -483,12 +483,14 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 //
 // Per tile: the points are classified (three ballots give each point's slot
 // and rank), one wavefront scans the slot counts, every array is staged in
-// LDS in slot order, and wavefront j stores run j (slot j's points, one
-// contiguous range of its grandchild).  That phase also sums, per wavefront and across
-// the tiles of one parent, the grandchild's first pass (count, weight and
-// smallest key at its first candidate t3) and, with FWD, the forwarding of its children's first pass (per side of t3: count, weight and
-// smallest key at their first candidate t4; per destination tile: the count
-// below t3 and per side the count below t4).
+// LDS in slot order (run j: slot j's points, one contiguous range of its
+// grandchild), and each wavefront stores a quarter of the staged tile, one
+// run segment after the other.  That phase also sums, across the tiles of
+// one parent, each grandchild's first pass (count, weight and smallest key at
+// its first candidate t3) and, with FWD, the forwarding of its children's
+// first pass (per side of t3: count, weight and smallest key at their first
+// candidate t4; per destination tile: the count below t3 and per side the
+// count below t4).
 // skip_axis (0..2, or 3 for none): a coordinate array that no later level
 // reads (3D, the last move: the children's axis is not split again), read for
 // the classification but not written.
