@@ -237,6 +237,16 @@ void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev
 }
 
 // Bins per part of the standalone LDS passes, bounded by the bin buffers.
+// Search passes to queue at a level whose search took `decisions` decisions
+// (its first one and one per pass) in the context's last call of the same
+// shape: none when the first decision ended every search there (the split
+// at the first candidate, the usual case on even data), else the passes it
+// took plus one to spare, at most spec.
+static int queued_passes(int spec, uint32_t decisions) {
+    if (decisions <= 1) return 0;
+    return std::min(spec, (int)decisions);
+}
+
 static int choose_bins(uint64_t parts_ub, int max_bins) {
     const uint64_t cap = 1ull << 23;   // entries of each bin array
     int nb = max_bins;
@@ -357,14 +367,15 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     if (tiles_cap > 0xffffffffull || parts_cap > 0x7fffffffull) throw HipFailure{hipErrorOutOfMemory};
     if ((uint64_t)L * (uint64_t)(spec_passes + 1) > (uint64_t)kSpecSlots) spec_passes = 0;
     // passes queued per level: spec_passes, or, when the context's last call
-    // had this shape, as many as that level's search needed then (at least 1;
-    // a search that needs more is closed by its last queued decision and the
-    // batch runs again with host checks)
+    // had this shape, queued_passes() of what that level's search needed
+    // then (a search that needs more is closed by its last queued decision,
+    // the first one when none is queued, and the batch runs again with host
+    // checks)
     const uint64_t shape = B.shape;
     const Hints *seen = cx.hints_for(shape);
     std::vector<int> depth(L, spec_passes);
     if (spec_passes > 0 && seen && seen->level_passes.size() == L)
-        for (size_t l = 0; l < L; l++) depth[l] = std::max(1, std::min(spec_passes, (int)seen->level_passes[l]));
+        for (size_t l = 0; l < L; l++) depth[l] = queued_passes(spec_passes, seen->level_passes[l]);
     std::vector<uint8_t> used(L, 0);   // decisions per level in this call
     std::vector<uint32_t> open1(L, 0);  // parts still searching after each level's first decision
     const std::vector<uint32_t> last_open = seen && seen->level_open.size() == L ? seen->level_open
@@ -526,9 +537,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
             const int queued = depth[level];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
+                // with no pass queued this decision closes the searches it leaves open
                 k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
                                                                &slot[0], swept_first, counts_first, nullptr,
-                                                               nullptr, xc);
+                                                               queued == 0 ? &ctr->unfinished : nullptr, xc);
             });
             for (int p = 1; p <= queued; p++) {
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
@@ -1148,7 +1160,7 @@ static uint64_t graph_key(const Context &cx, const BatchPlan &B, size_t dim, int
     mix((uint64_t)(int64_t)(e ? e->weight_hint : cx.weight_hint));
     const size_t L = B.max_levels;
     for (size_t l = 0; l < L; l++) {
-        const int d = e && e->level_passes.size() == L ? std::max(1, std::min(spec, (int)e->level_passes[l])) : spec;
+        const int d = e && e->level_passes.size() == L ? queued_passes(spec, e->level_passes[l]) : spec;
         const bool ff = e && e->level_open.size() == L &&
                         2 * (uint64_t)e->level_open[l] > std::max<uint64_t>(B.parts_ub[l], 1);
         mix((uint64_t)d | (uint64_t)ff << 16);
