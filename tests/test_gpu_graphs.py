@@ -202,3 +202,35 @@ def test_switches_from_the_environment(oracle_mod, monkeypatch, bins, wg, chunk)
             assert np.array_equal(out.cpu().numpy().astype(np.uint64), _ids(oracle_mod, pts, w, it, tol))
     finally:
         ctx.close()
+
+
+def test_level_with_no_queued_pass_reruns_when_new_data_needs_one(oracle_mod):
+    """A level whose search ended at its first decision in the context's last
+    call of the same shape queues no search pass: data of that shape which
+    needs passes there is closed by that first decision, and the call runs
+    again with host checks.  Grid cell centres (every split at its box
+    midpoint) and clustered points alternate on one context; every call has
+    the oracle's ids."""
+    torch = pytest.importorskip("torch")
+    m = 32
+    g = (np.arange(m) + 0.5) / m
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    grid = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    n = grid.shape[0]
+    rng = np.random.default_rng(91)
+    clustered = rng.normal(0, 1, (n, 3)) ** 5
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        for q, pts in enumerate([grid, clustered, grid, clustered, grid]):
+            w = np.ones(n, dtype=np.int64) if q % 2 == 0 else rng.integers(1, 50, n).astype(np.int64)
+            out = torch.full((n,), -1, dtype=torch.int64, device="cuda")
+            coupe_amd.Rcb(iter_count=9, tolerance=0.01).partition(
+                out, (torch.from_numpy(pts).cuda(), torch.from_numpy(w).cuda()), context=ctx)
+            torch.cuda.synchronize()
+            st = ctx.stats()
+            say(f"alternating data, call {q}: reruns {st['reruns']}, passes {st['passes']}")
+            assert np.array_equal(out.cpu().numpy().astype(np.uint64), _ids(oracle_mod, pts, w, 9, 0.01)), f"call {q}"
+            if q == 1:
+                assert st["reruns"] == 1, st   # no pass was queued where the grid needed none
+    finally:
+        ctx.close()
