@@ -503,10 +503,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
     const int real_mode = std::is_same<Acc, double>::value;
     const double payload = 4.0 * D + sizeof(WS);   // bytes of one point in the working arrays
-    static_assert(sizeof(Counters) % 4 == 0, "counters are cleared as 32-bit words");
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        k_fill_u32<<<grid_for(sizeof(Counters) / 4, kBlock), kBlock, 0, st>>>((uint32_t *)ctr, sizeof(Counters) / 4,
-                                                                               0u);
+        // (the counters were cleared by k_init_roots, before the prologue)
         k_roots_finalize<<<grid_for(n_inst, kBlock), kBlock, 0, st>>>(
             insts, n_inst, (Part *)cx.parts[0].ptr, D, real_mode, root_fwd ? (uint32_t *)cx.fwd_cnt.ptr : nullptr,
             (uint64_t *)cx.fwd_w.ptr, (uint32_t *)cx.fwd_mn.ptr, &ctr->n_parts[0], &ctr->n_tiles[0],
@@ -649,13 +647,14 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                                const Forward<Acc> &fclr) {
             Part *pf = (Part *)cx.parts[from].ptr;
             const uint64_t pub = std::max<uint64_t>(B.parts_ub[level], 1);
-            if (pub <= kPlanParts) {
+            const bool self_plan = pub <= kSelfPlanParts;   // the write kernel scans the offsets itself
+            if (!self_plan && pub <= kPlanParts) {
                 timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                     k_children_plan<<<1, kPlanThreads, 0, st>>>(pf, &ctr->n_parts[from], (uint64_t *)cx.nchild.ptr,
                                                                (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
                                                                &ctr->tiles_total, &ctr->n_parts[to], &ctr->n_tiles[to]);
                 });
-            } else {
+            } else if (!self_plan) {
                 timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                     k_children_count<<<grid_for(pub, kBlock), kBlock, 0, st>>>(
                         pf, &ctr->n_parts[from], (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
@@ -665,9 +664,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_children_write<Acc><<<(unsigned)pub, kBlock, 0, st>>>(
-                    pf, &ctr->n_parts[from], D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                    (Part *)cx.parts[to].ptr, next_tiles, nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr,
-                    &ctr->forwarded);
+                    pf, &ctr->n_parts[from], D, self_plan ? nullptr : (const uint64_t *)cx.nchild.ptr,
+                    self_plan ? nullptr : (const uint64_t *)cx.ntiles.ptr, (Part *)cx.parts[to].ptr, next_tiles,
+                    nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr, &ctr->forwarded,
+                    &ctr->n_parts[to], &ctr->n_tiles[to]);
                 if (pub > kPlanParts)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[to],
                                                   &ctr->n_tiles[to]);
@@ -791,13 +791,14 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
             // Children (next level's parts and tiles), split records, id offsets.
             const bool plan1 = parts_ub <= kPlanParts;
-            if (plan1) {
+            const bool self_plan = parts_ub <= kSelfPlanParts;   // the write kernel scans the offsets itself
+            if (!self_plan && plan1) {
                 timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                     k_children_plan<<<1, kPlanThreads, 0, st>>>(parts, np, (uint64_t *)cx.nchild.ptr,
                                                                (uint64_t *)cx.ntiles.ptr, &ctr->parts_total,
                                                                &ctr->tiles_total, &ctr->n_parts[nxt], &ctr->n_tiles[nxt]);
                 });
-            } else {
+            } else if (!self_plan) {
                 timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                     k_children_count<<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
                         parts, np, (uint64_t *)cx.nchild.ptr, (uint64_t *)cx.ntiles.ptr);
@@ -807,10 +808,11 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
             }
             timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
                 k_children_write<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(
-                    parts, np, D, (const uint64_t *)cx.nchild.ptr, (const uint64_t *)cx.ntiles.ptr,
-                    (Part *)cx.parts[nxt].ptr, (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level],
-                    node_base[std::min(level + 1, L)], heap_split, insts, Forward<Acc>{}, bcnt, bw, bmin,
-                    Forward<Acc>{}, &ctr->forwarded);
+                    parts, np, D, self_plan ? nullptr : (const uint64_t *)cx.nchild.ptr,
+                    self_plan ? nullptr : (const uint64_t *)cx.ntiles.ptr, (Part *)cx.parts[nxt].ptr,
+                    (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level], node_base[std::min(level + 1, L)], heap_split,
+                    insts, Forward<Acc>{}, bcnt, bw, bmin, Forward<Acc>{}, &ctr->forwarded, &ctr->n_parts[nxt],
+                    &ctr->n_tiles[nxt]);
                 if (!plan1)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
                                                   &ctr->n_tiles[nxt]);
@@ -942,15 +944,17 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
     if (!cx.capturing)
         check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
     timed(cx, st, COUPE_HIP_K_OTHER, 0.0, [&] {
-        // bounding boxes start empty (min key = max, max key = 0), flags clear
-        k_reset_instances<<<grid_for(std::max<uint32_t>(n_inst, kCopies), kBlock), kBlock, 0, st>>>(
-            (Instance *)cx.instances.ptr, n_inst, (Prologue *)cx.prologue.ptr, (InstReduce *)cx.ired.ptr);
-        // blocks (instance, share of its tiles): about 4 tiles per thread
+        // blocks (instance, share of its tiles): about 4 tiles per thread;
+        // the same launch clears the call's counters and the prologue's
+        // reductions (bounding boxes start empty, flags clear)
         uint64_t max_nt = 1;
         for (const Instance &I : B.inst) max_nt = std::max<uint64_t>(max_nt, (I.count + kTile - 1) / kTile);
         const unsigned gy = (unsigned)std::min<uint64_t>((max_nt + 4 * kBlock - 1) / (4 * kBlock), 1024);
-        k_init_roots<<<dim3(n_inst, gy), kBlock, 0, st>>>((const Instance *)cx.instances.ptr, n_inst,
-                                                           (Part *)cx.parts[0].ptr, (Tile *)cx.tiles0.ptr);
+        static_assert(sizeof(Counters) % 4 == 0, "counters are cleared as 32-bit words");
+        k_init_roots<<<dim3(n_inst, gy), kBlock, 0, st>>>(
+            (const Instance *)cx.instances.ptr, n_inst, (Part *)cx.parts[0].ptr, (Tile *)cx.tiles0.ptr,
+            (uint32_t *)cx.counters.ptr, (uint32_t)(sizeof(Counters) / 4), (Prologue *)cx.prologue.ptr,
+            (InstReduce *)cx.ired.ptr);
     });
     const uint32_t nt = (uint32_t)B.tiles0;
     if (nt > 0) {

@@ -982,6 +982,7 @@ k_children_count(const Part *__restrict__ parts, const uint32_t *n_parts, uint64
 constexpr int kPlanThreads = 1024;
 constexpr int kPlanItems = 8;
 constexpr uint32_t kPlanParts = kPlanThreads * kPlanItems;
+constexpr uint32_t kSelfPlanParts = 64;   // up to this many parts k_children_write scans by itself
 
 __global__ void __launch_bounds__(kPlanThreads)
 k_children_plan(const Part *__restrict__ parts, const uint32_t *n_parts, uint64_t *__restrict__ child_off,
@@ -1041,11 +1042,35 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
                  Part *__restrict__ next, Tile *__restrict__ next_tiles, Node *__restrict__ nodes,
                  uint64_t node_base, uint64_t next_node_base, float *__restrict__ heap_split,
                  Instance *__restrict__ insts, Forward<Acc> fsrc, uint32_t *__restrict__ bcnt,
-                 Acc *__restrict__ bw, uint32_t *__restrict__ bmin, Forward<Acc> fclr, uint32_t *n_forwarded) {
+                 Acc *__restrict__ bw, uint32_t *__restrict__ bmin, Forward<Acc> fclr, uint32_t *n_forwarded,
+                 uint32_t *next_n_parts, uint32_t *next_n_tiles) {
     // fsrc (non-null): a forwarded part's children get their first-pass bins
     // from it.  fclr (non-null): the new parts' forwarding sums and their
     // tiles' counts are cleared for the move that will fill them.
+    // child_off == nullptr (at most 64 parts): every workgroup scans the
+    // parts' children and tiles itself, and workgroup 0 sets the next level's
+    // table sizes (what k_children_plan does for up to kPlanParts parts).
     const uint32_t s = blockIdx.x;
+    __shared__ uint64_t s_off[2];
+    if (!child_off && threadIdx.x < 64) {
+        const uint32_t n = *n_parts;
+        const uint32_t r = threadIdx.x;
+        uint64_t c = 0, t = 0;
+        if (r < n) {
+            const Part &Q = parts[r];
+            const bool in = Q.status != kDone && Q.depth + 1 < Q.levels;
+            const uint64_t nl = in ? Q.cnt_left : 0, nr = in ? Q.count - Q.cnt_left : 0;
+            c = (nl > 0) + (nr > 0);
+            t = tiles_for(nl) + tiles_for(nr);
+        }
+        const uint64_t ic = wave_incl_add(c), it = wave_incl_add(t);
+        if (r == s) { s_off[0] = ic - c; s_off[1] = it - t; }
+        if (s == 0 && r == 63) {
+            *next_n_parts = (uint32_t)ic;
+            *next_n_tiles = (uint32_t)it;
+        }
+    }
+    if (!child_off) __syncthreads();
     if (s >= *n_parts) return;
     const Part P = parts[s];
     if (P.status == kDone) return;
@@ -1058,8 +1083,8 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     const uint32_t na = (a + 1) % dim;
     const Acc S = AccOps<Acc>::from_bits(P.sum);
     const Acc wl = AccOps<Acc>::from_bits(P.wl);
-    uint64_t co = child_off[s];
-    uint64_t to = tile_off[s];
+    uint64_t co = child_off ? child_off[s] : s_off[0];
+    uint64_t to = child_off ? tile_off[s] : s_off[1];
     int32_t id0 = -1, id1 = -1;
 #pragma unroll
     for (int side = 0; side < 2; side++) {
@@ -1152,19 +1177,26 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     }
 }
 
-// Empty per-instance reductions before the prologue.
-// Also empties the call's prologue flags (block 0).
-__global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uint32_t n_inst, Prologue *pro,
-                                                            InstReduce *red) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q < kCopies) {   // the flags' copies
+
+// Root parts, one per instance (part index = instance index), and their
+// tiles.  The box and the sum are set once the prologue has reduced them.
+__global__ void __launch_bounds__(kBlock)
+k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts,
+             Tile *__restrict__ tiles, uint32_t *ctr_words, uint32_t n_words, Prologue *pro, InstReduce *red) {
+    // the call's counters, the prologue's flags and the copies of its
+    // reductions start empty (bounding boxes: min key = max, max key = 0)
+    const uint32_t gt = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+    for (uint32_t k = gt; k < n_words; k += gridDim.x * gridDim.y * blockDim.x) ctr_words[k] = 0u;
+    if (gt < kCopies) {
         Prologue z;
         memset(&z, 0, sizeof z);
-        pro[q] = z;
+        pro[gt] = z;
     }
+    // block (q, y): instance q, a share y of its tiles
+    const uint32_t q = blockIdx.x;
     if (q >= n_inst) return;
-    for (int c = 0; c < kCopies; c++) {   // the reductions' copies start empty
-        InstReduce &R = red[(uint64_t)c * n_inst + q];
+    if (blockIdx.y == 0 && threadIdx.x < kCopies) {
+        InstReduce &R = red[(uint64_t)threadIdx.x * n_inst + q];
         for (int k = 0; k < kMaxDim; k++) {
             R.bmin[k] = ~0ull;
             R.bmax[k] = 0;
@@ -1172,17 +1204,6 @@ __global__ void __launch_bounds__(kBlock) k_reset_instances(Instance *insts, uin
         R.sum_u64 = 0;
         R.sum_f64 = 0.0;
     }
-    insts[q].min_id = ~0ull;
-}
-
-// Root parts, one per instance (part index = instance index), and their
-// tiles.  The box and the sum are set once the prologue has reduced them.
-__global__ void __launch_bounds__(kBlock)
-k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts,
-             Tile *__restrict__ tiles) {
-    // block (q, y): instance q, a share y of its tiles
-    const uint32_t q = blockIdx.x;
-    if (q >= n_inst) return;
     const Instance I = insts[q];
     const uint32_t nt = tiles_for(I.count);
     if (threadIdx.x == 0 && blockIdx.y == 0) {
