@@ -183,7 +183,7 @@ struct Context {
     DevBuf rib_moments;              // Rib: block partials, centroid, inertia
     coupe_hip_rib_info rib{};        // Rib: rotation of the last call
     const Rotation *rot = nullptr;   // change of basis for the running call
-    DevBuf prologue, counters;
+    DevBuf counters;
     DevBuf ired;                     // per-XCD copies of the prologue's instance reductions
     DevBuf in_points, in_weights, out_ids;   // staging for the host-memory ABI
     static constexpr int kStageBufs = 3;

@@ -149,15 +149,18 @@ struct Counters {
     uint32_t n_parts[3];
     uint32_t n_tiles[3];
     uint32_t searching;
-    uint32_t error;
+    uint32_t pad;
     uint64_t parts_total;
     uint64_t tiles_total;
     uint64_t min_id;
+    // read back at the end of the call, from here
     unsigned long long swept;                   // points swept by standalone passes
     unsigned long long swept_fused;             // points binned inside partitions
     unsigned long long first_read;              // points read by the children's first passes
+    Prologue pro[kCopies];                      // the prologue's weight flags, one copy per XCD
     uint32_t forwarded;                         // parts whose children's first pass the move forwarded
     uint32_t unfinished;                        // a speculative search did not end in its passes
+    uint32_t error;                             // the node-record descent found an inconsistent tree
     uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
     uint32_t end;                               // end of the call's readback
 };
@@ -905,14 +908,19 @@ static int weight_mode_of(const Prologue &p, int wkind, bool weights16) {
     return p.not_i32 ? 2 : 1;
 }
 
-// The prologue's flags are kept in one copy per XCD; the host reads the
-// copies (into the context's readback area) and combines them.
-static Prologue *prologue_copies(Context &cx) {
+// The prologue's flags are kept in one copy per XCD, in the call's
+// counters; the host reads the copies with the call's last counters (or,
+// when it must wait for them, right after the prologue, into the early
+// area) and combines them.
+static Prologue *prologue_early(Context &cx) {
     static_assert(kHostScratch >= 8192 + kCopies * sizeof(Prologue), "prologue readback");
     return (Prologue *)((char *)cx.host_scratch + 8192);
 }
-static Prologue prologue_readback(Context &cx) {
-    const Prologue *c = prologue_copies(cx);
+static const Prologue *prologue_late(Context &cx) {   // in the counters' readback area
+    return (const Prologue *)((const char *)cx.host_scratch + 256 + offsetof(Counters, pro) -
+                              offsetof(Counters, swept));
+}
+static Prologue combine_flags(const Prologue *c) {
     Prologue p = c[0];
     for (int k = 1; k < kCopies; k++) {
         p.sum_abs += c[k].sum_abs;
@@ -929,7 +937,7 @@ static Prologue prologue_readback(Context &cx) {
 // mode holds any narrower one; f64 and the integer modes differ), and the
 // call must run again.
 static bool mode_held(Context &cx) {
-    const int best = weight_mode_of(prologue_readback(cx), cx.pending.wkind, cx.weights16);
+    const int best = weight_mode_of(combine_flags(prologue_late(cx)), cx.pending.wkind, cx.weights16);
     cx.weight_hint = best;
     cx.remember(cx.pending.shape).weight_hint = best;
     const int mode = cx.pending.mode;
@@ -940,6 +948,7 @@ template <int D, typename WIN>
 static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const double *pts, const WIN *win,
                      Prologue *h, bool wait, int hint) {
     const uint32_t n_inst = (uint32_t)B.inst.size();
+    Prologue *flags = (Prologue *)((char *)cx.counters.ptr + offsetof(Counters, pro));
     // the instance table (a captured call's is copied before each launch)
     if (!cx.capturing)
         check(hipMemcpyAsync(cx.instances.ptr, B.inst.data(), n_inst * sizeof(Instance), hipMemcpyHostToDevice, st));
@@ -953,8 +962,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
         static_assert(sizeof(Counters) % 4 == 0, "counters are cleared as 32-bit words");
         k_init_roots<<<dim3(n_inst, gy), kBlock, 0, st>>>(
             (const Instance *)cx.instances.ptr, n_inst, (Part *)cx.parts[0].ptr, (Tile *)cx.tiles0.ptr,
-            (uint32_t *)cx.counters.ptr, (uint32_t)(sizeof(Counters) / 4), (Prologue *)cx.prologue.ptr,
-            (InstReduce *)cx.ired.ptr);
+            (uint32_t *)cx.counters.ptr, (uint32_t)(sizeof(Counters) / 4), (InstReduce *)cx.ired.ptr);
     });
     const uint32_t nt = (uint32_t)B.tiles0;
     if (nt > 0) {
@@ -979,7 +987,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
                 kernel<<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, pts, win, (float *)cx.cx[0][0].ptr,
                                              (float *)cx.cx[0][1].ptr, (float *)(D > 2 ? cx.cx[0][2].ptr : nullptr),
                                              w32, w16, (Instance *)cx.instances.ptr,
-                                             (Prologue *)cx.prologue.ptr, rot, (InstReduce *)cx.ired.ptr, n_inst);
+                                             flags, rot, (InstReduce *)cx.ired.ptr, n_inst);
             });
         };
         if (cx.slot0[0]) {
@@ -990,7 +998,7 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
                 k_prologue_soa<D, WIN><<<g, kBlock, 0, st>>>((const Tile *)cx.tiles0.ptr, nt, cx.coords0(0),
                                                              cx.coords0(1), D > 2 ? cx.coords0(2) : nullptr, win,
                                                              w32, w16, (Instance *)cx.instances.ptr,
-                                                             (Prologue *)cx.prologue.ptr, (InstReduce *)cx.ired.ptr,
+                                                             flags, (InstReduce *)cx.ired.ptr,
                                                              n_inst);
             });
         } else if (cx.rot) {
@@ -999,9 +1007,11 @@ static void prologue(Context &cx, hipStream_t st, const BatchPlan &B, const doub
             launch(k_prologue<D, WIN, false>);
         }
     }
-    // without `wait`, h is pinned memory read once the call's last sync is done
-    check(hipMemcpyAsync(h, cx.prologue.ptr, kCopies * sizeof(Prologue), hipMemcpyDeviceToHost, st));
-    if (wait) check(hipStreamSynchronize(st));
+    // without `wait` the flags come back with the call's last counters
+    if (wait) {
+        check(hipMemcpyAsync(h, flags, kCopies * sizeof(Prologue), hipMemcpyDeviceToHost, st));
+        check(hipStreamSynchronize(st));
+    }
 }
 
 template <int D, typename WIN>
@@ -1012,7 +1022,6 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     for (int b = cx.slot0[0] ? 1 : 0; b < 3; b++)
         for (int k = 0; k < D; k++) cx.cx[b][k].ensure(std::max<uint64_t>(n, 1) * sizeof(float));
     cx.w[0].ensure(std::max<uint64_t>(n, 1) * sizeof(int32_t));
-    cx.prologue.ensure(kCopies * sizeof(Prologue));
     cx.ired.ensure(kCopies * std::max<size_t>(B.inst.size(), 1) * sizeof(InstReduce));
     cx.counters.ensure(sizeof(Counters));
     cx.instances.ensure(std::max<size_t>(B.inst.size(), 1) * sizeof(Instance));
@@ -1055,7 +1064,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
     // prologue's flags are then read with the call's last counters, without a
     // wait after the prologue, and a call whose data does not fit that mode
     // runs again in its own.  A new context waits for the flags first.
-    Prologue *hp = prologue_copies(cx);
+    Prologue *hp = prologue_early(cx);
     for (;;) {
         const Hints *seen = cx.hints_for(B.shape);
         const int hint = seen ? seen->weight_hint : cx.weight_hint;
@@ -1066,7 +1075,7 @@ static bool run_typed(Context &cx, hipStream_t st, uint64_t *d_out, const BatchP
         if (hint >= 0) cx.w[1].ensure(std::max<uint64_t>(n, 1) * weight_bytes(hint));
         prologue<D, WIN>(cx, st, B, pts, win, hp, hint < 0, hint);
         if (hint < 0) {
-            mode = best_mode(prologue_readback(cx));
+            mode = best_mode(combine_flags(hp));
             if (mode >= 2) cx.w[0].ensure(std::max<uint64_t>(n, 1) * 8u);
         }
         cx.pending.mode = mode;
@@ -1509,7 +1518,7 @@ void Context::for_each_buf(F &&f) {
                      &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn,
                      &tile_bins2, &tile4, &bcnt, &bw, &bmin, &xcnt, &xw, &xmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
                      &heap_split, &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins,
-                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &prologue, &ired, &counters, &in_points,
+                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &ired, &counters, &in_points,
                      &in_weights, &out_ids};
     for (DevBuf *d : all) f(*d);
 }

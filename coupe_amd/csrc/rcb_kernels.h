@@ -1182,16 +1182,11 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
 // tiles.  The box and the sum are set once the prologue has reduced them.
 __global__ void __launch_bounds__(kBlock)
 k_init_roots(const Instance *__restrict__ insts, uint32_t n_inst, Part *__restrict__ parts,
-             Tile *__restrict__ tiles, uint32_t *ctr_words, uint32_t n_words, Prologue *pro, InstReduce *red) {
-    // the call's counters, the prologue's flags and the copies of its
-    // reductions start empty (bounding boxes: min key = max, max key = 0)
+             Tile *__restrict__ tiles, uint32_t *ctr_words, uint32_t n_words, InstReduce *red) {
+    // the call's counters (the prologue's flags among them) and the copies
+    // of its reductions start empty (bounding boxes: min key = max, max key = 0)
     const uint32_t gt = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
     for (uint32_t k = gt; k < n_words; k += gridDim.x * gridDim.y * blockDim.x) ctr_words[k] = 0u;
-    if (gt < kCopies) {
-        Prologue z;
-        memset(&z, 0, sizeof z);
-        pro[gt] = z;
-    }
     // block (q, y): instance q, a share y of its tiles
     const uint32_t q = blockIdx.x;
     if (q >= n_inst) return;
