@@ -103,7 +103,6 @@ struct Rotation {
 // sum of the tile's bins in the histogram pass that decided (kCountsBins).
 constexpr uint32_t kCountsRead = 0, kCountsBelow = 1, kCountsBins = 2;
 
-// Results of the prologue reductions over the whole batch.
 // One copy of an instance's prologue reductions (ordered keys of the f64
 // box, the wrapping integer and the f64 weight sums); the prologue's
 // workgroups on XCD c reduce into copy c and the roots' set-up combines them.
@@ -114,6 +113,8 @@ struct InstReduce {
     double sum_f64;
 };
 
+// Flags and guards of the prologue over the whole batch, one copy per XCD;
+// the host combines the copies.
 struct Prologue {
     double sum_abs;             // sum of |w| (exactness guard)
     uint64_t max_abs_bits;      // bits of max |w| (non-negative doubles order as integers)
