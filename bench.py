@@ -491,12 +491,15 @@ def bench_grid(torch, coupe_amd, with_cpu=True, side=10000, iters=12, threads=16
         ev.append((e0, e1))
     torch.cuda.synchronize()
     ms = sorted(a.elapsed_time(b) for a, b in ev)[steps // 2]
-    # bytes: one read of the weights per level, one write of the ids
+    # bytes of the per-level formulation (one read of the weights per level,
+    # one write of the ids), and of the summed-area path that runs here (two
+    # reads of the weights, one write of the table, one of the ids)
     gbps = n * (8.0 * iters + 8.0) / (ms * 1e-3) / 1e9
     res = {"workload": f"{side}x{side} grid, weight of cell i = i (f64), iter_count {iters}, rayon pool size "
                        f"{threads} for the median's chunking (coupe/benches/rcb_cartesian.rs)",
            "cells": n, "median_ms": round(ms, 4), "Mcells_s": round(n / (ms * 1e-3) / 1e6, 1),
-           "algo_GBps": round(gbps, 1)}
+           "algo_GBps": round(gbps, 1),
+           "path_GBps": round(n * 32.0 / (ms * 1e-3) / 1e9, 1)}
     log(f"grid {side}^2: {ms:.3f} ms, {res['Mcells_s']} Mcells/s")
     del w, out
     if with_cpu:

@@ -4,7 +4,9 @@
 // The grid is split level by level.  At level l every non-empty subgrid
 // (a box of cells) is cut across axis (1 + l) mod D at the weighted median
 // of its axis sums (the sums of its weights over the cross-section at each
-// index of that axis):
+// index of that axis).  Two paths: the summed-area path further down (calls
+// of at most 20 levels whose weights are exactly summable) and the
+// per-level path, which reads the weights at every level:
 //   k_grid_counts + scans  chunks of 4096 cells and axis-sum slots per subgrid
 //   k_grid_axis            one pass over the cells of every subgrid: axis sums
 //                          in LDS, flushed with atomics (exact for integers)
@@ -320,6 +322,619 @@ k_grid_fill(const GridPart *__restrict__ parts, const uint32_t *n_parts, const u
     }
 }
 
+// ---------------------------------------------------------------------------
+// Summed-area path.  Every axis sum weighted_median asks for is a box sum of
+// the weights (the subgrid's cross-sections from its first index to the
+// chunk's start), and a box sum is 2^D lookups into the table of inclusive
+// prefix sums S(x, y, z) = sum of w over [0, x] x [0, y] x [0, z].  Integer
+// weights are summed in 64-bit two's complement (a ring: any order gives the
+// wrapping sums of coupe's i32 / i64 loops); integer-valued doubles whose
+// absolute sum is at most 2^53 have exact sums in any order, so their
+// table, kept in int64, gives coupe's f64 sums bit for bit.  Other doubles
+// take the per-level path above.  The table costs two reads of the weights
+// and one write of 8 bytes per cell (2D; 3D adds a scan along z), where the
+// per-level path reads the weights once per level; each level is then one
+// wavefront per subgrid doing the median's rounds on lookups:
+//   k_sat_sums        per tile of 16 rows x 2048 columns: column and row sums
+//   k_sat_colscan     exclusive scans over the tiles' rows / columns
+//   k_sat_linescan    inclusive scans along rows of the row carries
+//   k_sat_tiles       per tile: the table from the weights and the carries
+//   k_sat_zsum/zapply 3D: the scan along z, in bands of 16 planes
+//   k_grid_level      one wavefront per subgrid of a level: weighted_median
+//                     on lookups, the split into a dense heap, the children
+//   k_grid_fill_heap  each cell walks the heap to its part id (part_of)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSatRows = 16;                 // rows per tile
+constexpr uint32_t kSatPer = 8;                   // consecutive columns per thread
+constexpr uint32_t kSatCols = kBlock * kSatPer;   // columns per tile
+constexpr uint32_t kSatGroup = 4;                 // rows per barrier in k_sat_tiles
+constexpr uint32_t kSatZBand = 16;                // planes per band of the z scan
+constexpr uint32_t kDenseGridLevels = 20;         // deeper calls take the per-level path
+constexpr uint32_t kFillTop = 12;                 // heap levels the fill keeps in LDS
+constexpr uint32_t kFillCells = 16;               // cells per thread of the fill
+
+struct SatGeom {
+    uint64_t gx, gy, gz;
+    uint32_t nb, nseg;     // tiles per plane: nb bands of rows x nseg segments of columns
+    uint32_t vec_w;        // every row of the weights starts at a multiple of two weights' size
+    uint32_t vec_s;        // every row of the table starts 16-byte aligned
+};
+
+struct GridSatFlags {
+    double abs_sum;        // sum of |w| (f64 weights)
+    uint32_t bad;          // a weight that is not an integer below 2^53 in magnitude
+    uint32_t pad;
+};
+
+// One subgrid of a level of the summed-area path; any size 0: no cells.
+struct GSlot {
+    uint32_t off[3], size[3];
+    uint64_t sum;          // weight, in the carrier of GW<K>
+};
+
+// The table's accumulator: 64-bit two's complement for integer weights,
+// f64 for f64 weights (exact while every partial sum is an integer of
+// magnitude at most 2^53, which the host checks).
+template <int K> using SatAcc = typename std::conditional<K == 2, double, uint64_t>::type;
+
+template <int K, typename WIN>
+__device__ inline SatAcc<K> sat_of(WIN v) {
+    if constexpr (K == 2) return (double)v;
+    else return (uint64_t)(int64_t)v;
+}
+
+__device__ inline uint64_t readlane64(uint64_t v, int l) {
+    return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l) |
+           (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l) << 32;
+}
+template <typename Acc>
+__device__ inline Acc readlane_acc(Acc v, int l) {
+    return __builtin_bit_cast(Acc, readlane64(__builtin_bit_cast(uint64_t, v), l));
+}
+
+// Layout of a tile row in the threads: wavefront w of the workgroup takes
+// the 64 * kSatPer columns from xw = c0 + w * 64 * kSatPer, in kSatPer / 2
+// chunks of 128 columns; lane l holds columns xw + 128 c + 2 l and + 1 of
+// chunk c (element e = 2 c + {0, 1}), so every load or store instruction
+// of the wavefront covers one contiguous run of 128 values.
+constexpr uint32_t kSatChunks = kSatPer / 2;
+struct SatTile {
+    uint64_t z, y0;
+    uint64_t xl;           // this lane's first column: xw + 2 * lane
+    uint32_t b, s, nr;
+    bool whole;            // every column of the wavefront lies in the grid
+};
+__device__ inline SatTile sat_tile(const SatGeom &g, uint64_t t) {
+    const uint64_t per_plane = (uint64_t)g.nb * g.nseg;
+    SatTile T;
+    T.z = t / per_plane;
+    const uint32_t r = (uint32_t)(t - T.z * per_plane);
+    T.b = r / g.nseg;
+    T.s = r - T.b * g.nseg;
+    const uint64_t xw = (uint64_t)T.s * kSatCols + (uint64_t)(threadIdx.x >> 6) * (64 * kSatPer);
+    T.xl = xw + 2 * (threadIdx.x & 63);
+    T.whole = xw + 64 * kSatPer <= g.gx;
+    T.y0 = (uint64_t)T.b * kSatRows;
+    T.nr = (uint32_t)std::min<uint64_t>(kSatRows, g.gy - T.y0);
+    return T;
+}
+__device__ inline uint64_t sat_col(const SatTile &T, uint32_t e) { return T.xl + (e >> 1) * 128 + (e & 1); }
+
+// The kSatPer values of this lane from row pointer p (columns past gx read
+// as 0); pairs as one load when the wavefront's columns are all in the grid
+// and rows start at a multiple of the pair's size.
+template <bool NT = false, typename T>
+__device__ inline void load_lane(const T *p, const SatTile &Tl, uint64_t gx, bool vec, T (&v)[kSatPer]) {
+    if (vec && Tl.whole) {
+        using V = __attribute__((ext_vector_type(2 * sizeof(T) / 4))) uint32_t;
+#pragma unroll
+        for (uint32_t c = 0; c < kSatChunks; c++) {
+            const V *q_at = reinterpret_cast<const V *>(p + Tl.xl + c * 128);
+            const V q = NT ? __builtin_nontemporal_load(q_at) : *q_at;
+            __builtin_memcpy(&v[2 * c], &q, sizeof(V));
+        }
+    } else {
+#pragma unroll
+        for (uint32_t e = 0; e < kSatPer; e++) {
+            const uint64_t x = sat_col(Tl, e);
+            v[e] = x < gx ? p[x] : (T)0;
+        }
+    }
+}
+template <typename Acc>
+__device__ inline void store_lane(Acc *p, const SatTile &Tl, uint64_t gx, bool vec, const Acc (&v)[kSatPer]) {
+    if (vec && Tl.whole) {
+        using V = __attribute__((ext_vector_type(4))) uint32_t;
+#pragma unroll
+        for (uint32_t c = 0; c < kSatChunks; c++) {
+            V q;
+            __builtin_memcpy(&q, &v[2 * c], 16);
+            *reinterpret_cast<V *>(p + Tl.xl + c * 128) = q;
+        }
+    } else {
+#pragma unroll
+        for (uint32_t e = 0; e < kSatPer; e++) {
+            const uint64_t x = sat_col(Tl, e);
+            if (x < gx) p[x] = v[e];
+        }
+    }
+}
+
+// Rows r0 .. r0 + B - 1 of a tile (rows past the tile read its last row
+// again; the caller masks them), all loads issued before any is used;
+// non-temporal when NT.
+template <int B, bool NT = false, typename WIN>
+__device__ inline void load_rows(const WIN *__restrict__ w, const SatGeom &g, const SatTile &T, uint32_t r0,
+                                 WIN (&v)[B][kSatPer]) {
+#pragma unroll
+    for (int b = 0; b < B; b++) {
+        const uint32_t row = std::min<uint32_t>(r0 + b, T.nr - 1);
+        load_lane<NT>(w + (T.z * g.gy + T.y0 + row) * g.gx, T, g.gx, g.vec_w, v[b]);
+    }
+}
+
+// Inclusive scan over the wavefront (every lane active): DPP row shifts
+// within rows of 16 lanes (lanes shifted in read 0), then the earlier rows'
+// totals by readlane.
+template <typename Acc>
+__device__ inline Acc wave_scan(Acc v) {
+    auto shr = [](Acc a, auto ctrl) {
+        return __builtin_bit_cast(Acc, dpp_row64<decltype(ctrl)::value>(__builtin_bit_cast(uint64_t, a)));
+    };
+    v += shr(v, std::integral_constant<int, 0x111>{});
+    v += shr(v, std::integral_constant<int, 0x112>{});
+    v += shr(v, std::integral_constant<int, 0x114>{});
+    v += shr(v, std::integral_constant<int, 0x118>{});
+    auto last = [&](int r) { return __builtin_bit_cast(Acc, rows_last_lane64(__builtin_bit_cast(uint64_t, v), r)); };
+    const Acc t0 = last(0), t1 = last(1), t2 = last(2);
+    const int row = (threadIdx.x & 63) >> 4;
+    return v + (row >= 1 ? t0 : (Acc)0) + (row >= 2 ? t1 : (Acc)0) + (row >= 3 ? t2 : (Acc)0);
+}
+
+// Inclusive prefix of a tile row over the wavefront's columns, in place:
+// per chunk the lanes' pair sums scanned across the wavefront, after the
+// earlier chunks' totals.  Returns the wavefront's total.
+template <typename Acc>
+__device__ inline Acc wave_row_prefix(Acc (&v)[kSatPer]) {
+    Acc carry = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < kSatChunks; c++) {
+        v[2 * c + 1] += v[2 * c];
+        const Acc inc = wave_scan(v[2 * c + 1]);
+        const Acc ex = carry + (inc - v[2 * c + 1]);
+        v[2 * c] += ex;
+        v[2 * c + 1] += ex;
+        carry += readlane_acc(inc, 63);
+    }
+    return carry;
+}
+
+// Exclusive scan of one value per thread over the workgroup; `total` gets
+// the workgroup's sum.  Every thread calls it.
+template <typename Acc>
+__device__ inline Acc block_excl(Acc v, Acc &total, Acc *lds4) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const Acc inc = wave_scan(v);
+    if (lane == 63) lds4[wave] = inc;
+    __syncthreads();
+    Acc off = 0;
+    for (int k = 0; k < wave; k++) off += lds4[k];
+    total = lds4[0] + lds4[1] + lds4[2] + lds4[3];
+    __syncthreads();
+    return off + (inc - v);
+}
+
+// Per tile: the sums over its rows of each column, and over its columns of
+// each row; for f64 weights, whether each is an integer below 2^53 and the
+// sum of their magnitudes (into the copy of the workgroup's XCD).
+template <int K, typename WIN>
+__global__ void __launch_bounds__(kBlock)
+k_sat_sums(const WIN *__restrict__ w, SatGeom g, SatAcc<K> *__restrict__ colsum, SatAcc<K> *__restrict__ rowsum,
+           GridSatFlags *flags) {
+    using Acc = SatAcc<K>;
+    __shared__ Acc red[kSatRows][kBlock / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t n_tiles = (uint64_t)g.nb * g.nseg * g.gz;
+    double abs_sum = 0.0;
+    uint32_t bad = 0;
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const SatTile T = sat_tile(g, t);
+        Acc col[kSatPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kSatPer; j++) col[j] = 0;
+        Acc rowp[kSatRows];
+#pragma unroll
+        for (uint32_t i0 = 0; i0 < kSatRows; i0 += kSatGroup) {
+            WIN v[kSatGroup][kSatPer];
+            load_rows<kSatGroup, true>(w, g, T, i0, v);   // non-temporal: measured faster here, not in k_sat_tiles
+#pragma unroll
+            for (uint32_t b = 0; b < kSatGroup; b++) {
+                const uint32_t i = i0 + b;
+                rowp[i] = 0;
+                if (i >= T.nr) continue;
+#pragma unroll
+                for (uint32_t j = 0; j < kSatPer; j++) {
+                    const Acc u = sat_of<K>(v[b][j]);
+                    col[j] += u;
+                    rowp[i] += u;
+                    if constexpr (K == 2) {
+                        const double d = (double)v[b][j];
+                        abs_sum += fabs(d);
+                        bad |= !(d == trunc(d) && fabs(d) < 0x1p53);
+                    }
+                }
+            }
+        }
+        store_lane(colsum + (T.z * g.nb + T.b) * g.gx, T, g.gx, g.vec_s, col);
+#pragma unroll
+        for (uint32_t i = 0; i < kSatRows; i++) {
+            const Acc r = wave_total(rowp[i]);
+            if (lane == 0) red[i][wave] = r;
+        }
+        __syncthreads();
+        if (threadIdx.x < T.nr)
+            rowsum[((T.z * g.nseg + T.s) * g.gy + T.y0 + threadIdx.x)] =
+                red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        __syncthreads();
+    }
+    if constexpr (K == 2) {
+        abs_sum = wave_total(abs_sum);
+        bad = wave_total(bad);
+        GridSatFlags *f = flags + blockIdx.x % kCopies;   // one copy per XCD
+        if (lane == 0) {
+            if (abs_sum != 0.0) atomicAdd(&f->abs_sum, abs_sum);
+            if (bad) atomicOr(&f->bad, 1u);
+        }
+    }
+}
+
+// Exclusive scan over the rows of each column of `batch` row-major arrays of
+// `rows` x W, in place: a workgroup takes 64 columns, its wavefronts a
+// quarter of the rows each.
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_sat_colscan(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch) {
+    __shared__ Acc part[kBlock / 64][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t ncg = (W + 63) / 64;
+    const uint32_t q = (rows + 3) / 4;
+    const uint32_t r0 = std::min(rows, wave * q), r1 = std::min(rows, r0 + q);
+    for (uint64_t blk = blockIdx.x; blk < ncg * batch; blk += gridDim.x) {
+        const uint64_t p = blk / ncg, cg = blk - p * ncg;
+        const uint64_t col = cg * 64 + lane;
+        const bool valid = col < W;
+        Acc *base = a + p * rows * W + col;
+        Acc sum = 0;
+        if (valid) {
+#pragma unroll 8
+            for (uint32_t r = r0; r < r1; r++) sum += base[(uint64_t)r * W];
+        }
+        part[wave][lane] = sum;
+        __syncthreads();
+        Acc carry = 0;
+        for (int k = 0; k < wave; k++) carry += part[k][lane];
+        if (valid) {
+#pragma unroll 8
+            for (uint32_t r = r0; r < r1; r++) {
+                const Acc v = base[(uint64_t)r * W];
+                base[(uint64_t)r * W] = carry;
+                carry += v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Inclusive scan along each of `lines` contiguous lines of n values, in
+// place; one workgroup per line.
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_sat_linescan(Acc *__restrict__ a, uint64_t n, uint64_t lines) {
+    __shared__ Acc lds4[kBlock / 64];
+    for (uint64_t line = blockIdx.x; line < lines; line += gridDim.x) {
+        Acc *p = a + line * n;
+        Acc carry = 0;
+        for (uint64_t x0 = 0; x0 < n; x0 += kSatCols) {
+            const uint64_t x = x0 + (uint64_t)threadIdx.x * kSatPer;
+            const uint32_t nv = x < n ? (uint32_t)std::min<uint64_t>(kSatPer, n - x) : 0;
+            Acc v[kSatPer];
+#pragma unroll
+            for (uint32_t j = 0; j < kSatPer; j++) v[j] = j < nv ? p[x + j] : (Acc)0;
+#pragma unroll
+            for (uint32_t j = 1; j < kSatPer; j++) v[j] += v[j - 1];
+            Acc total;
+            const Acc off = carry + block_excl(v[kSatPer - 1], total, lds4);
+#pragma unroll
+            for (uint32_t j = 0; j < kSatPer; j++)
+                if (j < nv) p[x + j] = v[j] + off;
+            carry += total;
+        }
+    }
+}
+
+// Per tile (rows y0.., columns c0..): S(y, x) = la(y) + above(x) + local(y, x)
+// where la(y) is the sum of w over rows <= y left of c0, above(x) over rows
+// < y0 and columns c0..x (prefix of the column carries cex), local(y, x)
+// over rows y0..y and columns c0..x.
+template <int K, typename WIN>
+__global__ void __launch_bounds__(kBlock)
+k_sat_tiles(const WIN *__restrict__ w, SatGeom g, const SatAcc<K> *__restrict__ cex,
+            const SatAcc<K> *__restrict__ la, SatAcc<K> *__restrict__ S) {
+    using Acc = SatAcc<K>;
+    __shared__ Acc xw[2][kSatGroup + 1][kBlock / 64];   // wavefront totals per row (+ the carries' row)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t n_tiles = (uint64_t)g.nb * g.nseg * g.gz;
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const SatTile T = sat_tile(g, t);
+        Acc abv[kSatPer];
+        load_lane(cex + (T.z * g.nb + T.b) * g.gx, T, g.gx, g.vec_s, abv);
+        const Acc atot = wave_row_prefix(abv);
+        if (lane == 0) xw[0][kSatGroup][wave] = atot;
+        Acc run[kSatPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kSatPer; j++) run[j] = 0;
+        const Acc *lap = la ? la + ((T.z * g.nseg + T.s) * g.gy + T.y0) : nullptr;
+        uint32_t buf = 0;
+        for (uint32_t g0 = 0; g0 < T.nr; g0 += kSatGroup) {
+            Acc rv[kSatGroup][kSatPer];
+            {
+                WIN v[kSatGroup][kSatPer];
+                load_rows<kSatGroup>(w, g, T, g0, v);
+#pragma unroll
+                for (uint32_t i = 0; i < kSatGroup; i++) {
+                    const bool in = g0 + i < T.nr;
+#pragma unroll
+                    for (uint32_t j = 0; j < kSatPer; j++) rv[i][j] = in ? sat_of<K>(v[i][j]) : (Acc)0;
+                }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kSatGroup; i++) {
+                const Acc tot = wave_row_prefix(rv[i]);
+                if (lane == 0) xw[buf][i][wave] = tot;
+            }
+            __syncthreads();
+            if (g0 == 0) {   // the carries' prefix, after the earlier wavefronts' totals
+                Acc off = 0;
+                for (int k = 0; k < wave; k++) off += xw[0][kSatGroup][k];
+#pragma unroll
+                for (uint32_t j = 0; j < kSatPer; j++) abv[j] += off;
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kSatGroup; i++) {
+                if (g0 + i >= T.nr) break;
+                Acc off = 0;
+                for (int k = 0; k < wave; k++) off += xw[buf][i][k];
+                const Acc lay = lap ? lap[g0 + i] : (Acc)0;
+                Acc o[kSatPer];
+#pragma unroll
+                for (uint32_t j = 0; j < kSatPer; j++) {
+                    run[j] += rv[i][j] + off;
+                    o[j] = lay + abv[j] + run[j];
+                }
+                store_lane(S + (T.z * g.gy + T.y0 + g0 + i) * g.gx, T, g.gx, g.vec_s, o);
+            }
+            buf ^= 1;
+        }
+        __syncthreads();
+    }
+}
+
+// 3D: the band sums over z of the per-plane tables (W = gx * gy cells per
+// plane), then, after their exclusive scan, each band's running sums.
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_sat_zsum(const Acc *__restrict__ S, uint64_t W, uint64_t gz, uint32_t nbz, Acc *__restrict__ bz) {
+    const uint64_t nwb = (W + kBlock - 1) / kBlock;
+    for (uint64_t blk = blockIdx.x; blk < nwb * nbz; blk += gridDim.x) {
+        const uint64_t band = blk / nwb, j = (blk - band * nwb) * kBlock + threadIdx.x;
+        if (j >= W) continue;
+        const uint64_t z0 = band * kSatZBand, z1 = std::min<uint64_t>(gz, z0 + kSatZBand);
+        Acc s = 0;
+#pragma unroll 8
+        for (uint64_t z = z0; z < z1; z++) s += S[z * W + j];
+        bz[band * W + j] = s;
+    }
+}
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_sat_zapply(Acc *__restrict__ S, uint64_t W, uint64_t gz, uint32_t nbz, const Acc *__restrict__ bz) {
+    const uint64_t nwb = (W + kBlock - 1) / kBlock;
+    for (uint64_t blk = blockIdx.x; blk < nwb * nbz; blk += gridDim.x) {
+        const uint64_t band = blk / nwb, j = (blk - band * nwb) * kBlock + threadIdx.x;
+        if (j >= W) continue;
+        const uint64_t z0 = band * kSatZBand, z1 = std::min<uint64_t>(gz, z0 + kSatZBand);
+        Acc c = bz[band * W + j];
+#pragma unroll 8
+        for (uint64_t z = z0; z < z1; z++) {
+            c += S[z * W + j];
+            S[z * W + j] = c;
+        }
+    }
+}
+
+// Sum of w over the box [lo, hi) (hi exclusive) from the inclusive table.
+// In f64 every partial result is a sum over cells with coefficients -1, 0
+// or 1, so at most the sum of |w| in magnitude: exact under the host's check.
+template <int D, typename Acc>
+__device__ inline Acc sat_box(const Acc *__restrict__ S, uint64_t gx, uint64_t gy, const uint64_t (&lo)[3],
+                              const uint64_t (&hi)[3]) {
+    auto at = [&](uint64_t X, uint64_t Y, uint64_t Z) -> Acc {
+        return (X == 0 || Y == 0 || Z == 0) ? (Acc)0 : S[((Z - 1) * gy + (Y - 1)) * gx + (X - 1)];
+    };
+    Acc r = at(hi[0], hi[1], hi[2]) - at(lo[0], hi[1], hi[2]) - at(hi[0], lo[1], hi[2]) + at(lo[0], lo[1], hi[2]);
+    if constexpr (D == 3) {
+        r = r - at(hi[0], hi[1], lo[2]);
+        r = r + at(lo[0], hi[1], lo[2]);
+        r = r + at(hi[0], lo[1], lo[2]);
+        r = r - at(lo[0], lo[1], lo[2]);
+    }
+    return r;
+}
+
+// The carrier of GW<K> of an exact sum of the table.
+template <int K>
+__device__ inline uint64_t sat_carrier(SatAcc<K> v) {
+    if constexpr (K == 0) return (uint64_t)(int64_t)(int32_t)(uint32_t)v;
+    else if constexpr (K == 1) return v;
+    else return __builtin_bit_cast(uint64_t, v);
+}
+
+// One wavefront per subgrid h of level l (2^l of them): weighted_median
+// (rcb.rs:52-99) with every chunk's prefix weight a lookup (the weight of
+// the subgrid's cross-sections before the chunk's start: coupe's
+// left_weight + prefix, the same integer), the decisions by ballot in
+// coupe's order; then the split into the heap and the two halves into
+// slots 2h and 2h + 1 of the next level.
+template <int K, int D>
+__global__ void __launch_bounds__(kBlock)
+k_grid_level(const GSlot *__restrict__ in, GSlot *__restrict__ out, uint32_t *__restrict__ heap, uint32_t l,
+             const SatAcc<K> *__restrict__ S, uint64_t gx, uint64_t gy, uint32_t threads) {
+    using Wt = GW<K>;
+    const int lane = threadIdx.x & 63;
+    const uint64_t n = (uint64_t)1 << l;
+    const uint64_t h = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (h >= n) return;
+    const int coord = (int)((1 + l) % D);
+    const GSlot P = in[h];
+    if (P.size[0] == 0 || P.size[1] == 0 || P.size[2] == 0) {   // no cells: nothing below reaches a cell
+        if (lane == 0) {
+            out[2 * h] = P;
+            out[2 * h + 1] = P;
+            heap[n - 1 + h] = 0;
+        }
+        return;
+    }
+    uint64_t lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = P.off[k];
+        hi[k] = (uint64_t)P.off[k] + P.size[k];
+    }
+    const uint64_t total = l == 0 ? sat_carrier<K>(sat_box<D>(S, gx, gy, lo, hi)) : P.sum;
+    const double ideal = Wt::f64(total) / 2.0;
+    const uint64_t minp = Wt::from_f64(ideal * (1.0 - kGridTolerance));
+    const uint64_t maxp = Wt::from_f64(ideal * (1.0 + kGridTolerance));
+    const uint64_t m = P.size[coord];
+    uint64_t mn = 0, mx = m, left = 0, pos_out = 0, left_out = 0;
+    for (;;) {
+        uint64_t cs = (mx - mn) / threads;
+        if (cs < 1) cs = 1;
+        const uint64_t nch = (mx - mn + cs - 1) / cs;
+        uint64_t nmin = mn, nmax = mx, nleft = left;
+        bool done = false;
+        for (uint64_t base = 0; base < nch; base += 64) {
+            const uint64_t k = base + lane;
+            const bool valid = k < nch;
+            const uint64_t pos = mn + k * cs;
+            uint64_t c = 0;
+            if (valid) {
+                uint64_t h2[3] = {hi[0], hi[1], hi[2]};
+                h2[coord] = lo[coord] + pos;
+                c = sat_carrier<K>(sat_box<D>(S, gx, gy, lo, h2));
+            }
+            const bool stop = valid && !Wt::lt(c, minp);
+            const uint64_t mask = __ballot(stop);
+            if (mask == 0) {   // every chunk of this batch below the minimum
+                const int last = (int)std::min<uint64_t>(63, nch - 1 - base);
+                nmin = readlane64(pos, last);
+                nleft = readlane64(c, last);
+                continue;
+            }
+            const int f = __builtin_ctzll(mask);
+            if (f > 0) {
+                nmin = readlane64(pos, f - 1);
+                nleft = readlane64(c, f - 1);
+            }
+            const uint64_t cf = readlane64(c, f), pf = readlane64(pos, f);
+            if (Wt::lt(maxp, cf)) {
+                nmax = pf;
+            } else {
+                pos_out = pf;
+                left_out = cf;
+                done = true;
+            }
+            break;
+        }
+        if (!done && (nmin + 1 >= nmax || (nmin == mn && nmax == mx))) {
+            pos_out = nmin;
+            left_out = nleft;
+            done = true;
+        }
+        if (done) break;
+        mn = nmin;
+        mx = nmax;
+        left = nleft;
+    }
+    if (lane == 0) {
+        GSlot A = P, B = P;
+        A.size[coord] = (uint32_t)pos_out;
+        A.sum = left_out;
+        B.off[coord] = P.off[coord] + (uint32_t)pos_out;
+        B.size[coord] = (uint32_t)(m - pos_out);
+        B.sum = Wt::sub(total, left_out);
+        out[2 * h] = A;
+        out[2 * h + 1] = B;
+        heap[n - 1 + h] = P.off[coord] + (uint32_t)pos_out;
+    }
+}
+
+// part_of (rcb.rs:21-42) from the heap of splits: at level l the cell goes
+// right when its index along axis (1 + l) mod D is at or past the split.
+// A workgroup takes kBlock * kFillCells cells of one row; each wavefront
+// stores kFillCells windows of 64 consecutive cells.  A leaf is a box, so
+// the cells of a row that lie in one leaf are consecutive: when a window's
+// first and last cells walk to the same leaf, so do all its cells.  The
+// wavefront walks the 2 * kFillCells window ends at once, one per lane, and
+// walks every cell only of the windows that cross a leaf boundary.  The
+// first kFillTop levels of the heap sit in LDS.
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_grid_fill_heap(const uint32_t *__restrict__ heap, uint32_t L, uint64_t gx, uint64_t gy, uint64_t rows,
+                 uint64_t nxc, uint64_t *__restrict__ out) {
+    static_assert(2 * kFillCells <= 64, "window ends per wavefront");
+    __shared__ uint32_t top[(1u << kFillTop) - 1];
+    const uint32_t lt = L < kFillTop ? L : kFillTop;
+    const uint32_t ntop = (1u << lt) - 1;
+    for (uint32_t i = threadIdx.x; i < ntop; i += kBlock) top[i] = heap[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    // the leaf (index within the last level) of cell (x, y, z); levels D at
+    // a time, so that each one's axis is known at compile time
+    auto leaf_of = [&](uint32_t x, uint32_t y, uint32_t z) -> uint32_t {
+        const uint32_t q[3] = {x, y, z};
+        uint32_t n = 0;   // heap node: children 2n + 1, 2n + 2
+        uint32_t l = 0;
+        for (; l + D <= lt; l += D) {
+#pragma unroll
+            for (int k = 0; k < D; k++) n = 2 * n + 1 + (q[(1 + k) % D] >= top[n]);
+        }
+        for (; l < lt; l++) n = 2 * n + 1 + (q[(1 + l) % D] >= top[n]);
+        for (; l < L; l++) n = 2 * n + 1 + (q[(1 + l) % D] >= heap[n]);
+        return n - ((1u << L) - 1);
+    };
+    const uint32_t last = (uint32_t)(gx - 1);
+    for (uint64_t blk = blockIdx.x; blk < rows * nxc; blk += gridDim.x) {
+        const uint64_t row = blk / nxc, xc = blk - row * nxc;
+        const uint32_t y = (uint32_t)(row % gy), z = (uint32_t)(row / gy);
+        // window i of this wavefront starts at w0 + i * kBlock
+        const uint64_t w0 = xc * (kBlock * kFillCells) + (threadIdx.x & ~63u);
+        const uint32_t ei = (uint32_t)(lane >> 1) % kFillCells;
+        const uint64_t xe = w0 + (uint64_t)ei * kBlock + ((lane & 1) ? 63 : 0);
+        const uint32_t end_leaf = leaf_of(xe < gx ? (uint32_t)xe : last, y, z);
+        uint64_t *o = out + row * gx;
+#pragma unroll
+        for (uint32_t i = 0; i < kFillCells; i++) {
+            const uint64_t ws = w0 + (uint64_t)i * kBlock;
+            if (ws >= gx) break;
+            const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)end_leaf, 2 * i);
+            const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)end_leaf, 2 * i + 1);
+            const uint64_t x = ws + lane;
+            uint32_t id = a;
+            if (a != b) id = leaf_of(x < gx ? (uint32_t)x : last, y, z);
+            if (x < gx) o[x] = id;
+        }
+    }
+}
+
 // The rayon pool size coupe would run with: RAYON_NUM_THREADS, else the
 // number of logical CPUs.
 uint32_t default_threads() {
@@ -429,6 +1044,129 @@ void grid_run(Context &cx, hipStream_t st, size_t dim, const uint64_t *size, con
     cx.grid_clean = true;
 }
 
+// Blocks for a grid-stride kernel: enough to fill the device, at most `work`.
+template <typename Kern>
+unsigned fill_grid(Context &cx, Kern kern, uint64_t work) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(work, (uint64_t)cx.cu_count * per_cu));
+}
+
+// The summed-area path.  Returns false when f64 weights are not exactly
+// summable (then nothing it wrote is a result and the caller runs the
+// per-level path).
+template <int K, typename WIN>
+bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size, const WIN *w, size_t L, uint32_t T,
+                  uint64_t *out) {
+    SatGeom g;
+    g.gx = size[0];
+    g.gy = size[1];
+    g.gz = dim == 3 ? size[2] : 1;
+    const uint64_t cells = g.gx * g.gy * g.gz;
+    g.nb = (uint32_t)((g.gy + kSatRows - 1) / kSatRows);
+    g.nseg = (uint32_t)((g.gx + kSatCols - 1) / kSatCols);
+    g.vec_w = g.gx % 2 == 0 && (uintptr_t)w % (2 * sizeof(WIN)) == 0;
+    g.vec_s = g.gx % 2 == 0;
+    const uint64_t n_tiles = (uint64_t)g.nb * g.nseg * g.gz;
+    const uint32_t nbz = (uint32_t)((g.gz + kSatZBand - 1) / kSatZBand);
+    const uint64_t plane = g.gx * g.gy;
+    cx.gsat.ensure(cells * 8);
+    cx.gcol.ensure(std::max<uint64_t>(g.gz * g.nb * g.gx, dim == 3 ? nbz * plane : 0) * 8);
+    cx.grow.ensure(g.gz * g.nseg * g.gy * 8);
+    const uint64_t slots = (uint64_t)1 << L;
+    cx.gparts[0].ensure(slots * sizeof(GSlot));
+    cx.gparts[1].ensure(slots * sizeof(GSlot));
+    cx.gheap.ensure(slots * 4);
+    cx.gctr.ensure(kCopies * sizeof(GridSatFlags));
+    GridSatFlags *flags = (GridSatFlags *)cx.gctr.ptr;
+    using Acc = SatAcc<K>;
+    Acc *S = (Acc *)cx.gsat.ptr, *col = (Acc *)cx.gcol.ptr, *row = (Acc *)cx.grow.ptr;
+    uint32_t *heap = (uint32_t *)cx.gheap.ptr;
+    if (K == 2) ck(hipMemsetAsync(flags, 0, kCopies * sizeof(GridSatFlags), st));
+
+    int r = cx.prof.begin(st, COUPE_HIP_K_PROLOGUE, L ? (double)cells * 2.0 * sizeof(WIN) + (double)cells * 8.0 : 0.0);
+    if (L) {   // no table for a call without levels
+        const unsigned tile_grid = (unsigned)std::min<uint64_t>(n_tiles, 1u << 24);   // a workgroup per tile
+        k_sat_sums<K, WIN><<<tile_grid, kBlock, 0, st>>>(w, g, col, row, flags);
+        ck(hipGetLastError());
+        k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (g.gx + 63) / 64 * g.gz), kBlock, 0, st>>>(
+            col, g.gx, g.nb, (uint32_t)g.gz);
+        ck(hipGetLastError());
+        if (g.nseg > 1) {
+            k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (g.gy + 63) / 64 * g.gz), kBlock, 0, st>>>(
+                row, g.gy, g.nseg, (uint32_t)g.gz);
+            ck(hipGetLastError());
+            k_sat_linescan<Acc><<<fill_grid(cx, k_sat_linescan<Acc>, g.gz * g.nseg), kBlock, 0, st>>>(row, g.gy,
+                                                                                                    g.gz * g.nseg);
+            ck(hipGetLastError());
+        }
+        k_sat_tiles<K, WIN><<<tile_grid, kBlock, 0, st>>>(
+            w, g, col, g.nseg > 1 ? row : nullptr, S);
+        ck(hipGetLastError());
+        if (dim == 3) {
+            const uint64_t work = (plane + kBlock - 1) / kBlock * nbz;
+            k_sat_zsum<Acc><<<fill_grid(cx, k_sat_zsum<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
+            ck(hipGetLastError());
+            k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (plane + 63) / 64), kBlock, 0, st>>>(col, plane, nbz,
+                                                                                                      1);
+            ck(hipGetLastError());
+            k_sat_zapply<Acc><<<fill_grid(cx, k_sat_zapply<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
+            ck(hipGetLastError());
+        }
+    }
+    cx.prof.end(st, r);
+
+    GSlot root;
+    memset(&root, 0, sizeof root);
+    root.size[0] = (uint32_t)g.gx;
+    root.size[1] = (uint32_t)g.gy;
+    root.size[2] = (uint32_t)g.gz;
+    ck(hipMemcpyAsync(cx.gparts[0].ptr, &root, sizeof root, hipMemcpyHostToDevice, st));
+    int cur = 0;
+    r = cx.prof.begin(st, COUPE_HIP_K_DECIDE, 0.0);
+    for (size_t l = 0; l < L; l++) {
+        const GSlot *in = (const GSlot *)cx.gparts[cur].ptr;
+        GSlot *nx = (GSlot *)cx.gparts[cur ^ 1].ptr;
+        const unsigned blocks = (unsigned)((((uint64_t)1 << l) + kBlock / 64 - 1) / (kBlock / 64));
+        if (dim == 2)
+            k_grid_level<K, 2><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, T);
+        else
+            k_grid_level<K, 3><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, T);
+        ck(hipGetLastError());
+        cur ^= 1;
+    }
+    cx.prof.end(st, r);
+
+    const uint64_t rows = g.gy * g.gz, nxc = (g.gx + kBlock * kFillCells - 1) / (kBlock * kFillCells);
+    r = cx.prof.begin(st, COUPE_HIP_K_IDS, (double)cells * 8.0);
+    if (dim == 2)
+        k_grid_fill_heap<2><<<fill_grid(cx, k_grid_fill_heap<2>, rows * nxc), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx,
+                                                                                            g.gy, rows, nxc, out);
+    else
+        k_grid_fill_heap<3><<<fill_grid(cx, k_grid_fill_heap<3>, rows * nxc), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx,
+                                                                                            g.gy, rows, nxc, out);
+    ck(hipGetLastError());
+    cx.prof.end(st, r);
+    if (K != 2) {
+        ck(hipStreamSynchronize(st));
+        return true;
+    }
+    if (!cx.host_scratch) ck(hipHostMalloc(&cx.host_scratch, kHostScratch, hipHostMallocDefault));
+    GridSatFlags *hf = (GridSatFlags *)cx.host_scratch;
+    ck(hipMemcpyAsync(hf, flags, kCopies * sizeof(GridSatFlags), hipMemcpyDeviceToHost, st));
+    ck(hipStreamSynchronize(st));
+    double abs_sum = 0.0;
+    uint32_t bad = 0;
+    for (int c = 0; c < kCopies; c++) {
+        abs_sum += hf[c].abs_sum;
+        bad |= hf[c].bad;
+    }
+    // Every partial sum is an integer of magnitude at most sum |w|; the f64
+    // sum of the magnitudes, over at most 2^36 terms, is within a factor
+    // 1 + 2^-17 of the exact one, so this bound keeps the exact sum <= 2^53.
+    return bad == 0 && abs_sum <= 0x1p53 * (1.0 - 0x1p-15);
+}
+
 }  // namespace
 
 enum coupe_err grid_rcb(Context &cx, hipStream_t st, size_t dim, const uint64_t *size, int wkind, const void *w,
@@ -450,9 +1188,19 @@ enum coupe_err grid_rcb(Context &cx, hipStream_t st, size_t dim, const uint64_t 
             ck(hipDeviceGetAttribute(&cx.cu_count, hipDeviceAttributeMultiprocessorCount, cx.device));
             if (cx.cu_count <= 0) cx.cu_count = 1;
         }
-        if (wkind == COUPE_INT) grid_run<0, int32_t>(cx, st, dim, size, (const int32_t *)w, iter_count, T, out);
-        else if (wkind == COUPE_INT64) grid_run<1, int64_t>(cx, st, dim, size, (const int64_t *)w, iter_count, T, out);
-        else grid_run<2, double>(cx, st, dim, size, (const double *)w, iter_count, T, out);
+        // the summed-area path up to kDenseGridLevels levels (COUPE_HIP_GRID_SAT=0: never)
+        const char *e = getenv("COUPE_HIP_GRID_SAT");
+        const bool sat = iter_count <= kDenseGridLevels && !(e && e[0] == '0');
+        if (wkind == COUPE_INT) {
+            if (!sat || !grid_run_sat<0, int32_t>(cx, st, dim, size, (const int32_t *)w, iter_count, T, out))
+                grid_run<0, int32_t>(cx, st, dim, size, (const int32_t *)w, iter_count, T, out);
+        } else if (wkind == COUPE_INT64) {
+            if (!sat || !grid_run_sat<1, int64_t>(cx, st, dim, size, (const int64_t *)w, iter_count, T, out))
+                grid_run<1, int64_t>(cx, st, dim, size, (const int64_t *)w, iter_count, T, out);
+        } else {
+            if (!sat || !grid_run_sat<2, double>(cx, st, dim, size, (const double *)w, iter_count, T, out))
+                grid_run<2, double>(cx, st, dim, size, (const double *)w, iter_count, T, out);
+        }
         cx.prof.collect();
     } catch (const HipFailure &f) {
         cx.grid_clean = false;

@@ -65,6 +65,49 @@ def test_grid_matches_oracle(oracle_mod, size, kind, iters, threads):
     assert np.array_equal(got, exp), f"{(got != exp).sum()} ids differ"
 
 
+WIDE = [([4500, 40], "i32", 10, 16), ([10000, 300], "f64int", 12, 16), ([2100, 5, 20], "i64", 9, 8),
+        ([4097, 33], "f64int", 11, 5), ([9, 2049, 3], "i32", 8, 16)]
+
+
+@pytest.mark.parametrize("size,kind,iters,threads", WIDE)
+def test_grid_wide_rows_and_z_bands(oracle_mod, size, kind, iters, threads):
+    """Rows wider than one tile of the summed-area table (2048 columns),
+    more than one band of 16 planes along z, odd and even widths."""
+    test_grid_matches_oracle(oracle_mod, size, kind, iters, threads)
+
+
+@pytest.mark.parametrize("sat", ["1", "0"])
+def test_both_paths_agree(oracle_mod, monkeypatch, sat):
+    """The summed-area path and the per-level path (COUPE_HIP_GRID_SAT=0)
+    give the oracle's ids."""
+    monkeypatch.setenv("COUPE_HIP_GRID_SAT", sat)
+    for size, kind, iters, threads in (([300, 200], "i64", 9, 16), ([40, 30, 20], "i32", 8, 3),
+                                       ([2500, 20], "f64int", 7, 16)):
+        test_grid_matches_oracle(oracle_mod, size, kind, iters, threads)
+
+
+def test_non_integral_weights_take_the_level_path(oracle_mod):
+    """Quarter weights are exactly summable in any order, but not integers:
+    the summed-area path hands the call to the per-level path."""
+    rng = np.random.default_rng(11)
+    w = rng.integers(0, 400, 300 * 170).astype(np.float64) / 4.0
+    got = gpu_grid([300, 170], w, 10, 16)
+    assert np.array_equal(got, oracle_mod.grid_rcb([300, 170], w, 10, threads=16))
+
+
+def test_deeper_than_the_dense_heap(oracle_mod):
+    """iter_count above 20 takes the per-level path (most leaves empty)."""
+    rng = np.random.default_rng(12)
+    w = rng.integers(1, 9, 48 * 40).astype(np.int64)
+    got = gpu_grid([48, 40], w, 22, 16)
+    assert np.array_equal(got, oracle_mod.grid_rcb([48, 40], w, 22, threads=16))
+
+
+def test_no_levels(oracle_mod):
+    got = gpu_grid([30, 20], np.ones(600), 0, 16)
+    assert not got.any()
+
+
 def test_i32_sums_wrap(oracle_mod):
     w = np.full(64 * 64, 2**24, dtype=np.int32)   # the total (2^36) wraps in i32
     assert np.array_equal(gpu_grid([64, 64], w, 6, 16), oracle_mod.grid_rcb([64, 64], w, 6, threads=16))
