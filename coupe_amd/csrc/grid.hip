@@ -589,15 +589,18 @@ k_sat_sums(const WIN *__restrict__ w, SatGeom g, SatAcc<K> *__restrict__ colsum,
 }
 
 // Exclusive scan over the rows of each column of `batch` row-major arrays of
-// `rows` x W, in place: a workgroup takes 64 columns, its wavefronts a
-// quarter of the rows each.
+// `rows` x W, in place: a workgroup takes 64 columns, each of its
+// kScanWaves wavefronts a contiguous share of the rows (the columns are few
+// next to the lanes of the device: the rows are split to keep more loads in
+// flight).
+constexpr int kScanWaves = 8;
 template <typename Acc>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(64 * kScanWaves)
 k_sat_colscan(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch) {
-    __shared__ Acc part[kBlock / 64][64];
+    __shared__ Acc part[kScanWaves][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t ncg = (W + 63) / 64;
-    const uint32_t q = (rows + 3) / 4;
+    const uint32_t q = (rows + kScanWaves - 1) / kScanWaves;
     const uint32_t r0 = std::min(rows, wave * q), r1 = std::min(rows, r0 + q);
     for (uint64_t blk = blockIdx.x; blk < ncg * batch; blk += gridDim.x) {
         const uint64_t p = blk / ncg, cg = blk - p * ncg;
@@ -606,7 +609,7 @@ k_sat_colscan(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch) {
         Acc *base = a + p * rows * W + col;
         Acc sum = 0;
         if (valid) {
-#pragma unroll 8
+#pragma unroll 16
             for (uint32_t r = r0; r < r1; r++) sum += base[(uint64_t)r * W];
         }
         part[wave][lane] = sum;
@@ -614,7 +617,7 @@ k_sat_colscan(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch) {
         Acc carry = 0;
         for (int k = 0; k < wave; k++) carry += part[k][lane];
         if (valid) {
-#pragma unroll 8
+#pragma unroll 16
             for (uint32_t r = r0; r < r1; r++) {
                 const Acc v = base[(uint64_t)r * W];
                 base[(uint64_t)r * W] = carry;
@@ -771,6 +774,32 @@ __device__ inline Acc sat_box(const Acc *__restrict__ S, uint64_t gx, uint64_t g
     return r;
 }
 
+// Sum of w over the cells whose index along `coord` is below t and whose
+// other indices lie in [lo, hi): the box's lower face along coord at 0, so
+// 2^(D-1) lookups.  A subgrid's axis prefix is face(off + pos) - face(off).
+template <int D, typename Acc>
+__device__ inline Acc sat_face(const Acc *__restrict__ S, uint64_t gx, uint64_t gy, const uint64_t (&lo)[3],
+                               const uint64_t (&hi)[3], int coord, uint64_t t) {
+    auto at = [&](const uint64_t (&q)[3]) -> Acc {
+        return (q[0] == 0 || q[1] == 0 || q[2] == 0) ? (Acc)0 : S[((q[2] - 1) * gy + (q[1] - 1)) * gx + (q[0] - 1)];
+    };
+    uint64_t a[3] = {hi[0], hi[1], hi[2]};
+    a[coord] = t;
+    const int o1 = coord == 0 ? 1 : 0;
+    uint64_t b[3] = {a[0], a[1], a[2]};
+    b[o1] = lo[o1];
+    Acc r = at(a) - at(b);
+    if constexpr (D == 3) {
+        const int o2 = 3 - coord - o1;
+        uint64_t c[3] = {a[0], a[1], a[2]}, d[3] = {b[0], b[1], b[2]};
+        c[o2] = lo[o2];
+        d[o2] = lo[o2];
+        r = r - at(c);
+        r = r + at(d);
+    }
+    return r;
+}
+
 // The carrier of GW<K> of an exact sum of the table.
 template <int K>
 __device__ inline uint64_t sat_carrier(SatAcc<K> v) {
@@ -782,20 +811,23 @@ __device__ inline uint64_t sat_carrier(SatAcc<K> v) {
 // One wavefront per subgrid h of level l (2^l of them): weighted_median
 // (rcb.rs:52-99) with every chunk's prefix weight a lookup (the weight of
 // the subgrid's cross-sections before the chunk's start: coupe's
-// left_weight + prefix, the same integer), the decisions by ballot in
+// left_weight + prefix, the same integer; the first chunk's is the left
+// weight itself), the decisions by ballot in
 // coupe's order; then the split into the heap and the two halves into
 // slots 2h and 2h + 1 of the next level.
 template <int K, int D>
 __global__ void __launch_bounds__(kBlock)
 k_grid_level(const GSlot *__restrict__ in, GSlot *__restrict__ out, uint32_t *__restrict__ heap, uint32_t l,
-             const SatAcc<K> *__restrict__ S, uint64_t gx, uint64_t gy, uint32_t threads) {
+             const SatAcc<K> *__restrict__ S, uint64_t gx, uint64_t gy, uint64_t gz, uint32_t threads) {
     using Wt = GW<K>;
     const int lane = threadIdx.x & 63;
     const uint64_t n = (uint64_t)1 << l;
     const uint64_t h = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     if (h >= n) return;
     const int coord = (int)((1 + l) % D);
-    const GSlot P = in[h];
+    GSlot P;   // level 0: the whole grid
+    if (l == 0) P = GSlot{{0, 0, 0}, {(uint32_t)gx, (uint32_t)gy, (uint32_t)gz}, 0};
+    else P = in[h];
     if (P.size[0] == 0 || P.size[1] == 0 || P.size[2] == 0) {   // no cells: nothing below reaches a cell
         if (lane == 0) {
             out[2 * h] = P;
@@ -814,6 +846,7 @@ k_grid_level(const GSlot *__restrict__ in, GSlot *__restrict__ out, uint32_t *__
     const uint64_t minp = Wt::from_f64(ideal * (1.0 - kGridTolerance));
     const uint64_t maxp = Wt::from_f64(ideal * (1.0 + kGridTolerance));
     const uint64_t m = P.size[coord];
+    const SatAcc<K> f0 = sat_face<D>(S, gx, gy, lo, hi, coord, lo[coord]);
     uint64_t mn = 0, mx = m, left = 0, pos_out = 0, left_out = 0;
     for (;;) {
         uint64_t cs = (mx - mn) / threads;
@@ -825,12 +858,8 @@ k_grid_level(const GSlot *__restrict__ in, GSlot *__restrict__ out, uint32_t *__
             const uint64_t k = base + lane;
             const bool valid = k < nch;
             const uint64_t pos = mn + k * cs;
-            uint64_t c = 0;
-            if (valid) {
-                uint64_t h2[3] = {hi[0], hi[1], hi[2]};
-                h2[coord] = lo[coord] + pos;
-                c = sat_carrier<K>(sat_box<D>(S, gx, gy, lo, h2));
-            }
+            uint64_t c = left;   // the first chunk starts at the left weight's position
+            if (valid && k > 0) c = sat_carrier<K>(sat_face<D>(S, gx, gy, lo, hi, coord, lo[coord] + pos) - f0);
             const bool stop = valid && !Wt::lt(c, minp);
             const uint64_t mask = __ballot(stop);
             if (mask == 0) {   // every chunk of this batch below the minimum
@@ -1045,10 +1074,40 @@ void grid_run(Context &cx, hipStream_t st, size_t dim, const uint64_t *size, con
 }
 
 // Blocks for a grid-stride kernel: enough to fill the device, at most `work`.
+// The occupancy query is a runtime call: asked once per kernel and context.
 template <typename Kern>
 unsigned fill_grid(Context &cx, Kern kern, uint64_t work) {
+    const void *key = reinterpret_cast<const void *>(kern);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 2;
+    for (const auto &e : cx.occupancy)
+        if (e.first == key) {
+            per_cu = e.second;
+            break;
+        }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0) != hipSuccess || per_cu < 1)
+            per_cu = 2;
+        cx.occupancy.emplace_back(key, per_cu);
+    }
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(work, (uint64_t)cx.cu_count * per_cu));
+}
+
+// The same for the column scans' workgroups of 64 * kScanWaves threads.
+template <typename Kern>
+unsigned scan_grid(Context &cx, Kern kern, uint64_t work) {
+    const void *key = reinterpret_cast<const void *>(kern);
+    int per_cu = 0;
+    for (const auto &e : cx.occupancy)
+        if (e.first == key) {
+            per_cu = e.second;
+            break;
+        }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * kScanWaves, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        cx.occupancy.emplace_back(key, per_cu);
+    }
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(work, (uint64_t)cx.cu_count * per_cu));
 }
 
@@ -1089,12 +1148,12 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
         const unsigned tile_grid = (unsigned)std::min<uint64_t>(n_tiles, 1u << 24);   // a workgroup per tile
         k_sat_sums<K, WIN><<<tile_grid, kBlock, 0, st>>>(w, g, col, row, flags);
         ck(hipGetLastError());
-        k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (g.gx + 63) / 64 * g.gz), kBlock, 0, st>>>(
+        k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (g.gx + 63) / 64 * g.gz), 64 * kScanWaves, 0, st>>>(
             col, g.gx, g.nb, (uint32_t)g.gz);
         ck(hipGetLastError());
         if (g.nseg > 1) {
-            k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (g.gy + 63) / 64 * g.gz), kBlock, 0, st>>>(
-                row, g.gy, g.nseg, (uint32_t)g.gz);
+            k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (g.gy + 63) / 64 * g.gz), 64 * kScanWaves, 0,
+                                 st>>>(row, g.gy, g.nseg, (uint32_t)g.gz);
             ck(hipGetLastError());
             k_sat_linescan<Acc><<<fill_grid(cx, k_sat_linescan<Acc>, g.gz * g.nseg), kBlock, 0, st>>>(row, g.gy,
                                                                                                     g.gz * g.nseg);
@@ -1107,8 +1166,8 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
             const uint64_t work = (plane + kBlock - 1) / kBlock * nbz;
             k_sat_zsum<Acc><<<fill_grid(cx, k_sat_zsum<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
             ck(hipGetLastError());
-            k_sat_colscan<Acc><<<fill_grid(cx, k_sat_colscan<Acc>, (plane + 63) / 64), kBlock, 0, st>>>(col, plane, nbz,
-                                                                                                      1);
+            k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (plane + 63) / 64), 64 * kScanWaves, 0, st>>>(
+                col, plane, nbz, 1);
             ck(hipGetLastError());
             k_sat_zapply<Acc><<<fill_grid(cx, k_sat_zapply<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
             ck(hipGetLastError());
@@ -1116,12 +1175,6 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
     }
     cx.prof.end(st, r);
 
-    GSlot root;
-    memset(&root, 0, sizeof root);
-    root.size[0] = (uint32_t)g.gx;
-    root.size[1] = (uint32_t)g.gy;
-    root.size[2] = (uint32_t)g.gz;
-    ck(hipMemcpyAsync(cx.gparts[0].ptr, &root, sizeof root, hipMemcpyHostToDevice, st));
     int cur = 0;
     r = cx.prof.begin(st, COUPE_HIP_K_DECIDE, 0.0);
     for (size_t l = 0; l < L; l++) {
@@ -1129,9 +1182,9 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
         GSlot *nx = (GSlot *)cx.gparts[cur ^ 1].ptr;
         const unsigned blocks = (unsigned)((((uint64_t)1 << l) + kBlock / 64 - 1) / (kBlock / 64));
         if (dim == 2)
-            k_grid_level<K, 2><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, T);
+            k_grid_level<K, 2><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, g.gz, T);
         else
-            k_grid_level<K, 3><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, T);
+            k_grid_level<K, 3><<<blocks, kBlock, 0, st>>>(in, nx, heap, (uint32_t)l, S, g.gx, g.gy, g.gz, T);
         ck(hipGetLastError());
         cur ^= 1;
     }
