@@ -33,7 +33,7 @@ sample of the same generator.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                        [--no-cpu] [--cpu-threads T] [--plain-only]
-                       [--serve-requests R]
+                       [--serve-requests R] [--grid-only]
 """
 import argparse
 import json
@@ -108,6 +108,8 @@ def main():
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--grid-only", action="store_true",
+                    help="only coupe's Grid benchmark (10000 x 10000, L = 12); prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the partitioner runs on a "
@@ -118,6 +120,10 @@ def main():
 
     import coupe_amd
     from coupe_amd import workloads
+
+    if args.grid_only:
+        print(json.dumps(bench_grid(torch, coupe_amd, with_cpu=not args.no_cpu, steps=args.steps)), flush=True)
+        return
 
     stream_probe = probe_stream()
     t0 = time.time()
