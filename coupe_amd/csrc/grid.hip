@@ -346,7 +346,8 @@ k_grid_fill(const GridPart *__restrict__ parts, const uint32_t *n_parts, const u
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSatRows = 16;                 // rows per tile
 constexpr uint32_t kSatPer = 8;                   // consecutive columns per thread
-constexpr uint32_t kSatCols = kBlock * kSatPer;   // columns per tile
+constexpr uint32_t kSatCols = 64 * kSatPer;       // columns per tile (one wavefront)
+constexpr uint32_t kLineChunk = kBlock * kSatPer;  // values per step of a line scan
 constexpr uint32_t kSatGroup = 4;                 // rows per barrier in k_sat_tiles
 constexpr uint32_t kSatZBand = 16;                // planes per band of the z scan
 constexpr uint32_t kDenseGridLevels = 20;         // deeper calls take the per-level path
@@ -392,11 +393,12 @@ __device__ inline Acc readlane_acc(Acc v, int l) {
     return __builtin_bit_cast(Acc, readlane64(__builtin_bit_cast(uint64_t, v), l));
 }
 
-// Layout of a tile row in the threads: wavefront w of the workgroup takes
-// the 64 * kSatPer columns from xw = c0 + w * 64 * kSatPer, in kSatPer / 2
-// chunks of 128 columns; lane l holds columns xw + 128 c + 2 l and + 1 of
-// chunk c (element e = 2 c + {0, 1}), so every load or store instruction
-// of the wavefront covers one contiguous run of 128 values.
+// A tile is kSatRows rows x kSatCols columns, one wavefront's work (the
+// wavefronts of a workgroup take tiles independently).  Layout of a tile row
+// in the lanes: kSatPer / 2 chunks of 128 columns from the tile's first
+// column xw; lane l holds columns xw + 128 c + 2 l and + 1 of chunk c
+// (element e = 2 c + {0, 1}), so every load or store instruction of the
+// wavefront covers one contiguous run of 128 values.
 constexpr uint32_t kSatChunks = kSatPer / 2;
 struct SatTile {
     uint64_t z, y0;
@@ -411,7 +413,7 @@ __device__ inline SatTile sat_tile(const SatGeom &g, uint64_t t) {
     const uint32_t r = (uint32_t)(t - T.z * per_plane);
     T.b = r / g.nseg;
     T.s = r - T.b * g.nseg;
-    const uint64_t xw = (uint64_t)T.s * kSatCols + (uint64_t)(threadIdx.x >> 6) * (64 * kSatPer);
+    const uint64_t xw = (uint64_t)T.s * kSatCols;
     T.xl = xw + 2 * (threadIdx.x & 63);
     T.whole = xw + 64 * kSatPer <= g.gx;
     T.y0 = (uint64_t)T.b * kSatRows;
@@ -532,17 +534,17 @@ __global__ void __launch_bounds__(kBlock)
 k_sat_sums(const WIN *__restrict__ w, SatGeom g, SatAcc<K> *__restrict__ colsum, SatAcc<K> *__restrict__ rowsum,
            GridSatFlags *flags) {
     using Acc = SatAcc<K>;
-    __shared__ Acc red[kSatRows][kBlock / 64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint64_t n_tiles = (uint64_t)g.nb * g.nseg * g.gz;
     double abs_sum = 0.0;
     uint32_t bad = 0;
-    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t t = (uint64_t)blockIdx.x * (kBlock / 64) + wave; t < n_tiles; t += stride) {
         const SatTile T = sat_tile(g, t);
         Acc col[kSatPer];
 #pragma unroll
         for (uint32_t j = 0; j < kSatPer; j++) col[j] = 0;
-        Acc rowp[kSatRows];
+        Acc mine = 0;   // lane i: the sum of row i of the tile
 #pragma unroll
         for (uint32_t i0 = 0; i0 < kSatRows; i0 += kSatGroup) {
             WIN v[kSatGroup][kSatPer];
@@ -550,32 +552,25 @@ k_sat_sums(const WIN *__restrict__ w, SatGeom g, SatAcc<K> *__restrict__ colsum,
 #pragma unroll
             for (uint32_t b = 0; b < kSatGroup; b++) {
                 const uint32_t i = i0 + b;
-                rowp[i] = 0;
                 if (i >= T.nr) continue;
+                Acc rowp = 0;
 #pragma unroll
                 for (uint32_t j = 0; j < kSatPer; j++) {
                     const Acc u = sat_of<K>(v[b][j]);
                     col[j] += u;
-                    rowp[i] += u;
+                    rowp += u;
                     if constexpr (K == 2) {
                         const double d = (double)v[b][j];
                         abs_sum += fabs(d);
                         bad |= !(d == trunc(d) && fabs(d) < 0x1p53);
                     }
                 }
+                const Acc r = wave_total(rowp);
+                if (lane == (int)i) mine = r;
             }
         }
         store_lane(colsum + (T.z * g.nb + T.b) * g.gx, T, g.gx, g.vec_s, col);
-#pragma unroll
-        for (uint32_t i = 0; i < kSatRows; i++) {
-            const Acc r = wave_total(rowp[i]);
-            if (lane == 0) red[i][wave] = r;
-        }
-        __syncthreads();
-        if (threadIdx.x < T.nr)
-            rowsum[((T.z * g.nseg + T.s) * g.gy + T.y0 + threadIdx.x)] =
-                red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
-        __syncthreads();
+        if (lane < (int)T.nr) rowsum[(T.z * g.nseg + T.s) * g.gy + T.y0 + lane] = mine;
     }
     if constexpr (K == 2) {
         abs_sum = wave_total(abs_sum);
@@ -628,6 +623,74 @@ k_sat_colscan(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch) {
     }
 }
 
+// The same scan for tall arrays (few columns, many rows), in three steps:
+// the sums of chunks of kScanChunk rows (k_sat_chunksum), their exclusive
+// scan over the chunks (k_sat_colscan), then each chunk's scan from its
+// carry (k_sat_chunkapply); a workgroup per chunk and 64 columns.
+constexpr uint32_t kScanChunk = 64 * kScanWaves;
+template <typename Acc>
+__global__ void __launch_bounds__(64 * kScanWaves)
+k_sat_chunksum(const Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch, Acc *__restrict__ part_sums) {
+    __shared__ Acc part[kScanWaves][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t ncg = (W + 63) / 64;
+    const uint32_t nch = (rows + kScanChunk - 1) / kScanChunk;
+    for (uint64_t blk = blockIdx.x; blk < ncg * nch * batch; blk += gridDim.x) {
+        const uint64_t p = blk / (ncg * nch), r = blk - p * ncg * nch, ch = r / ncg, cg = r - ch * ncg;
+        const uint64_t col = cg * 64 + lane;
+        const uint32_t r0 = (uint32_t)ch * kScanChunk + wave * (kScanChunk / kScanWaves);
+        const uint32_t r1 = std::min<uint32_t>(rows, r0 + kScanChunk / kScanWaves);
+        Acc sum = 0;
+        if (col < W) {
+            const Acc *base = a + p * rows * W + col;
+#pragma unroll 16
+            for (uint32_t q = r0; q < r1; q++) sum += base[(uint64_t)q * W];
+        }
+        part[wave][lane] = sum;
+        __syncthreads();
+        if (wave == 0 && col < W) {
+            Acc t = 0;
+            for (int k = 0; k < kScanWaves; k++) t += part[k][lane];
+            part_sums[(p * nch + ch) * W + col] = t;
+        }
+        __syncthreads();
+    }
+}
+template <typename Acc>
+__global__ void __launch_bounds__(64 * kScanWaves)
+k_sat_chunkapply(Acc *__restrict__ a, uint64_t W, uint32_t rows, uint32_t batch, const Acc *__restrict__ carries) {
+    __shared__ Acc part[kScanWaves][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint64_t ncg = (W + 63) / 64;
+    const uint32_t nch = (rows + kScanChunk - 1) / kScanChunk;
+    for (uint64_t blk = blockIdx.x; blk < ncg * nch * batch; blk += gridDim.x) {
+        const uint64_t p = blk / (ncg * nch), r = blk - p * ncg * nch, ch = r / ncg, cg = r - ch * ncg;
+        const uint64_t col = cg * 64 + lane;
+        const bool valid = col < W;
+        const uint32_t r0 = (uint32_t)ch * kScanChunk + wave * (kScanChunk / kScanWaves);
+        const uint32_t r1 = std::min<uint32_t>(rows, r0 + kScanChunk / kScanWaves);
+        Acc *base = a + p * rows * W + col;
+        Acc sum = 0;
+        if (valid) {
+#pragma unroll 16
+            for (uint32_t q = r0; q < r1; q++) sum += base[(uint64_t)q * W];
+        }
+        part[wave][lane] = sum;
+        __syncthreads();
+        Acc carry = valid ? carries[(p * nch + ch) * W + col] : (Acc)0;
+        for (int k = 0; k < wave; k++) carry += part[k][lane];
+        if (valid) {
+#pragma unroll 16
+            for (uint32_t q = r0; q < r1; q++) {
+                const Acc v = base[(uint64_t)q * W];
+                base[(uint64_t)q * W] = carry;
+                carry += v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Inclusive scan along each of `lines` contiguous lines of n values, in
 // place; one workgroup per line.
 template <typename Acc>
@@ -637,7 +700,7 @@ k_sat_linescan(Acc *__restrict__ a, uint64_t n, uint64_t lines) {
     for (uint64_t line = blockIdx.x; line < lines; line += gridDim.x) {
         Acc *p = a + line * n;
         Acc carry = 0;
-        for (uint64_t x0 = 0; x0 < n; x0 += kSatCols) {
+        for (uint64_t x0 = 0; x0 < n; x0 += kLineChunk) {
             const uint64_t x = x0 + (uint64_t)threadIdx.x * kSatPer;
             const uint32_t nv = x < n ? (uint32_t)std::min<uint64_t>(kSatPer, n - x) : 0;
             Acc v[kSatPer];
@@ -664,20 +727,18 @@ __global__ void __launch_bounds__(kBlock)
 k_sat_tiles(const WIN *__restrict__ w, SatGeom g, const SatAcc<K> *__restrict__ cex,
             const SatAcc<K> *__restrict__ la, SatAcc<K> *__restrict__ S) {
     using Acc = SatAcc<K>;
-    __shared__ Acc xw[2][kSatGroup + 1][kBlock / 64];   // wavefront totals per row (+ the carries' row)
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const uint64_t n_tiles = (uint64_t)g.nb * g.nseg * g.gz;
-    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * (kBlock / 64);
+    for (uint64_t t = (uint64_t)blockIdx.x * (kBlock / 64) + wave; t < n_tiles; t += stride) {
         const SatTile T = sat_tile(g, t);
         Acc abv[kSatPer];
         load_lane(cex + (T.z * g.nb + T.b) * g.gx, T, g.gx, g.vec_s, abv);
-        const Acc atot = wave_row_prefix(abv);
-        if (lane == 0) xw[0][kSatGroup][wave] = atot;
+        wave_row_prefix(abv);
         Acc run[kSatPer];
 #pragma unroll
         for (uint32_t j = 0; j < kSatPer; j++) run[j] = 0;
         const Acc *lap = la ? la + ((T.z * g.nseg + T.s) * g.gy + T.y0) : nullptr;
-        uint32_t buf = 0;
         for (uint32_t g0 = 0; g0 < T.nr; g0 += kSatGroup) {
             Acc rv[kSatGroup][kSatPer];
             {
@@ -692,33 +753,18 @@ k_sat_tiles(const WIN *__restrict__ w, SatGeom g, const SatAcc<K> *__restrict__ 
             }
 #pragma unroll
             for (uint32_t i = 0; i < kSatGroup; i++) {
-                const Acc tot = wave_row_prefix(rv[i]);
-                if (lane == 0) xw[buf][i][wave] = tot;
-            }
-            __syncthreads();
-            if (g0 == 0) {   // the carries' prefix, after the earlier wavefronts' totals
-                Acc off = 0;
-                for (int k = 0; k < wave; k++) off += xw[0][kSatGroup][k];
-#pragma unroll
-                for (uint32_t j = 0; j < kSatPer; j++) abv[j] += off;
-            }
-#pragma unroll
-            for (uint32_t i = 0; i < kSatGroup; i++) {
                 if (g0 + i >= T.nr) break;
-                Acc off = 0;
-                for (int k = 0; k < wave; k++) off += xw[buf][i][k];
+                wave_row_prefix(rv[i]);
                 const Acc lay = lap ? lap[g0 + i] : (Acc)0;
                 Acc o[kSatPer];
 #pragma unroll
                 for (uint32_t j = 0; j < kSatPer; j++) {
-                    run[j] += rv[i][j] + off;
+                    run[j] += rv[i][j];
                     o[j] = lay + abv[j] + run[j];
                 }
                 store_lane(S + (T.z * g.gy + T.y0 + g0 + i) * g.gx, T, g.gx, g.vec_s, o);
             }
-            buf ^= 1;
         }
-        __syncthreads();
     }
 }
 
@@ -1111,6 +1157,28 @@ unsigned scan_grid(Context &cx, Kern kern, uint64_t work) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(work, (uint64_t)cx.cu_count * per_cu));
 }
 
+// Exclusive scan over the rows of each column of `batch` arrays of rows x W
+// (in place).  Arrays too narrow to give the device enough workgroups of 64
+// columns are scanned in chunks of rows (`tmp`: room for their sums).
+template <typename Acc>
+void colscan(Context &cx, hipStream_t st, Acc *a, uint64_t W, uint32_t rows, uint32_t batch, Acc *tmp) {
+    const uint64_t ncg = (W + 63) / 64;
+    const uint32_t nch = (rows + kScanChunk - 1) / kScanChunk;
+    if (ncg * batch >= 2 * (uint64_t)cx.cu_count || nch < 2 || !tmp) {
+        k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, ncg * batch), 64 * kScanWaves, 0, st>>>(a, W, rows,
+                                                                                                     batch);
+        ck(hipGetLastError());
+        return;
+    }
+    const uint64_t work = ncg * nch * batch;
+    k_sat_chunksum<Acc><<<scan_grid(cx, k_sat_chunksum<Acc>, work), 64 * kScanWaves, 0, st>>>(a, W, rows, batch, tmp);
+    ck(hipGetLastError());
+    colscan<Acc>(cx, st, tmp, W, nch, batch, nullptr);
+    k_sat_chunkapply<Acc><<<scan_grid(cx, k_sat_chunkapply<Acc>, work), 64 * kScanWaves, 0, st>>>(a, W, rows, batch,
+                                                                                                  tmp);
+    ck(hipGetLastError());
+}
+
 // The summed-area path.  Returns false when f64 weights are not exactly
 // summable (then nothing it wrote is a result and the caller runs the
 // per-level path).
@@ -1132,6 +1200,9 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
     cx.gsat.ensure(cells * 8);
     cx.gcol.ensure(std::max<uint64_t>(g.gz * g.nb * g.gx, dim == 3 ? nbz * plane : 0) * 8);
     cx.grow.ensure(g.gz * g.nseg * g.gy * 8);
+    auto chunks = [](uint64_t rows) { return (rows + kScanChunk - 1) / kScanChunk; };
+    cx.gscan.ensure(std::max({chunks(g.nb) * g.gx * g.gz, chunks(g.nseg) * g.gy * g.gz,
+                              dim == 3 ? chunks(nbz) * plane : (uint64_t)0, (uint64_t)1}) * 8);
     const uint64_t slots = (uint64_t)1 << L;
     cx.gparts[0].ensure(slots * sizeof(GSlot));
     cx.gparts[1].ensure(slots * sizeof(GSlot));
@@ -1139,22 +1210,18 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
     cx.gctr.ensure(kCopies * sizeof(GridSatFlags));
     GridSatFlags *flags = (GridSatFlags *)cx.gctr.ptr;
     using Acc = SatAcc<K>;
-    Acc *S = (Acc *)cx.gsat.ptr, *col = (Acc *)cx.gcol.ptr, *row = (Acc *)cx.grow.ptr;
+    Acc *S = (Acc *)cx.gsat.ptr, *col = (Acc *)cx.gcol.ptr, *row = (Acc *)cx.grow.ptr, *tmp = (Acc *)cx.gscan.ptr;
     uint32_t *heap = (uint32_t *)cx.gheap.ptr;
     if (K == 2) ck(hipMemsetAsync(flags, 0, kCopies * sizeof(GridSatFlags), st));
 
     int r = cx.prof.begin(st, COUPE_HIP_K_PROLOGUE, L ? (double)cells * 2.0 * sizeof(WIN) + (double)cells * 8.0 : 0.0);
     if (L) {   // no table for a call without levels
-        const unsigned tile_grid = (unsigned)std::min<uint64_t>(n_tiles, 1u << 24);   // a workgroup per tile
+        const unsigned tile_grid = (unsigned)std::min<uint64_t>((n_tiles + 3) / 4, 1u << 24);   // wavefront per tile
         k_sat_sums<K, WIN><<<tile_grid, kBlock, 0, st>>>(w, g, col, row, flags);
         ck(hipGetLastError());
-        k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (g.gx + 63) / 64 * g.gz), 64 * kScanWaves, 0, st>>>(
-            col, g.gx, g.nb, (uint32_t)g.gz);
-        ck(hipGetLastError());
+        colscan<Acc>(cx, st, col, g.gx, g.nb, (uint32_t)g.gz, tmp);
         if (g.nseg > 1) {
-            k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (g.gy + 63) / 64 * g.gz), 64 * kScanWaves, 0,
-                                 st>>>(row, g.gy, g.nseg, (uint32_t)g.gz);
-            ck(hipGetLastError());
+            colscan<Acc>(cx, st, row, g.gy, g.nseg, (uint32_t)g.gz, tmp);
             k_sat_linescan<Acc><<<fill_grid(cx, k_sat_linescan<Acc>, g.gz * g.nseg), kBlock, 0, st>>>(row, g.gy,
                                                                                                     g.gz * g.nseg);
             ck(hipGetLastError());
@@ -1166,9 +1233,7 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
             const uint64_t work = (plane + kBlock - 1) / kBlock * nbz;
             k_sat_zsum<Acc><<<fill_grid(cx, k_sat_zsum<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
             ck(hipGetLastError());
-            k_sat_colscan<Acc><<<scan_grid(cx, k_sat_colscan<Acc>, (plane + 63) / 64), 64 * kScanWaves, 0, st>>>(
-                col, plane, nbz, 1);
-            ck(hipGetLastError());
+            colscan<Acc>(cx, st, col, plane, nbz, 1, tmp);
             k_sat_zapply<Acc><<<fill_grid(cx, k_sat_zapply<Acc>, work), kBlock, 0, st>>>(S, plane, g.gz, nbz, col);
             ck(hipGetLastError());
         }

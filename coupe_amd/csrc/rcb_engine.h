@@ -179,7 +179,7 @@ struct Context {
     DevBuf loads;                    // per-part loads (imbalance)
     DevBuf tile_bins;                // per-tile bin counts of the histogram passes
     DevBuf gparts[2], gaxis, gchunk, gcount, gctr;   // Grid::rcb: subgrids, axis sums, chunk offsets
-    DevBuf gsat, gcol, grow, gheap;  // Grid::rcb summed-area path: table, carries, heap of splits
+    DevBuf gsat, gcol, grow, gheap, gscan;   // Grid::rcb summed-area path: table, carries, heap of splits, chunk sums
     bool grid_clean = false;         // Grid::rcb axis sums are all zero
     DevBuf rib_moments;              // Rib: block partials, centroid, inertia
     coupe_hip_rib_info rib{};        // Rib: rotation of the last call

@@ -1518,8 +1518,8 @@ void Context::for_each_buf(F &&f) {
                      &tiles[2], &pleft[0], &pleft[1], &pbelow2[0], &pbelow2[1], &fwd_cnt, &fwd_w, &fwd_mn,
                      &tile_bins2, &tile4, &bcnt, &bw, &bmin, &xcnt, &xw, &xmin, &nchild, &ntiles, &bsum, &nodes, &leaf,
                      &heap_split, &instances, &tiles0, &below, &tile_left, &loads, &rib_moments, &tile_bins,
-                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &gsat, &gcol, &grow, &gheap, &ired, &counters, &in_points,
-                     &in_weights, &out_ids};
+                     &gparts[0], &gparts[1], &gaxis, &gchunk, &gcount, &gctr, &gsat, &gcol, &grow, &gheap, &gscan,
+                     &ired, &counters, &in_points, &in_weights, &out_ids};
     for (DevBuf *d : all) f(*d);
 }
 

@@ -66,13 +66,15 @@ def test_grid_matches_oracle(oracle_mod, size, kind, iters, threads):
 
 
 WIDE = [([4500, 40], "i32", 10, 16), ([10000, 300], "f64int", 12, 16), ([2100, 5, 20], "i64", 9, 8),
-        ([4097, 33], "f64int", 11, 5), ([9, 2049, 3], "i32", 8, 16)]
+        ([4097, 33], "f64int", 11, 5), ([9, 2049, 3], "i32", 8, 16), ([100, 9000], "i64", 10, 16),
+        ([3, 4, 9000], "i32", 9, 16), ([1100000, 3], "f64int", 8, 16)]
 
 
 @pytest.mark.parametrize("size,kind,iters,threads", WIDE)
 def test_grid_wide_rows_and_z_bands(oracle_mod, size, kind, iters, threads):
     """Rows wider than one tile of the summed-area table (2048 columns),
-    more than one band of 16 planes along z, odd and even widths."""
+    more than one band of 16 planes along z, odd and even widths, and tall
+    narrow shapes whose scans over bands, segments or planes run in chunks."""
     test_grid_matches_oracle(oracle_mod, size, kind, iters, threads)
 
 
