@@ -954,17 +954,19 @@ k_grid_level(const GSlot *__restrict__ in, GSlot *__restrict__ out, uint32_t *__
 
 // part_of (rcb.rs:21-42) from the heap of splits: at level l the cell goes
 // right when its index along axis (1 + l) mod D is at or past the split.
-// A workgroup takes kBlock * kFillCells cells of one row; each wavefront
-// stores kFillCells windows of 64 consecutive cells.  A leaf is a box, so
-// the cells of a row that lie in one leaf are consecutive: when a window's
-// first and last cells walk to the same leaf, so do all its cells.  The
-// wavefront walks the 2 * kFillCells window ends at once, one per lane, and
-// walks every cell only of the windows that cross a leaf boundary.  The
-// first kFillTop levels of the heap sit in LDS.
+// Cells in memory order: a workgroup takes kBlock * kFillCells consecutive
+// cells, each wavefront stores kFillCells windows of 64 consecutive cells.
+// A leaf is a box, so the cells of a row that lie in one leaf are
+// consecutive: when a window lies in one row and its first and last cells
+// walk to the same leaf, so do all its cells.  The wavefront walks the
+// 2 * kFillCells window ends at once, one per lane, and walks every cell only
+// of the windows that cross a leaf boundary or a row.  Rows and columns come
+// from f64 reciprocals of the sizes, corrected by one step (exact for
+// indices below 2^52).  The first kFillTop levels of the heap sit in LDS.
 template <int D>
 __global__ void __launch_bounds__(kBlock)
-k_grid_fill_heap(const uint32_t *__restrict__ heap, uint32_t L, uint64_t gx, uint64_t gy, uint64_t rows,
-                 uint64_t nxc, uint64_t *__restrict__ out) {
+k_grid_fill_heap(const uint32_t *__restrict__ heap, uint32_t L, uint64_t gx, uint64_t gy, uint64_t cells,
+                 double inv_gx, double inv_gy, uint64_t *__restrict__ out) {
     static_assert(2 * kFillCells <= 64, "window ends per wavefront");
     __shared__ uint32_t top[(1u << kFillTop) - 1];
     const uint32_t lt = L < kFillTop ? L : kFillTop;
@@ -986,26 +988,57 @@ k_grid_fill_heap(const uint32_t *__restrict__ heap, uint32_t L, uint64_t gx, uin
         for (; l < L; l++) n = 2 * n + 1 + (q[(1 + l) % D] >= heap[n]);
         return n - ((1u << L) - 1);
     };
-    const uint32_t last = (uint32_t)(gx - 1);
-    for (uint64_t blk = blockIdx.x; blk < rows * nxc; blk += gridDim.x) {
-        const uint64_t row = blk / nxc, xc = blk - row * nxc;
-        const uint32_t y = (uint32_t)(row % gy), z = (uint32_t)(row / gy);
+    // a / b and a % b for a < 2^52
+    auto divmod = [](uint64_t a, uint64_t b, double inv_b, uint64_t &r) -> uint64_t {
+        uint64_t qt = (uint64_t)((double)a * inv_b);
+        int64_t rr = (int64_t)(a - qt * b);
+        if (rr < 0) {
+            qt--;
+            rr += (int64_t)b;
+        } else if ((uint64_t)rr >= b) {
+            qt++;
+            rr -= (int64_t)b;
+        }
+        r = (uint64_t)rr;
+        return qt;
+    };
+    // the leaf of cell f of row `row` (the index of its run of gx cells)
+    auto leaf_in_row = [&](uint64_t f, uint64_t row) -> uint32_t {
+        const uint64_t x = f - row * gx;
+        uint64_t y = row, z = 0;
+        if constexpr (D == 3) z = divmod(row, gy, inv_gy, y);
+        return leaf_of((uint32_t)x, (uint32_t)y, (uint32_t)z);
+    };
+    // the row and the leaf of cell f
+    auto cell_leaf = [&](uint64_t f, uint64_t &row) -> uint32_t {
+        uint64_t x;
+        row = divmod(f, gx, inv_gx, x);
+        return leaf_in_row(f, row);
+    };
+    const uint64_t chunks = (cells + kBlock * kFillCells - 1) / (kBlock * kFillCells);
+    for (uint64_t blk = blockIdx.x; blk < chunks; blk += gridDim.x) {
         // window i of this wavefront starts at w0 + i * kBlock
-        const uint64_t w0 = xc * (kBlock * kFillCells) + (threadIdx.x & ~63u);
+        const uint64_t w0 = blk * (kBlock * kFillCells) + (threadIdx.x & ~63u);
         const uint32_t ei = (uint32_t)(lane >> 1) % kFillCells;
-        const uint64_t xe = w0 + (uint64_t)ei * kBlock + ((lane & 1) ? 63 : 0);
-        const uint32_t end_leaf = leaf_of(xe < gx ? (uint32_t)xe : last, y, z);
-        uint64_t *o = out + row * gx;
+        const uint64_t fe = std::min<uint64_t>(w0 + (uint64_t)ei * kBlock + ((lane & 1) ? 63 : 0), cells - 1);
+        uint64_t end_row;
+        const uint32_t end_leaf = cell_leaf(fe, end_row);
 #pragma unroll
         for (uint32_t i = 0; i < kFillCells; i++) {
             const uint64_t ws = w0 + (uint64_t)i * kBlock;
-            if (ws >= gx) break;
+            if (ws >= cells) break;
             const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)end_leaf, 2 * i);
             const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)end_leaf, 2 * i + 1);
-            const uint64_t x = ws + lane;
+            const uint64_t ra = readlane64(end_row, 2 * i), rb = readlane64(end_row, 2 * i + 1);
+            const uint64_t f = ws + lane;
             uint32_t id = a;
-            if (a != b) id = leaf_of(x < gx ? (uint32_t)x : last, y, z);
-            if (x < gx) o[x] = id;
+            if (ra != rb) {   // the window crosses a row
+                uint64_t row;
+                id = cell_leaf(f < cells ? f : cells - 1, row);
+            } else if (a != b) {   // one row, more than one leaf
+                id = leaf_in_row(f < cells ? f : cells - 1, ra);
+            }
+            if (f < cells) out[f] = id;
         }
     }
 }
@@ -1255,14 +1288,15 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
     }
     cx.prof.end(st, r);
 
-    const uint64_t rows = g.gy * g.gz, nxc = (g.gx + kBlock * kFillCells - 1) / (kBlock * kFillCells);
+    const uint64_t fill_chunks = (cells + kBlock * kFillCells - 1) / (kBlock * kFillCells);
+    const double inv_gx = 1.0 / (double)g.gx, inv_gy = 1.0 / (double)g.gy;
     r = cx.prof.begin(st, COUPE_HIP_K_IDS, (double)cells * 8.0);
     if (dim == 2)
-        k_grid_fill_heap<2><<<fill_grid(cx, k_grid_fill_heap<2>, rows * nxc), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx,
-                                                                                            g.gy, rows, nxc, out);
+        k_grid_fill_heap<2><<<fill_grid(cx, k_grid_fill_heap<2>, fill_chunks), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx, g.gy,
+                                                                                        cells, inv_gx, inv_gy, out);
     else
-        k_grid_fill_heap<3><<<fill_grid(cx, k_grid_fill_heap<3>, rows * nxc), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx,
-                                                                                            g.gy, rows, nxc, out);
+        k_grid_fill_heap<3><<<fill_grid(cx, k_grid_fill_heap<3>, fill_chunks), kBlock, 0, st>>>(heap, (uint32_t)L, g.gx, g.gy,
+                                                                                        cells, inv_gx, inv_gy, out);
     ck(hipGetLastError());
     cx.prof.end(st, r);
     if (K != 2) {
