@@ -767,8 +767,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                                                      skip_axis);
                     });
                 };
-                if (fwd_out) move(k_partition4<D, WS, Acc, true>);
-                else move(k_partition4<D, WS, Acc, false>);
+                // the stores are non-temporal for moves larger than the switch
+                const bool nt = (2.0 * payload - (skip_axis < 3 ? 4.0 : 0.0)) * (double)n > cx.move_nt_bytes;
+                if (fwd_out) nt ? move(k_partition4<D, WS, Acc, true, true>) : move(k_partition4<D, WS, Acc, true, false>);
+                else nt ? move(k_partition4<D, WS, Acc, false, true>) : move(k_partition4<D, WS, Acc, false, false>);
             }
             // the grandchildren's search over their own tiles
             const uint64_t pub2 = std::max<uint64_t>(B.parts_ub[level + 2], 1);

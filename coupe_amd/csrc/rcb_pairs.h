@@ -493,8 +493,10 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
 // count below t4).
 // skip_axis (0..2, or 3 for none): a coordinate array that no later level
 // reads (3D, the last move: the children's axis is not split again), read for
-// the classification but not written.
-template <int D, typename WS, typename Acc, bool FWD>
+// the classification but not written.  NT: the stores are non-temporal
+// (measured faster where the arrays written are much larger than the
+// last-level cache, slower where they fit in it).
+template <int D, typename WS, typename Acc, bool FWD, bool NT>
 __global__ void __launch_bounds__(kBlock)
 k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, const Part *__restrict__ grand,
              const Tile *__restrict__ tiles, const uint32_t *n_tiles, const uint64_t *__restrict__ p0,
@@ -809,11 +811,19 @@ k_partition4(const Part *__restrict__ parts, const Part *__restrict__ children, 
                         const uint64_t vm = __builtin_amdgcn_ballot_w64(vl);   // lanes holding a position of the segment
                         const uint32_t i = (pos + (uint32_t)lane) & (4 * kTile - 1);
                         if (vl) {
-                            if constexpr (D == 3)
-                                if (keep2) __builtin_nontemporal_store(xc[u], &pa2[i]);
-                            __builtin_nontemporal_store(xa[u], &pa3[i]);
-                            __builtin_nontemporal_store(xb[u], &pa4[i]);
-                            __builtin_nontemporal_store(wq[u], &pw[i]);
+                            if constexpr (NT) {
+                                if constexpr (D == 3)
+                                    if (keep2) __builtin_nontemporal_store(xc[u], &pa2[i]);
+                                __builtin_nontemporal_store(xa[u], &pa3[i]);
+                                __builtin_nontemporal_store(xb[u], &pa4[i]);
+                                __builtin_nontemporal_store(wq[u], &pw[i]);
+                            } else {
+                                if constexpr (D == 3)
+                                    if (keep2) pa2[i] = xc[u];
+                                pa3[i] = xa[u];
+                                pa4[i] = xb[u];
+                                pw[i] = wq[u];
+                            }
                         }
                         const bool lt3 = !(xa[u] >= t3w);
                         const uint64_t bb = __builtin_amdgcn_ballot_w64(lt3) & vm;
