@@ -330,13 +330,15 @@ k_grid_fill(const GridPart *__restrict__ parts, const uint32_t *n_parts, const u
 // weights are summed in 64-bit two's complement (a ring: any order gives the
 // wrapping sums of coupe's i32 / i64 loops); integer-valued doubles whose
 // absolute sum is at most 2^53 have exact sums in any order, so their
-// table, kept in int64, gives coupe's f64 sums bit for bit.  Other doubles
+// table, kept in f64, gives coupe's f64 sums bit for bit.  Other doubles
 // take the per-level path above.  The table costs two reads of the weights
 // and one write of 8 bytes per cell (2D; 3D adds a scan along z), where the
 // per-level path reads the weights once per level; each level is then one
 // wavefront per subgrid doing the median's rounds on lookups:
-//   k_sat_sums        per tile of 16 rows x 2048 columns: column and row sums
-//   k_sat_colscan     exclusive scans over the tiles' rows / columns
+//   k_sat_sums        per tile of 16 rows x 512 columns (one wavefront):
+//                     column and row sums
+//   k_sat_colscan     exclusive scans over the tiles' rows / columns (tall
+//   k_sat_chunk*      arrays in chunks of rows: sums, scan, apply)
 //   k_sat_linescan    inclusive scans along rows of the row carries
 //   k_sat_tiles       per tile: the table from the weights and the carries
 //   k_sat_zsum/zapply 3D: the scan along z, in bands of 16 planes
@@ -345,10 +347,10 @@ k_grid_fill(const GridPart *__restrict__ parts, const uint32_t *n_parts, const u
 //   k_grid_fill_heap  each cell walks the heap to its part id (part_of)
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSatRows = 16;                 // rows per tile
-constexpr uint32_t kSatPer = 8;                   // consecutive columns per thread
+constexpr uint32_t kSatPer = 8;                   // columns per lane of a tile row
 constexpr uint32_t kSatCols = 64 * kSatPer;       // columns per tile (one wavefront)
 constexpr uint32_t kLineChunk = kBlock * kSatPer;  // values per step of a line scan
-constexpr uint32_t kSatGroup = 4;                 // rows per barrier in k_sat_tiles
+constexpr uint32_t kSatGroup = 4;                 // rows whose loads are issued together
 constexpr uint32_t kSatZBand = 16;                // planes per band of the z scan
 constexpr uint32_t kDenseGridLevels = 20;         // deeper calls take the per-level path
 constexpr uint32_t kFillTop = 12;                 // heap levels the fill keeps in LDS
