@@ -120,6 +120,19 @@ __device__ __forceinline__ double wave_total(double v) {   // 0 bits are 0.0
            (f(rows_last_lane64(u(v), 2)) + f(rows_last_lane64(u(v), 3)));
 }
 
+// The call's counters that many workgroups bump (parts still searching,
+// points swept) are kept in one copy per XCD (copy blockIdx mod kCopies):
+// an atomic on one line shared by every XCD moves the line between their
+// L2s at each update.  A gate (the parts still searching after a decision)
+// is the sum of its copies.
+__device__ __forceinline__ uint32_t xcd_copy() { return blockIdx.x % kCopies; }
+__device__ __forceinline__ uint32_t gate_count(const uint32_t *g) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int c = 0; c < kCopies; c++) s += g[c];
+    return s;
+}
+
 // Copies 1..7 of a search pass's bins (copy 0 is the bins themselves).
 // Workgroups go to the XCDs in turn (blockIdx mod 8), each XCD with its own
 // L2; with few parts, every workgroup of the grid flushes into the same few

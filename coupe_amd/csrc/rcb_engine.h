@@ -17,7 +17,7 @@ namespace coupe_hip {
 
 struct Rotation;   // rcb_common.h
 
-constexpr size_t kHostScratch = 16384;   // pinned readback area of a context
+constexpr size_t kHostScratch = 65536;   // pinned readback area of a context
 
 // A HIP failure inside the engine; entry points map it onto coupe's codes.
 struct HipFailure {

@@ -145,25 +145,33 @@ static void timed(Context &cx, hipStream_t st, int kind, double bytes, F &&launc
 // Small device counters shared by the kernels of one call.
 
 constexpr int kSpecSlots = 1024;   // (level, pass) slots of the speculative search
+// The counters that many workgroups bump hold one copy per XCD (the kernels
+// add into copy blockIdx mod kCopies); readers sum the copies.
 struct Counters {
     uint32_t n_parts[3];
     uint32_t n_tiles[3];
-    uint32_t searching;
-    uint32_t pad;
+    uint32_t searching[kCopies];                // parts still searching (host-checked search)
     uint64_t parts_total;
     uint64_t tiles_total;
     uint64_t min_id;
     // read back at the end of the call, from here
-    unsigned long long swept;                   // points swept by standalone passes
-    unsigned long long swept_fused;             // points binned inside partitions
-    unsigned long long first_read;              // points read by the children's first passes
-    Prologue pro[kCopies];                      // the prologue's weight flags, one copy per XCD
-    uint32_t forwarded;                         // parts whose children's first pass the move forwarded
+    unsigned long long swept[kCopies];          // points swept by standalone passes
+    unsigned long long swept_fused[kCopies];    // points binned inside partitions
+    unsigned long long first_read[kCopies];     // points read by the children's first passes
+    Prologue pro[kCopies];                      // the prologue's weight flags
+    uint32_t forwarded[kCopies];                // parts whose children's first pass the move forwarded
     uint32_t unfinished;                        // a speculative search did not end in its passes
     uint32_t error;                             // the node-record descent found an inconsistent tree
-    uint32_t spec[kSpecSlots];                  // parts still searching after pass p of level l
+    uint32_t spec[kSpecSlots * kCopies];        // parts still searching after pass p of level l (slot's copies)
     uint32_t end;                               // end of the call's readback
 };
+
+template <typename T>
+static T sum_copies(const T *c) {
+    T s = 0;
+    for (int k = 0; k < kCopies; k++) s += c[k];
+    return s;
+}
 
 // The call's counters, read back into host_scratch once the stream has run
 // it: errors, the bytes its search passes read, and per level the decisions
@@ -175,9 +183,10 @@ static bool finish_levels(Context &cx) {
     const char *h_rb = (const char *)cx.host_scratch + 256;
     const Counters *h = (const Counters *)(h_rb - rb_off);   // only the copied fields are read
     if (h->error) throw HipFailure{hipErrorUnknown};
-    cx.swept_bytes = (double)h->swept * (4.0 + P.ws_bytes) + (double)h->first_read * (8.0 + P.ws_bytes);
-    cx.stats.first_read = h->first_read;
-    cx.stats.forwarded = h->forwarded;
+    cx.swept_bytes = (double)sum_copies(h->swept) * (4.0 + P.ws_bytes) +
+                     (double)sum_copies(h->first_read) * (8.0 + P.ws_bytes);
+    cx.stats.first_read = sum_copies(h->first_read);
+    cx.stats.forwarded = sum_copies(h->forwarded);
     cx.bins_clean = true;
     if (P.spec == 0) return false;
     // passes per level: the first decision plus every pass whose gate was open
@@ -186,11 +195,11 @@ static bool finish_levels(Context &cx) {
     for (size_t l = 0; l < P.levels; l++) {
         uint32_t passes = 1;
         for (int p = 1; p <= P.depth[l]; p++)
-            if (h->spec[l * (P.spec + 1) + p - 1]) passes++;
+            if (sum_copies(&h->spec[(l * (P.spec + 1) + p - 1) * kCopies])) passes++;
         cx.stats.passes += passes;
         cx.stats.max_passes = std::max(cx.stats.max_passes, passes);
         used[l] = (uint8_t)passes;
-        open1[l] = h->spec[l * (P.spec + 1)];
+        open1[l] = sum_copies(&h->spec[l * (P.spec + 1) * kCopies]);
     }
     if (h->unfinished == 0) {   // (an unfinished run keeps the last call's figures for its rerun)
         Hints &e = cx.remember(P.shape);
@@ -532,48 +541,58 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         const int nb = choose_bins(parts_ub, cx.max_bins);
         const BinCopies<Acc> xc = copies_for(parts_ub);
         cx.stats.bins = std::max<uint32_t>(cx.stats.bins, (uint32_t)nb);
+        // a decision over nbd bins: a lane per part up to kFirstBins bins
+        // without bin copies (levels of many parts), a wavefront per part up
+        // to 64 bins, else a workgroup
+        auto decide = [&](int nbd, uint32_t *searching, unsigned long long *swept, uint32_t counts,
+                          const uint32_t *gate, uint32_t *unfin) {
+            if (nbd <= kFirstBins && !xc.cnt)
+                k_decide_lanes<Acc><<<grid_for(parts_ub, kBlock), kBlock, 0, st>>>(
+                    parts, np, bcnt, bw, bmin, nbd, wrap32, searching, swept, counts, gate, unfin, xc);
+            else if (nbd <= 64)
+                k_decide_wave<Acc><<<(unsigned)((parts_ub + kBlock / 64 - 1) / (kBlock / 64)), kBlock, 0, st>>>(
+                    parts, np, bcnt, bw, bmin, nbd, wrap32, searching, swept, counts, gate, unfin, xc);
+            else
+                k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, nbd, wrap32, searching,
+                                                                   swept, counts, gate, unfin, xc);
+        };
         if (spec_passes > 0) {
             // slot p counts the parts still searching after decision p; pass
             // p + 1 runs only if it is non-zero
-            uint32_t *slot = &ctr->spec[level * (spec_passes + 1)];
+            uint32_t *slot = &ctr->spec[level * (spec_passes + 1) * kCopies];   // slot p: slot + p * kCopies
             const int queued = depth[level];
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
                 // with no pass queued this decision closes the searches it leaves open
-                k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
-                                                               &slot[0], swept_first, counts_first, nullptr,
-                                                               queued == 0 ? &ctr->unfinished : nullptr, xc);
+                decide(first_nb, slot, swept_first, counts_first, nullptr,
+                       queued == 0 ? &ctr->unfinished : nullptr);
             });
             for (int p = 1; p <= queued; p++) {
-                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, &slot[p - 1]); });
+                timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, slot + (p - 1) * kCopies); });
                 // the last queued decision also closes the searches it leaves open
                 timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                    k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(
-                        parts, np, bcnt, bw, bmin, nb, wrap32, &slot[p], &ctr->swept, counts_hist, &slot[p - 1],
-                        p == queued ? &ctr->unfinished : nullptr, xc);
+                    decide(nb, slot + p * kCopies, ctr->swept, counts_hist, slot + (p - 1) * kCopies,
+                           p == queued ? &ctr->unfinished : nullptr);
                 });
             }
             return;
         }
-        check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
+        check(hipMemsetAsync(ctr->searching, 0, sizeof(ctr->searching), st));
         timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-            k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, first_nb, wrap32,
-                                                           &ctr->searching, swept_first, counts_first, nullptr,
-                                                           nullptr, xc);
+            decide(first_nb, ctr->searching, swept_first, counts_first, nullptr, nullptr);
         });
         uint32_t passes = 1;
         for (;;) {
-            check(hipMemcpyAsync(h_search, &ctr->searching, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            check(hipMemcpyAsync(h_search, ctr->searching, sizeof(ctr->searching), hipMemcpyDeviceToHost, st));
             check(hipStreamSynchronize(st));
-            if (passes == 1) open1[level] = *h_search;
-            if (*h_search == 0) break;
+            const uint32_t searching = sum_copies(h_search);
+            if (passes == 1) open1[level] = searching;
+            if (searching == 0) break;
             // f32 bisection ends within a few hundred candidates; each pass takes 8
             if (passes >= 4096) throw HipFailure{hipErrorUnknown};
-            check(hipMemsetAsync(&ctr->searching, 0, sizeof(uint32_t), st));
+            check(hipMemsetAsync(ctr->searching, 0, sizeof(ctr->searching), st));
             timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist(nb, nullptr); });
             timed(cx, st, COUPE_HIP_K_DECIDE, 0.0, [&] {
-                k_decide<Acc><<<(unsigned)parts_ub, kBlock, 0, st>>>(parts, np, bcnt, bw, bmin, nb, wrap32,
-                                                               &ctr->searching, &ctr->swept, counts_hist, nullptr,
-                                                               nullptr, xc);
+                decide(nb, ctr->searching, ctr->swept, counts_hist, nullptr, nullptr);
             });
             passes++;
         }
@@ -590,7 +609,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         uint32_t *ntl = &ctr->n_tiles[sl];
         const unsigned g = persistent_grid(cx, k_hist<WS, Acc>, tiles_ub);
         search_gen(sl, level, parts_ub, first_standalone ? kFirstBins : 2,
-                   first_standalone ? &ctr->swept : &ctr->swept_fused, counts_first, hist_counts,
+                   first_standalone ? ctr->swept : ctr->swept_fused, counts_first, hist_counts,
                    [&](int nb, const uint32_t *gate) {
                        k_hist<WS, Acc><<<g, kBlock, 0, st>>>(parts, tiles, ntl, cl[0], cl[1], cl[2], wl, nb, bcnt,
                                                              bw, bmin, gate, tile_bins, copies_for(parts_ub),
@@ -669,7 +688,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 k_children_write<Acc><<<(unsigned)pub, kBlock, 0, st>>>(
                     pf, &ctr->n_parts[from], D, self_plan ? nullptr : (const uint64_t *)cx.nchild.ptr,
                     self_plan ? nullptr : (const uint64_t *)cx.ntiles.ptr, (Part *)cx.parts[to].ptr, next_tiles,
-                    nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr, &ctr->forwarded,
+                    nodes, 0, 0, heap_split, insts, fsrc, bcnt, bw, bmin, fclr, ctr->forwarded,
                     &ctr->n_parts[to], &ctr->n_tiles[to]);
                 if (pub > kPlanParts)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[to],
@@ -717,7 +736,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 // of every point (counted with the first passes by the decision)
                 const int nbf = choose_bins(pub1, cx.max_bins);
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] { hist_v(nbf, nullptr); });
-                search_gen(ib, level + 1, pub1, nbf, &ctr->first_read, kCountsBins, kCountsBins, hist_v);
+                search_gen(ib, level + 1, pub1, nbf, ctr->first_read, kCountsBins, kCountsBins, hist_v);
             } else {
                 const unsigned g = persistent_grid(cx, k_child_first<D, WS, Acc>, tl_ub);
                 // bytes: parent-axis and child-axis coordinates and the weight
@@ -725,9 +744,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 timed(cx, st, COUPE_HIP_K_HIST, 0.0, [&] {
                     k_child_first<D, WS, Acc><<<g, kBlock, 0, st>>>(pa, tl, ntl, cl[0], cl[1], cl[2], wl, bcnt, bw,
                                                                    bmin, pb2, pl, fwd_in ? 1u : 0u,
-                                                                   &ctr->first_read);
+                                                                   ctr->first_read);
                 });
-                search_gen(ib, level + 1, pub1, 2, &ctr->swept_fused, kCountsBelow, kCountsBins, hist_v);
+                search_gen(ib, level + 1, pub1, 2, ctr->swept_fused, kCountsBelow, kCountsBins, hist_v);
             }
             // grandchildren, with tiles
             Tile *tn = (Tile *)cx.tiles[ic].ptr;
@@ -816,7 +835,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                     parts, np, D, self_plan ? nullptr : (const uint64_t *)cx.nchild.ptr,
                     self_plan ? nullptr : (const uint64_t *)cx.ntiles.ptr, (Part *)cx.parts[nxt].ptr,
                     (Tile *)cx.tiles[nxt].ptr, nodes, node_base[level], node_base[std::min(level + 1, L)], heap_split,
-                    insts, Forward<Acc>{}, bcnt, bw, bmin, Forward<Acc>{}, &ctr->forwarded, &ctr->n_parts[nxt],
+                    insts, Forward<Acc>{}, bcnt, bw, bmin, Forward<Acc>{}, ctr->forwarded, &ctr->n_parts[nxt],
                     &ctr->n_tiles[nxt]);
                 if (!plan1)
                     k_set_sizes<<<1, 64, 0, st>>>(&ctr->parts_total, &ctr->tiles_total, &ctr->n_parts[nxt],
@@ -878,7 +897,9 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
     // One readback for the whole call: the counters from `swept` to the end
     // of the speculative slots.
     const size_t rb_off = offsetof(Counters, swept);
-    const size_t rb_len = offsetof(Counters, end) - rb_off;
+    // up to the last speculative slot this call uses
+    const size_t rb_len = offsetof(Counters, spec) - rb_off +
+                          (spec_passes > 0 ? L * (size_t)(spec_passes + 1) * kCopies * sizeof(uint32_t) : 0);
     static_assert(offsetof(Counters, end) - offsetof(Counters, swept) <= kHostScratch - 256, "readback");
     char *h_rb = (char *)cx.host_scratch + 256;
     check(hipMemcpyAsync(h_rb, (char *)ctr + rb_off, rb_len, hipMemcpyDeviceToHost, st));
@@ -914,9 +935,11 @@ static int weight_mode_of(const Prologue &p, int wkind, bool weights16) {
 // counters; the host reads the copies with the call's last counters (or,
 // when it must wait for them, right after the prologue, into the early
 // area) and combines them.
+constexpr size_t kPrologueEarly = 49152;   // the early prologue readback, past the counters' readback
 static Prologue *prologue_early(Context &cx) {
-    static_assert(kHostScratch >= 8192 + kCopies * sizeof(Prologue), "prologue readback");
-    return (Prologue *)((char *)cx.host_scratch + 8192);
+    static_assert(kHostScratch >= kPrologueEarly + kCopies * sizeof(Prologue), "prologue readback");
+    static_assert(256 + offsetof(Counters, end) - offsetof(Counters, swept) <= kPrologueEarly, "readback areas");
+    return (Prologue *)((char *)cx.host_scratch + kPrologueEarly);
 }
 static const Prologue *prologue_late(Context &cx) {   // in the counters' readback area
     return (const Prologue *)((const char *)cx.host_scratch + 256 + offsetof(Counters, pro) -

@@ -582,7 +582,7 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
        const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt, Acc *__restrict__ bw,
        uint32_t *__restrict__ bmin, const uint32_t *gate, uint16_t *__restrict__ tile_bins, BinCopies<Acc> xc,
        uint32_t chunk) {
-    if (gate && *gate == 0) return;   // no part left searching after the previous decision
+    if (gate && gate_count(gate) == 0) return;   // no part left searching after the previous decision
     __shared__ float cand[kMaxBins];
     __shared__ uint32_t scnt[kMaxBins];
     __shared__ Acc sw[kMaxBins];
@@ -657,7 +657,71 @@ k_hist(const Part *__restrict__ parts, const Tile *__restrict__ tiles, const uin
 }
 
 // ---------------------------------------------------------------------------
-// Decision replay: one wavefront per searching part.
+// Decision replay of one part from its pass's bins: pc, pw the inclusive
+// prefix counts and weights over the bins, sm[j] the smallest key of bins
+// j.. (sm[nb]: none), as arrays in LDS or in the thread's registers.  Run by
+// one thread per part; true when the part is still searching after it (the
+// caller counts those).
+// ---------------------------------------------------------------------------
+template <typename Acc, typename PC, typename PW, typename SM>
+__device__ bool decide_replay(Part *P, const PC &pc, const PW &pw, const SM &sm, int nb, int wrap32,
+                              uint32_t counts_ready, uint32_t *unfinished) {
+    // Replay of par_rcb_split (:470-571) over the candidate tree.
+    using Ops = AccOps<Acc>;
+    const Acc S = Ops::from_bits(P->sum);
+    const uint64_t n = P->count;
+    const double tol = P->tol;
+    float lo = P->blo, hi = P->bhi;
+    uint64_t prev = P->prev;
+    int step = nb >> 1;
+    int q = step - 1;
+    for (;;) {
+        const float t = (lo + hi) / 2.0f;
+        const uint64_t cnt = pc[q];
+        const Acc wl = pw[q];
+        bool left;
+        if (cnt == n) {                                   // no point at or right of t
+            if (prev == cnt) {                            // :528-539
+                P->status = kAllLeft; P->split = hi; P->wl = Ops::to_bits(S);
+                P->cnt_left = n; P->kstar = -1;
+                return false;
+            }
+            hi = t; prev = cnt;                           // :540-544
+            left = true;
+        } else {
+            const float mnear = key_f32(sm[q + 1]);       // smallest c >= t
+            const float d = mnear - t;                    // nearest_distance
+            const double ideal = Ops::f64(S, wrap32) / 2.0;
+            const double imb = fabs((Ops::f64(wl, wrap32) - ideal) / ideal);
+            if (cnt == prev || hi <= t + d || imb <= tol) {   // :552-563
+                P->status = kSplitAtT; P->split = t; P->wl = Ops::to_bits(wl);
+                P->cnt_left = cnt; P->kstar = q;
+                P->counts_ready = counts_ready;   // per-tile left counts already known
+                return false;
+            }
+            prev = cnt;                                   // :564
+            if (Ops::lt(wl, Ops::sub(S, wl), wrap32)) { lo = t; left = false; }
+            else { hi = t; left = true; }
+        }
+        step >>= 1;
+        if (step == 0) break;
+        q = left ? q - step : q + step;
+    }
+    P->blo = lo; P->bhi = hi; P->prev = prev;             // next pass continues here
+    if (unfinished) {
+        P->status = kAllLeft;
+        P->split = hi;
+        P->cnt_left = n;
+        P->wl = P->sum;
+        P->kstar = -1;
+        atomicOr(unfinished, 1u);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Decision replay: one workgroup per searching part (passes of more than 64
+// bins; k_decide_wave takes the others).
 // ---------------------------------------------------------------------------
 template <typename Acc>
 __global__ void __launch_bounds__(kBlock)
@@ -669,7 +733,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     // search; a part still searching after it is closed as "all left" (a
     // consistent tree for the kernels that follow) and the call is flagged,
     // so the host runs it again with a search that checks every pass
-    if (gate && *gate == 0) return;   // no part left searching after the previous decision
+    if (gate && gate_count(gate) == 0) return;   // no part left searching after the previous decision
     __shared__ uint64_t pc[kMaxBins];
     __shared__ Acc pw[kMaxBins];
     __shared__ uint32_t sm[kMaxBins + 1];
@@ -709,7 +773,7 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     // the first wavefront scans the bins (the others hold no bins: j0 >= nb)
     // and its lane 0 replays the decisions
     const int lane = threadIdx.x;
-    if (lane == 0) atomicAdd(points_swept, (unsigned long long)P->count);   // swept by the last pass
+    if (lane == 0) atomicAdd(&points_swept[xcd_copy()], (unsigned long long)P->count);   // swept by the last pass
     const int per = nb >= 64 ? nb / 64 : 1;
     const int j0 = lane * per;
     const bool mine = j0 < nb;
@@ -761,60 +825,131 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
     if (lane == 0) sm[nb] = 0xffffffffu;
     __syncthreads();
     if (lane != 0) return;
-
-    // Replay of par_rcb_split (:470-571) over the candidate tree.
-    using Ops = AccOps<Acc>;
-    const Acc S = Ops::from_bits(P->sum);
-    const uint64_t n = P->count;
-    const double tol = P->tol;
-    float lo = P->blo, hi = P->bhi;
-    uint64_t prev = P->prev;
-    int step = nb >> 1;
-    int q = step - 1;
-    for (;;) {
-        const float t = (lo + hi) / 2.0f;
-        const uint64_t cnt = pc[q];
-        const Acc wl = pw[q];
-        bool left;
-        if (cnt == n) {                                   // no point at or right of t
-            if (prev == cnt) {                            // :528-539
-                P->status = kAllLeft; P->split = hi; P->wl = Ops::to_bits(S);
-                P->cnt_left = n; P->kstar = -1;
-                return;
-            }
-            hi = t; prev = cnt;                           // :540-544
-            left = true;
-        } else {
-            const float mnear = key_f32(sm[q + 1]);       // smallest c >= t
-            const float d = mnear - t;                    // nearest_distance
-            const double ideal = Ops::f64(S, wrap32) / 2.0;
-            const double imb = fabs((Ops::f64(wl, wrap32) - ideal) / ideal);
-            if (cnt == prev || hi <= t + d || imb <= tol) {   // :552-563
-                P->status = kSplitAtT; P->split = t; P->wl = Ops::to_bits(wl);
-                P->cnt_left = cnt; P->kstar = q;
-                P->counts_ready = counts_ready;   // per-tile left counts already known
-                return;
-            }
-            prev = cnt;                                   // :564
-            if (Ops::lt(wl, Ops::sub(S, wl), wrap32)) { lo = t; left = false; }
-            else { hi = t; left = true; }
-        }
-        step >>= 1;
-        if (step == 0) break;
-        q = left ? q - step : q + step;
-    }
-    P->blo = lo; P->bhi = hi; P->prev = prev;             // next pass continues here
-    atomicAdd(n_searching, 1u);
-    if (unfinished) {
-        P->status = kAllLeft;
-        P->split = hi;
-        P->cnt_left = n;
-        P->wl = P->sum;
-        P->kstar = -1;
-        atomicOr(unfinished, 1u);
-    }
+    if (decide_replay<Acc>(P, pc, pw, sm, nb, wrap32, counts_ready, unfinished))
+        atomicAdd(&n_searching[xcd_copy()], 1u);
 }
 
+// The same decision, one wavefront per part, for passes of at most 64 bins
+// (every first decision; the deep levels of large batches, whose bin buffers
+// hold fewer bins per part): four parts per workgroup, each wavefront's
+// lanes one bin each, no workgroup barrier.
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_decide_wave(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
+              Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
+              uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
+              const uint32_t *gate, uint32_t *unfinished, BinCopies<Acc> xc) {
+    if (gate && gate_count(gate) == 0) return;   // no part left searching after the previous decision
+    __shared__ uint64_t pc_all[kBlock / 64][64];
+    __shared__ Acc pw_all[kBlock / 64][64];
+    __shared__ uint32_t sm_all[kBlock / 64][65];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t s = blockIdx.x * (kBlock / 64) + wv;
+    if (s >= *n_parts) return;
+    Part *P = &parts[s];
+    if (P->status != kSearching) return;
+    uint32_t cn = 0, mn = 0xffffffffu;
+    Acc wt = (Acc)0;
+    if (lane < nb) {   // the bin and its copies read, and left cleared for the next pass
+        const uint64_t g = (uint64_t)s * nb + lane;
+        cn = bcnt[g];
+        wt = bw[g];
+        mn = bmin[g];
+        bcnt[g] = 0; bw[g] = (Acc)0; bmin[g] = 0xffffffffu;
+        if (xc.cnt) {
+#pragma unroll
+            for (int cp = 1; cp < kCopies; cp++) {
+                const uint64_t x = copy_index(xc, cp, s, lane);
+                cn += xc.cnt[x]; wt += xc.w[x]; mn = min(mn, xc.mn[x]);
+            }
+#pragma unroll
+            for (int cp = 1; cp < kCopies; cp++) {
+                const uint64_t x = copy_index(xc, cp, s, lane);
+                xc.cnt[x] = 0; xc.w[x] = (Acc)0; xc.mn[x] = 0xffffffffu;
+            }
+        }
+    }
+    const uint64_t ic = wave_incl_add((uint64_t)cn);
+    const Acc iw = wave_incl_add(wt);
+    uint32_t suf = mn;   // the smallest key of the bins from this lane's on
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_down(suf, o, 64);
+        if (lane + o < 64) suf = v < suf ? v : suf;
+    }
+    if (lane < nb) {
+        pc_all[wv][lane] = ic;
+        pw_all[wv][lane] = iw;
+        sm_all[wv][lane] = suf;
+    }
+    if (lane == 0) sm_all[wv][nb] = 0xffffffffu;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+    atomicAdd(&points_swept[xcd_copy()], (unsigned long long)P->count);   // swept by the last pass
+    if (decide_replay<Acc>(P, pc_all[wv], pw_all[wv], sm_all[wv], nb, wrap32, counts_ready, unfinished))
+        atomicAdd(&n_searching[xcd_copy()], 1u);
+}
+
+// The same decision, one lane per part, for passes of at most kFirstBins
+// bins (every first decision): each lane gathers its part's bins, and a
+// wavefront counts its parts still searching and their points with one
+// atomic each.
+template <typename Acc>
+__global__ void __launch_bounds__(kBlock)
+k_decide_lanes(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
+               Acc *__restrict__ bw, uint32_t *__restrict__ bmin, int nb, int wrap32,
+               uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
+               const uint32_t *gate, uint32_t *unfinished, BinCopies<Acc> xc) {
+    if (gate && gate_count(gate) == 0) return;   // no part left searching after the previous decision
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    bool searching = false;
+    uint64_t swept = 0;
+    if (s < *n_parts && parts[s].status == kSearching) {
+        Part *P = &parts[s];
+        uint64_t pc[kFirstBins];
+        Acc pw[kFirstBins];
+        uint32_t mk[kFirstBins], sm[kFirstBins + 1];
+        uint64_t c = 0;
+        Acc w = (Acc)0;
+#pragma unroll
+        for (int j = 0; j < kFirstBins; j++) {
+            uint32_t cn = 0, mn = 0xffffffffu;
+            Acc wt = (Acc)0;
+            if (j < nb) {   // the bin and its copies read, and left cleared for the next pass
+                const uint64_t g = (uint64_t)s * nb + j;
+                cn = bcnt[g];
+                wt = bw[g];
+                mn = bmin[g];
+                bcnt[g] = 0; bw[g] = (Acc)0; bmin[g] = 0xffffffffu;
+                if (xc.cnt) {
+                    for (int cp = 1; cp < kCopies; cp++) {
+                        const uint64_t x = copy_index(xc, cp, s, j);
+                        cn += xc.cnt[x]; wt += xc.w[x]; mn = min(mn, xc.mn[x]);
+                        xc.cnt[x] = 0; xc.w[x] = (Acc)0; xc.mn[x] = 0xffffffffu;
+                    }
+                }
+            }
+            c += cn;
+            w += wt;
+            pc[j] = c;
+            pw[j] = w;
+            mk[j] = mn;
+        }
+        sm[kFirstBins] = 0xffffffffu;
+#pragma unroll
+        for (int j = kFirstBins - 1; j >= 0; j--) sm[j] = min(mk[j], sm[j + 1]);
+        swept = P->count;
+        searching = decide_replay<Acc>(P, pc, pw, sm, nb, wrap32, counts_ready, unfinished);
+    }
+    const uint64_t sw = wave_sum(swept);
+    const uint32_t ns = (uint32_t)__popcll(__ballot(searching));
+    if ((threadIdx.x & 63) == 0) {
+        if (sw) atomicAdd(&points_swept[xcd_copy()], (unsigned long long)sw);
+        if (ns) atomicAdd(&n_searching[xcd_copy()], ns);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan of uint64 (3 kernels).  n is read on the device;
@@ -1075,7 +1210,7 @@ k_children_write(Part *__restrict__ parts, const uint32_t *n_parts, int dim,
     const Part P = parts[s];
     if (P.status == kDone) return;
     const bool fwd_in = fsrc.cnt != nullptr && forwarded(P);
-    if (fwd_in && threadIdx.x == 0 && P.depth + 1 < P.levels) atomicAdd(n_forwarded, 1u);
+    if (fwd_in && threadIdx.x == 0 && P.depth + 1 < P.levels) atomicAdd(&n_forwarded[xcd_copy()], 1u);
     // A part at its instance's last level has leaves, not parts (:589-601).
     const bool inner = P.depth + 1 < P.levels;
     const uint64_t nl = inner ? P.cnt_left : 0, nr = inner ? P.count - P.cnt_left : 0;

@@ -127,7 +127,7 @@ k_child_first(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
         }
         __syncthreads();
     }
-    if (tid == 0 && read) atomicAdd(points_read, (unsigned long long)read);
+    if (tid == 0 && read) atomicAdd(&points_read[xcd_copy()], (unsigned long long)read);
     if (cur != 0xffffffffu) flush_child_bins(r, sh, ch0, ch1, bcnt, bw, bmin);
 }
 
@@ -271,7 +271,7 @@ k_hist_virtual(const Part *__restrict__ parts, const Tile *__restrict__ tiles, c
                const float *__restrict__ c2, const WS *__restrict__ w, int nb, uint32_t *__restrict__ bcnt,
                Acc *__restrict__ bw, uint32_t *__restrict__ bmin, const uint32_t *gate,
                uint16_t *__restrict__ tile_bins2, BinCopies<Acc> xc, uint32_t chunk) {
-    if (gate && *gate == 0) return;
+    if (gate && gate_count(gate) == 0) return;
     __shared__ float cand[2][kMaxBins];
     __shared__ uint32_t scnt[2][kMaxBins];
     __shared__ Acc sw[2][kMaxBins];
