@@ -543,7 +543,8 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
         cx.stats.bins = std::max<uint32_t>(cx.stats.bins, (uint32_t)nb);
         // a decision over nbd bins: a lane per part up to kFirstBins bins
         // without bin copies (levels of many parts), a wavefront per part up
-        // to 64 bins, else a workgroup
+        // to 64 bins, else a workgroup (measured faster than a wavefront with
+        // four bins per lane on 2^24 Gaussian points, no slower on the batch)
         auto decide = [&](int nbd, uint32_t *searching, unsigned long long *swept, uint32_t counts,
                           const uint32_t *gate, uint32_t *unfin) {
             if (nbd <= kFirstBins && !xc.cnt)

@@ -829,10 +829,9 @@ k_decide(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict
         atomicAdd(&n_searching[xcd_copy()], 1u);
 }
 
-// The same decision, one wavefront per part, for passes of at most 64 bins
-// (every first decision; the deep levels of large batches, whose bin buffers
-// hold fewer bins per part): four parts per workgroup, each wavefront's
-// lanes one bin each, no workgroup barrier.
+// The same decision, one wavefront per part: four parts per workgroup,
+// each lane up to four consecutive bins, no workgroup barrier (the engine
+// takes it for passes of at most 64 bins).
 template <typename Acc>
 __global__ void __launch_bounds__(kBlock)
 k_decide_wave(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__restrict__ bcnt,
@@ -840,47 +839,82 @@ k_decide_wave(Part *__restrict__ parts, const uint32_t *n_parts, uint32_t *__res
               uint32_t *n_searching, unsigned long long *points_swept, uint32_t counts_ready,
               const uint32_t *gate, uint32_t *unfinished, BinCopies<Acc> xc) {
     if (gate && gate_count(gate) == 0) return;   // no part left searching after the previous decision
-    __shared__ uint64_t pc_all[kBlock / 64][64];
-    __shared__ Acc pw_all[kBlock / 64][64];
-    __shared__ uint32_t sm_all[kBlock / 64][65];
+    static_assert(kMaxBins <= 4 * 64, "at most four bins per lane");
+    __shared__ uint64_t pc_all[kBlock / 64][kMaxBins];
+    __shared__ Acc pw_all[kBlock / 64][kMaxBins];
+    __shared__ uint32_t sm_all[kBlock / 64][kMaxBins + 1];
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t s = blockIdx.x * (kBlock / 64) + wv;
     if (s >= *n_parts) return;
     Part *P = &parts[s];
     if (P->status != kSearching) return;
-    uint32_t cn = 0, mn = 0xffffffffu;
-    Acc wt = (Acc)0;
-    if (lane < nb) {   // the bin and its copies read, and left cleared for the next pass
-        const uint64_t g = (uint64_t)s * nb + lane;
-        cn = bcnt[g];
-        wt = bw[g];
-        mn = bmin[g];
-        bcnt[g] = 0; bw[g] = (Acc)0; bmin[g] = 0xffffffffu;
-        if (xc.cnt) {
+    // lane l holds bins l * per .. l * per + per - 1 (nb is a power of two)
+    const int per = nb > 64 ? nb / 64 : 1;
+    const int j0 = lane * per;
+    uint64_t c[4];
+    Acc w[4];
+    uint32_t m[4];
 #pragma unroll
-            for (int cp = 1; cp < kCopies; cp++) {
-                const uint64_t x = copy_index(xc, cp, s, lane);
-                cn += xc.cnt[x]; wt += xc.w[x]; mn = min(mn, xc.mn[x]);
-            }
+    for (int k = 0; k < 4; k++) {
+        c[k] = 0; w[k] = (Acc)0; m[k] = 0xffffffffu;
+        if (k < per && j0 + k < nb) {   // the bin and its copies read, and left cleared for the next pass
+            const int j = j0 + k;
+            const uint64_t g = (uint64_t)s * nb + j;
+            uint32_t cn = bcnt[g], mn = bmin[g];
+            Acc wt = bw[g];
+            bcnt[g] = 0; bw[g] = (Acc)0; bmin[g] = 0xffffffffu;
+            if (xc.cnt) {
 #pragma unroll
-            for (int cp = 1; cp < kCopies; cp++) {
-                const uint64_t x = copy_index(xc, cp, s, lane);
-                xc.cnt[x] = 0; xc.w[x] = (Acc)0; xc.mn[x] = 0xffffffffu;
+                for (int cp = 1; cp < kCopies; cp++) {
+                    const uint64_t x = copy_index(xc, cp, s, j);
+                    cn += xc.cnt[x]; wt += xc.w[x]; mn = min(mn, xc.mn[x]);
+                }
+#pragma unroll
+                for (int cp = 1; cp < kCopies; cp++) {
+                    const uint64_t x = copy_index(xc, cp, s, j);
+                    xc.cnt[x] = 0; xc.w[x] = (Acc)0; xc.mn[x] = 0xffffffffu;
+                }
             }
+            c[k] = cn; w[k] = wt; m[k] = mn;
         }
     }
-    const uint64_t ic = wave_incl_add((uint64_t)cn);
-    const Acc iw = wave_incl_add(wt);
-    uint32_t suf = mn;   // the smallest key of the bins from this lane's on
+    uint64_t lc = 0;
+    Acc lw = (Acc)0;
+    uint32_t lm = 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        lc += c[k];
+        lw += w[k];
+        lm = m[k] < lm ? m[k] : lm;
+    }
+    const uint64_t ic = wave_incl_add(lc);
+    const Acc iw = wave_incl_add(lw);
+    uint32_t suf = lm;   // the smallest key of the bins from this lane's on
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t v = __shfl_down(suf, o, 64);
         if (lane + o < 64) suf = v < suf ? v : suf;
     }
-    if (lane < nb) {
-        pc_all[wv][lane] = ic;
-        pw_all[wv][lane] = iw;
-        sm_all[wv][lane] = suf;
+    uint32_t after = __shfl_down(suf, 1, 64);   // the smallest key of the later lanes' bins
+    if (lane == 63) after = 0xffffffffu;
+    if (j0 < nb) {
+        uint64_t rc = ic - lc;
+        Acc rw = iw - lw;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (k < per) {
+                rc += c[k];
+                rw += w[k];
+                pc_all[wv][j0 + k] = rc;
+                pw_all[wv][j0 + k] = rw;
+            }
+        uint32_t r = after;
+#pragma unroll
+        for (int k = 3; k >= 0; k--)
+            if (k < per) {
+                r = m[k] < r ? m[k] : r;
+                sm_all[wv][j0 + k] = r;
+            }
     }
     if (lane == 0) sm_all[wv][nb] = 0xffffffffu;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
