@@ -158,6 +158,7 @@ struct Counters {
     unsigned long long swept[kCopies];          // points swept by standalone passes
     unsigned long long swept_fused[kCopies];    // points binned inside partitions
     unsigned long long first_read[kCopies];     // points read by the children's first passes
+    unsigned long long move_runs[2 * kCopies];  // destinations of the moved tiles, then the moved tiles
     Prologue pro[kCopies];                      // the prologue's weight flags
     uint32_t forwarded[kCopies];                // parts whose children's first pass the move forwarded
     uint32_t unfinished;                        // a speculative search did not end in its passes
@@ -187,6 +188,10 @@ static bool finish_levels(Context &cx) {
                      (double)sum_copies(h->first_read) * (8.0 + P.ws_bytes);
     cx.stats.first_read = sum_copies(h->first_read);
     cx.stats.forwarded = sum_copies(h->forwarded);
+    {   // moved tiles going to at most 2.5 destinations on average (of four)
+        const unsigned long long r = sum_copies(h->move_runs), t = sum_copies(h->move_runs + kCopies);
+        if (t) cx.remember(P.shape).moves_coherent = 2 * r <= 5 * t ? 1 : 0;
+    }
     cx.bins_clean = true;
     if (P.spec == 0) return false;
     // passes per level: the first decision plus every pass whose gate was open
@@ -765,7 +770,7 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                 const unsigned g = grid_for(tl_ub, kBlock);   // one thread per tile
                 k_tile_count4<D><<<g, kBlock, 0, st>>>(pa, tl, ntl, pb, cl[0], cl[1], cl[2],
                                                       pl, pb2, tile_bins2, q4[0],
-                                                      q4[1], q4[2], q4[3]);
+                                                      q4[1], q4[2], q4[3], ctr->move_runs);
             });
             scan_u64(cx, st, q4[0], nullptr, 4 * tl_ub, nullptr);
             const int csn = csl == 1 ? 2 : 1, wsn = wsl ^ 1;
@@ -787,8 +792,10 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
                                                      skip_axis);
                     });
                 };
-                // the stores are non-temporal for moves larger than the switch
-                const bool nt = (2.0 * payload - (skip_axis < 3 ? 4.0 : 0.0)) * (double)n > cx.move_nt_bytes;
+                // the stores are non-temporal for moves larger than the switch whose
+                // tiles went to few destinations in the shape's last call
+                const bool nt = (2.0 * payload - (skip_axis < 3 ? 4.0 : 0.0)) * (double)n > cx.move_nt_bytes &&
+                                seen && seen->moves_coherent == 1;
                 if (fwd_out) nt ? move(k_partition4<D, WS, Acc, true, true>) : move(k_partition4<D, WS, Acc, true, false>);
                 else nt ? move(k_partition4<D, WS, Acc, false, true>) : move(k_partition4<D, WS, Acc, false, false>);
             }

@@ -413,9 +413,13 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
               const Part *__restrict__ children, const float *__restrict__ c0, const float *__restrict__ c1,
               const float *__restrict__ c2, const uint64_t *__restrict__ tile_left,
               const uint32_t *__restrict__ below2, const uint16_t *__restrict__ tile_bins2, uint64_t *__restrict__ q0,
-              uint64_t *__restrict__ q1, uint64_t *__restrict__ q2, uint64_t *__restrict__ q3) {
+              uint64_t *__restrict__ q1, uint64_t *__restrict__ q2, uint64_t *__restrict__ q3,
+              unsigned long long *__restrict__ runs) {
+    // runs: per XCD, the destinations (non-empty slots) of the moved tiles,
+    // then the moved tiles (the host's measure of how scattered a move is)
     const uint32_t nt = *n_tiles;
     const int lane = lane_id();
+    uint32_t n_dests = 0, n_moved = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < nt; base += gridDim.x * kBlock) {
         const uint32_t t = base + threadIdx.x;
         uint64_t out[4] = {0, 0, 0, 0};
@@ -473,6 +477,28 @@ k_tile_count4(const Part *__restrict__ parts, const Tile *__restrict__ tiles, co
             q1[t] = out[1];
             q2[t] = out[2];
             q3[t] = out[3];
+        }
+        const uint32_t dests = (out[0] != 0) + (out[1] != 0) + (out[2] != 0) + (out[3] != 0);
+        n_dests += dests;
+        n_moved += dests ? 1u : 0u;
+    }
+    // one update of the counters per workgroup
+    __shared__ uint32_t s_runs[2][kBlock / 64];
+    const uint32_t rsum = wave_sum(n_dests), tsum = wave_sum(n_moved);
+    if (lane == 0) {
+        s_runs[0][threadIdx.x >> 6] = rsum;
+        s_runs[1][threadIdx.x >> 6] = tsum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0, m = 0;
+        for (int k = 0; k < kBlock / 64; k++) {
+            r += s_runs[0][k];
+            m += s_runs[1][k];
+        }
+        if (m) {
+            atomicAdd(&runs[xcd_copy()], (unsigned long long)r);
+            atomicAdd(&runs[kCopies + xcd_copy()], (unsigned long long)m);
         }
     }
 }
