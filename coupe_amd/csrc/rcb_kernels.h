@@ -721,7 +721,7 @@ __device__ bool decide_replay(Part *P, const PC &pc, const PW &pw, const SM &sm,
 
 // ---------------------------------------------------------------------------
 // Decision replay: one workgroup per searching part (passes of more than 64
-// bins; k_decide_wave takes the others).
+// bins; k_decide_wave and k_decide_lanes take the others).
 // ---------------------------------------------------------------------------
 template <typename Acc>
 __global__ void __launch_bounds__(kBlock)
