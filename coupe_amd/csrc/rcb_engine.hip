@@ -253,7 +253,6 @@ void scan_u64(Context &cx, hipStream_t st, uint64_t *data, const uint32_t *n_dev
     });
 }
 
-// Bins per part of the standalone LDS passes, bounded by the bin buffers.
 // Search passes to queue at a level whose search took `decisions` decisions
 // (its first one and one per pass) in the context's last call of the same
 // shape: none when the first decision ended every search there (the split
@@ -264,6 +263,7 @@ static int queued_passes(int spec, uint32_t decisions) {
     return std::min(spec, (int)decisions);
 }
 
+// Bins per part of the standalone LDS passes, bounded by the bin buffers.
 static int choose_bins(uint64_t parts_ub, int max_bins) {
     const uint64_t cap = 1ull << 23;   // entries of each bin array
     int nb = max_bins;
