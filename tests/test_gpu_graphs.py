@@ -234,3 +234,30 @@ def test_level_with_no_queued_pass_reruns_when_new_data_needs_one(oracle_mod):
                 assert st["reruns"] == 1, st   # no pass was queued where the grid needed none
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("nt_bytes", ["0", "1e15"])
+def test_move_store_modes(oracle_mod, monkeypatch, nt_bytes):
+    """The move stores non-temporally when its size passes the switch and the
+    shape's last call moved tiles to few destinations (grid order), else
+    through the caches: both forms give the oracle's ids, on data in grid
+    order and in random order, on the first and the later calls of a shape."""
+    torch = pytest.importorskip("torch")
+    monkeypatch.setenv("COUPE_HIP_MOVE_NT_BYTES", nt_bytes)
+    from coupe_amd import workloads
+    ctx = coupe_amd.DeviceContext(0)
+    try:
+        pts, w = workloads.jgrid(48, 2, True)
+        perm = np.random.default_rng(9).permutation(len(w))
+        for name, p, ww in (("grid order", pts, w), ("random order", pts[perm], w[perm])):
+            exp = _ids(oracle_mod, p, ww, 10, 0.01)
+            d_p, d_w = torch.from_numpy(np.ascontiguousarray(p)).cuda(), torch.from_numpy(ww).cuda()
+            out = torch.empty(len(ww), dtype=torch.int64, device="cuda")
+            for call in range(3):
+                out.fill_(-1)
+                coupe_amd.Rcb(iter_count=10, tolerance=0.01).partition(out, (d_p, d_w), context=ctx)
+                torch.cuda.synchronize()
+                say(f"store mode switch {nt_bytes}, {name}, call {call}")
+                assert np.array_equal(out.cpu().numpy().astype(np.uint64), exp), f"{name} call {call}"
+    finally:
+        ctx.close()
