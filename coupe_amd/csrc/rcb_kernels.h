@@ -109,10 +109,10 @@ struct WeightScan {
                 su += (uint64_t)(int64_t)w;
                 const bool fits = w >= -2147483648.0 && w <= 2147483647.0;
                 not32 |= !fits;
-                if (w32) w32[i] = fits ? (int32_t)w : 0;
+                if (w32) __builtin_nontemporal_store(fits ? (int32_t)w : 0, &w32[i]);
                 const bool f16 = w >= 0.0 && w <= 65535.0;
                 not16 |= !f16;
-                if (w16) w16[i] = f16 ? (uint16_t)w : (uint16_t)0;
+                if (w16) __builtin_nontemporal_store(f16 ? (uint16_t)w : (uint16_t)0, &w16[i]);
             } else {
                 not32 = 1;
                 not16 = 1;
@@ -126,10 +126,10 @@ struct WeightScan {
             ma = aw > ma ? aw : ma;
             const bool fits = wi >= INT32_MIN && wi <= INT32_MAX;
             not32 |= !fits;
-            if (w32) w32[i] = (int32_t)wi;
+            if (w32) __builtin_nontemporal_store((int32_t)wi, &w32[i]);
             const bool f16 = wi >= 0 && wi <= 65535;
             not16 |= !f16;
-            if (w16) w16[i] = (uint16_t)wi;
+            if (w16) __builtin_nontemporal_store((uint16_t)wi, &w16[i]);
         }
     }
 
@@ -214,15 +214,18 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
             const double *src = ipts + (base - ibeg) * D;
             const WIN *wsrc = iwin + (base - ibeg);
             // every load of the round issued back to back: the AoS doubles
-            // (clamped to the round's last one) and the points' weights
+            // (clamped to the round's last one) and the points' weights.
+            // The caller's arrays are read once and the working arrays
+            // written once, each far larger than the caches: non-temporal
+            // loads and stores (2^27 points 3D: 1.23 -> 1.20 ms)
             constexpr int kDv = kProPoints * D / kBlock;
             const uint32_t nd = cnt * D;
             double dv[kDv];
             WIN wv[4];
 #pragma unroll
-            for (int j = 0; j < kDv; j++) dv[j] = src[min((uint32_t)(j * kBlock + tid), nd - 1u)];
+            for (int j = 0; j < kDv; j++) dv[j] = __builtin_nontemporal_load(&src[min((uint32_t)(j * kBlock + tid), nd - 1u)]);
 #pragma unroll
-            for (int k = 0; k < 4; k++) wv[k] = wsrc[min((uint32_t)(k * kBlock + tid), cnt - 1u)];
+            for (int k = 0; k < 4; k++) wv[k] = __builtin_nontemporal_load(&wsrc[min((uint32_t)(k * kBlock + tid), cnt - 1u)]);
 #pragma unroll
             for (int j = 0; j < kDv; j++)
                 if ((uint32_t)(j * kBlock + tid) < nd) sp[j * kBlock + tid] = dv[j];
@@ -252,7 +255,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                         if (v < mn[a]) mn[a] = v;
                         if (mx[a] < v) mx[a] = v;
                         float *cs = a == 0 ? c0 : (a == 1 ? c1 : c2);
-                        cs[i] = (float)v;
+                        __builtin_nontemporal_store((float)v, &cs[i]);
                     }
                     ws.add(wv[k], i, w32, w16);
                 }
