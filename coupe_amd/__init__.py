@@ -603,7 +603,20 @@ class Grid:
         if len(weights) != n or len(partition) != n:
             raise ValueError(f"a {self.size} grid has {n} cells")
         device_io = _is_device_tensor(weights)
-        dev = weights.device if device_io else torch.device("cuda", 0)
+        device_out = _is_device_tensor(partition)
+        # the ids are written as 64-bit integers straight into a device
+        # partition, so it must be one, on the device the weights are on
+        if device_out and (partition.dtype not in (torch.int64, torch.uint64) or not partition.is_contiguous()):
+            raise TypeError("partition must be a contiguous 64-bit integer tensor")
+        if device_io and device_out:
+            index = _common_device(partition=partition, weights=weights)
+        elif device_io or device_out:
+            index = (weights if device_io else partition).device.index or 0
+        else:
+            index = context.device if context is not None else 0
+        if context is not None and context.device != index:
+            raise TypeError(f"the context is on device {context.device}, the grid's tensors on device {index}")
+        dev = torch.device("cuda", index)
         if device_io:
             w = weights.contiguous()
             wtype = _weight_type(w.dtype)
@@ -611,15 +624,15 @@ class Grid:
             wn = np.ascontiguousarray(weights)
             wtype = _weight_type(wn.dtype)
             w = torch.from_numpy(wn).to(dev)
-        out = partition if _is_device_tensor(partition) else torch.empty(n, dtype=torch.int64, device=dev)
-        ctx = context or _context_for(dev.index or 0)
+        out = partition if device_out else torch.empty(n, dtype=torch.int64, device=dev)
+        ctx = context or _context_for(index)
         st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         sz = (C.c_size_t * 3)(*(list(self.size) + [1] * (3 - len(self.size))))
         err = _native.lib().coupe_hip_grid_rcb(ctx.handle, C.c_void_p(st), len(self.size), sz, wtype, w.data_ptr(),
                                                 int(iter_count), int(threads), out.data_ptr())
         if err != _native.OK:
             raise CoupeError(err)
-        if not _is_device_tensor(partition):
+        if not device_out:
             torch.cuda.synchronize(dev)
             partition[:] = out.cpu().numpy().astype(partition.dtype, copy=False)
         return None

@@ -1318,7 +1318,7 @@ bool grid_run_sat(Context &cx, hipStream_t st, size_t dim, const uint64_t *size,
     // Every partial sum is an integer of magnitude at most sum |w|; the f64
     // sum of the magnitudes, over at most 2^36 terms, is within a factor
     // 1 + 2^-17 of the exact one, so this bound keeps the exact sum <= 2^53.
-    return bad == 0 && abs_sum <= 0x1p53 * (1.0 - 0x1p-15);
+    return bad == 0 && cells <= (1ull << 36) && abs_sum <= 0x1p53 * (1.0 - 0x1p-15);
 }
 
 }  // namespace
@@ -1329,7 +1329,9 @@ enum coupe_err grid_rcb(Context &cx, hipStream_t st, size_t dim, const uint64_t 
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
     for (size_t k = 0; k < dim; k++)
         if (size[k] == 0 || size[k] > 0xffffffffull) return COUPE_ERR_CRASH;   // NonZeroUsize, u32 positions
-    const uint64_t cells = size[0] * size[1] * (dim == 3 ? size[2] : 1);
+    uint64_t cells = size[0];
+    for (size_t k = 1; k < dim; k++)
+        if (__builtin_mul_overflow(cells, size[k], &cells)) return COUPE_ERR_CRASH;   // no memory holds such a grid
     if (iter_count > 62) return COUPE_ERR_CRASH;
     const uint32_t T = std::min<uint32_t>(threads ? threads : default_threads(), kGMaxThreads);
     memset(&cx.stats, 0, sizeof cx.stats);

@@ -182,7 +182,9 @@ enum coupe_err coupe_hip_last_rib(coupe_hip_ctx *ctx, struct coupe_hip_rib_info 
  * most significant first.  `threads` is the rayon pool size whose chunking
  * coupe's weighted median follows (0: RAYON_NUM_THREADS, else the number of
  * logical CPUs, as rayon's global pool).  Integer weights give coupe's ids
- * bit for bit.
+ * bit for bit.  An axis of 0 or more than 2^32 - 1 cells, a cell count past
+ * 64 bits or iter_count above 62 returns COUPE_ERR_CRASH before any device
+ * work.
  * Replaces: Grid::new_2d / new_3d + Grid::rcb, coupe/src/cartesian/mod.rs:127-180,
  * coupe/src/cartesian/rcb.rs:52-281.
  */

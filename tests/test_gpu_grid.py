@@ -147,3 +147,21 @@ def test_errors():
         coupe_amd.Grid([0, 4])
     with pytest.raises(ValueError):
         coupe_amd.Grid([4, 4]).rcb(np.zeros(15, dtype=np.uint64), np.ones(15), 2)
+
+
+def test_sizes_rejected_before_device_work():
+    """An axis of 0 or past u32, a cell count past 64 bits and iter_count
+    above 62 are refused before any device work (null weights and ids are
+    never touched)."""
+    import ctypes as C
+
+    from coupe_amd import _native
+
+    ctx = coupe_amd._context_for(0)
+    lib = _native.lib()
+    for dim, size, iters in [(3, (2**32 - 1, 2**32 - 1, 2**32 - 1), 4), (2, (0, 4, 1), 2),
+                             (2, (2**32, 4, 1), 2), (2, (4, 4, 1), 63)]:
+        say(f"sizes {size[:dim]} iter_count {iters}")
+        sz = (C.c_size_t * 3)(*size)
+        err = lib.coupe_hip_grid_rcb(ctx.handle, None, dim, sz, _native.COUPE_DOUBLE, None, iters, 0, None)
+        assert err == _native.CRASH
