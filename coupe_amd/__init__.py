@@ -82,6 +82,35 @@ def _common_device(**tensors) -> int:
     return devs.pop()
 
 
+def _check_out_ids(part_ids, device=None) -> bool:
+    """Checks a buffer the ids are written into as 64-bit integers: a
+    contiguous int64/uint64 GPU tensor (on ``device`` when given) or numpy
+    array.  True for a GPU tensor; TypeError otherwise (a narrower or strided
+    buffer would be written past its end)."""
+    import torch
+
+    if _is_device_tensor(part_ids):
+        if part_ids.dtype not in (torch.int64, torch.uint64) or not part_ids.is_contiguous():
+            raise TypeError("part_ids must be a contiguous 64-bit integer tensor")
+        if device is not None and (part_ids.device.index or 0) != device:
+            raise TypeError(f"part_ids is on device {part_ids.device.index or 0}, the work runs on device {device}")
+        return True
+    if (not isinstance(part_ids, np.ndarray) or part_ids.dtype not in (np.int64, np.uint64)
+            or not part_ids.flags.c_contiguous):
+        raise TypeError("part_ids must be a contiguous 64-bit integer numpy array or GPU tensor")
+    return False
+
+
+def _check_device_points(points, device):
+    """Device coordinates are read as contiguous float64 on ``device``."""
+    import torch
+
+    if points.dtype != torch.float64 or not points.is_contiguous():
+        raise TypeError("points must be a contiguous float64 tensor")
+    if (points.device.index or 0) != device:
+        raise TypeError(f"points are on device {points.device.index or 0}, the work runs on device {device}")
+
+
 def _weight_type(dtype) -> int:
     key = np.dtype(str(dtype).replace("torch.", ""))
     if key not in _WTYPES:
@@ -268,6 +297,7 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
             specs.append((int(n), int(iters), float(tol)))
     total = sum(n for n, _, _ in specs)
     dev = _common_device(part_ids=part_ids, points=points, weights=weights)
+    _check_out_ids(part_ids, dev)
     for name, t in (("part_ids", part_ids), ("points", points), ("weights", weights)):
         if t.shape[0] != total:
             raise InputLenMismatch(total, t.shape[0])
@@ -291,6 +321,30 @@ def partition_batch(part_ids, points, weights, instances, *, stream=None, contex
     return None
 
 
+def _mixed_instances(part_ids, points, weights, instances):
+    """The instance table of a batch of both dimensions and the tensors'
+    device, after the checks every such call needs (lengths, dtypes,
+    layout, one device)."""
+    import torch
+
+    specs = [(int(n), int(d), int(it), float(tol)) for n, d, it, tol in instances]
+    total = sum(n for n, _, _, _ in specs)
+    coords = sum(n * d for n, d, _, _ in specs)
+    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
+    _check_out_ids(part_ids, dev)
+    for name, t, want in (("part_ids", part_ids, total), ("weights", weights, total)):
+        if t.shape[0] != want:
+            raise InputLenMismatch(want, t.shape[0])
+        if not t.is_contiguous():
+            raise TypeError(f"{name} must be contiguous")
+    if points.dtype != torch.float64 or not points.is_contiguous() or points.numel() != coords:
+        raise TypeError(f"points must be a contiguous float64 tensor of {coords} coordinates")
+    arr = (_native.InstanceND * max(len(specs), 1))()
+    for q, (n, d, it, tol) in enumerate(specs):
+        arr[q].n, arr[q].dimension, arr[q].iter_count, arr[q].tolerance = n, d, it, tol
+    return arr, len(specs), dev
+
+
 def partition_batch_mixed(part_ids, points, weights, instances, *, stream=None, context: DeviceContext = None):
     """Batched RCB of instances of dimension 2 and 3 in one call
     (coupe_hip_rcb_batch_mixed, include/coupe_hip.h).
@@ -303,27 +357,14 @@ def partition_batch_mixed(part_ids, points, weights, instances, *, stream=None, 
     """
     import torch
 
-    specs = [(int(n), int(d), int(it), float(tol)) for n, d, it, tol in instances]
-    total = sum(n for n, _, _, _ in specs)
-    coords = sum(n * d for n, d, _, _ in specs)
-    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
-    for name, t, want in (("part_ids", part_ids, total), ("weights", weights, total)):
-        if t.shape[0] != want:
-            raise InputLenMismatch(want, t.shape[0])
-        if not t.is_contiguous():
-            raise TypeError(f"{name} must be contiguous")
-    if points.dtype != torch.float64 or not points.is_contiguous() or points.numel() != coords:
-        raise TypeError(f"points must be a contiguous float64 tensor of {coords} coordinates")
-    arr = (_native.InstanceND * max(len(specs), 1))()
-    for q, (n, d, it, tol) in enumerate(specs):
-        arr[q].n, arr[q].dimension, arr[q].iter_count, arr[q].tolerance = n, d, it, tol
+    arr, count, dev = _mixed_instances(part_ids, points, weights, instances)
     ctx = context or _context_for(dev)
     if ctx.device != dev:
         raise TypeError(f"the context is on device {ctx.device}, the tensors on device {dev}")
     st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
     err = _native.lib().coupe_hip_rcb_batch_mixed(ctx.handle, C.c_void_p(st), part_ids.data_ptr(),
                                                   points.data_ptr(), _weight_type(weights.dtype),
-                                                  weights.data_ptr(), arr, len(specs))
+                                                  weights.data_ptr(), arr, count)
     if err != _native.OK:
         raise CoupeError(err)
     return None
@@ -337,16 +378,14 @@ def autotune(part_ids, points, weights, instances, *, trials: int = 3, stream=No
     Returns them with the fastest mean call time (``best_ms``)."""
     import torch
 
-    specs = [(int(n), int(d), int(it), float(tol)) for n, d, it, tol in instances]
-    dev = _common_device(part_ids=part_ids, points=points, weights=weights)
-    arr = (_native.InstanceND * max(len(specs), 1))()
-    for q, (n, d, it, tol) in enumerate(specs):
-        arr[q].n, arr[q].dimension, arr[q].iter_count, arr[q].tolerance = n, d, it, tol
+    arr, count, dev = _mixed_instances(part_ids, points, weights, instances)
     ctx = context or _context_for(dev)
+    if ctx.device != dev:
+        raise TypeError(f"the context is on device {ctx.device}, the tensors on device {dev}")
     st = stream if stream is not None else torch.cuda.current_stream(points.device).cuda_stream
     chosen, ms = _native.Tuning(), C.c_double()
     err = _native.lib().coupe_hip_autotune(ctx.handle, C.c_void_p(st), part_ids.data_ptr(), points.data_ptr(),
-                                           _weight_type(weights.dtype), weights.data_ptr(), arr, len(specs),
+                                           _weight_type(weights.dtype), weights.data_ptr(), arr, count,
                                            int(trials), C.byref(chosen), C.byref(ms))
     if err != _native.OK:
         raise CoupeError(err)
@@ -391,11 +430,13 @@ class Server:
             raise TypeError("points and weights must both be host arrays or both device tensors")
         if dev_in:
             _common_device(points=points, weights=weights)
-            if not points.is_contiguous() or not weights.is_contiguous():
-                raise TypeError("points and weights must be contiguous")
+            _check_device_points(points, self.device)
+            if not weights.is_contiguous():
+                raise TypeError("weights must be contiguous")
         else:
             points = np.ascontiguousarray(points, dtype=np.float64)
             weights = np.ascontiguousarray(weights)
+        _check_out_ids(part_ids, self.device)
         n = points.shape[0]
         if weights.shape[0] != n:
             raise InputLenMismatch(n, weights.shape[0])
@@ -427,7 +468,7 @@ class Server:
         tensor, or None to keep them on the mesh only) receives its ids."""
         if part_ids is not None and part_ids.shape[0] != mesh.n:
             raise InputLenMismatch(mesh.n, part_ids.shape[0])
-        dev_out = part_ids is not None and _is_device_tensor(part_ids)
+        dev_out = part_ids is not None and _check_out_ids(part_ids, self.device)
         optr = None if part_ids is None else (part_ids.data_ptr() if dev_out else part_ids.ctypes.data)
         stream = self._stream_of(part_ids) if dev_out else None
         req = _native.Request(None, None, 0, 0, 0, int(iter_count), float(tolerance), optr, 0,
@@ -490,6 +531,7 @@ class Mesh:
         if dev_in:
             _common_device(points=points, weights=weights)
             points, weights = points.contiguous(), weights.contiguous()
+            _check_device_points(points, server.device)
             ptr = lambda a: a.data_ptr()
             stream = Server._stream_of(points)
         else:
@@ -526,17 +568,21 @@ class Mesh:
         if dev_in:
             conv = lambda a: None if a is None else a.contiguous()
             ptr = lambda a: None if a is None else a.data_ptr()
-            stream = Server._stream_of(ref)
         else:
             conv = lambda a: None if a is None else np.ascontiguousarray(a)
             ptr = lambda a: None if a is None else a.ctypes.data
-            stream = None
         points, weights = conv(points), conv(weights)
-        if points is not None and (points.dtype != (np.float64 if not dev_in else points.dtype)
-                                   or points.shape[1] != self.dimension):
+        if dev_in:
+            for a in (points, weights):
+                if a is not None and (a.device.index or 0) != self.server.device:
+                    raise TypeError(f"the mesh lives on device {self.server.device}, an update on "
+                                    f"device {a.device.index or 0}")
+        if points is not None and (str(points.dtype).replace("torch.", "") != "float64"
+                                   or points.ndim != 2 or points.shape[1] != self.dimension):
             raise TypeError(f"points must be float64 of shape (k, {self.dimension})")
         if weights is not None and _weight_type(weights.dtype) != self.weight_type:
             raise TypeError("weights must keep the mesh's weight type")
+        stream = Server._stream_of(ref) if dev_in else None
         err = _native.lib().coupe_hip_mesh_update(self.server._s, self.handle, int(first), count, ptr(points),
                                                   ptr(weights), 1 if dev_in else 0, stream)
         if err != _native.OK:

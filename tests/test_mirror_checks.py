@@ -27,6 +27,13 @@ class FakeGpuTensor:
     def dim(self):
         return len(self.shape)
 
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
     def is_contiguous(self):
         return self._contiguous
 
@@ -112,3 +119,61 @@ def test_grid_context_on_another_device():
 def test_grid_cell_count():
     with pytest.raises(ValueError):
         GRID.rcb(FakeGpuTensor([N - 1], torch.int64), FakeGpuTensor([N], torch.float64), 2)
+
+
+def test_batch_ids_width():
+    with pytest.raises(TypeError, match="64-bit"):
+        coupe_amd.partition_batch(FakeGpuTensor([N], torch.int32), FakeGpuTensor([N, 3], torch.float64),
+                                  FakeGpuTensor([N], torch.int64), [(N, 2, 0.05)])
+    with pytest.raises(TypeError, match="64-bit"):
+        coupe_amd.partition_batch_mixed(FakeGpuTensor([N], torch.int32), FakeGpuTensor([3 * N], torch.float64),
+                                        FakeGpuTensor([N], torch.int64), [(N, 3, 2, 0.05)])
+
+
+def test_autotune_checks_like_the_batch():
+    pts = FakeGpuTensor([3 * N], torch.float64)
+    with pytest.raises(coupe_amd.InputLenMismatch):
+        coupe_amd.autotune(FakeGpuTensor([N - 1], torch.int64), pts, FakeGpuTensor([N], torch.int64),
+                           [(N, 3, 2, 0.05)])
+    with pytest.raises(TypeError, match="coordinates"):
+        coupe_amd.autotune(FakeGpuTensor([N], torch.int64), FakeGpuTensor([2 * N], torch.float64),
+                           FakeGpuTensor([N], torch.int64), [(N, 3, 2, 0.05)])
+    with pytest.raises(TypeError, match="context"):
+        coupe_amd.autotune(FakeGpuTensor([N], torch.int64), pts, FakeGpuTensor([N], torch.int64),
+                           [(N, 3, 2, 0.05)], context=FakeContext(1))
+
+
+def server_stub(device=0):
+    srv = object.__new__(coupe_amd.Server)   # the checks run before the server is used
+    srv._s, srv._keep, srv.device = None, {}, device
+    return srv
+
+
+def test_server_submit_checks():
+    srv = server_stub()
+    pts, w = np.zeros((N, 3)), np.ones(N)
+    for ids in (np.zeros(N, dtype=np.int32), np.zeros(2 * N, dtype=np.uint64)[::2],
+                FakeGpuTensor([N], torch.int32), FakeGpuTensor([N], torch.int64, index=1)):
+        with pytest.raises(TypeError):
+            srv.submit(ids, pts, w, 2, 0.05)
+    with pytest.raises(TypeError, match="float64"):
+        srv.submit(np.zeros(N, dtype=np.uint64), FakeGpuTensor([N, 3], torch.float32),
+                   FakeGpuTensor([N], torch.float64), 2, 0.05)
+    with pytest.raises(TypeError, match="device"):
+        server_stub(device=1).submit(np.zeros(N, dtype=np.uint64), FakeGpuTensor([N, 3], torch.float64),
+                                     FakeGpuTensor([N], torch.float64), 2, 0.05)
+
+
+def test_mesh_checks():
+    srv = server_stub()
+    with pytest.raises(TypeError, match="float64"):
+        coupe_amd.Mesh(srv, FakeGpuTensor([N, 3], torch.float32), FakeGpuTensor([N], torch.float64))
+    mesh = object.__new__(coupe_amd.Mesh)
+    mesh.server, mesh.n, mesh.dimension, mesh.handle = srv, N, 3, 1
+    mesh.weight_type = coupe_amd._native.COUPE_DOUBLE
+    with pytest.raises(TypeError, match="float64"):
+        mesh.update(points=FakeGpuTensor([4, 3], torch.float32))
+    with pytest.raises(TypeError, match="device"):
+        mesh.update(points=FakeGpuTensor([4, 3], torch.float64, index=1))
+    with pytest.raises(TypeError, match="64-bit"):
+        srv.submit_mesh(mesh, FakeGpuTensor([N], torch.int32), 2, 0.05)
