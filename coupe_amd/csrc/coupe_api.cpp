@@ -446,6 +446,7 @@ extern "C" enum coupe_err coupe_hip_grid_rcb(coupe_hip_ctx *ctx, void *stream, u
                                              uint64_t *d_partition) {
     if (!ctx || !size) return COUPE_ERR_CRASH;
     if (dimension != 2 && dimension != 3) return COUPE_ERR_BAD_DIMENSION;
+    if (!d_weights || !d_partition) return COUPE_ERR_CRASH;   // every grid has a cell
     Context *c = reinterpret_cast<Context *>(ctx);
     std::lock_guard<std::mutex> g(c->lock);
     const uint64_t sz[3] = {size[0], size[1], dimension == 3 ? size[2] : 1};

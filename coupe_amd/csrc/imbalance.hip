@@ -91,6 +91,7 @@ void loads_typed(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, c
 enum coupe_err parts_load(Context &cx, hipStream_t st, const uint64_t *ids, uint64_t n, int wkind, const void *w,
                           uint64_t num_parts, void *host_loads) {
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
+    if (n && (!ids || !w)) return COUPE_ERR_CRASH;
     try {
         ck(hipSetDevice(cx.device));
         uint64_t bad = 0;

@@ -54,6 +54,7 @@ enum coupe_err barycentres(Context &cx, hipStream_t st, const double *coords, ui
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (k == 0 || k > 64) return COUPE_ERR_BAD_TYPE;
     if (n_el == 0) return COUPE_ERR_OK;
+    if (!coords || !nodes || !out) return COUPE_ERR_CRASH;
     try {
         ck(hipSetDevice(cx.device));
         cx.loads.ensure(64);

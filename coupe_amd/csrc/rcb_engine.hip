@@ -1304,6 +1304,12 @@ enum coupe_err rcb_batch(Context &cx, hipStream_t st, uint64_t *d_out, size_t di
     if (dim != 2 && dim != 3) return COUPE_ERR_BAD_DIMENSION;
     if (wkind != COUPE_INT && wkind != COUPE_INT64 && wkind != COUPE_DOUBLE) return COUPE_ERR_BAD_TYPE;
     if (rot && src) return COUPE_ERR_CRASH;   // a rotation needs the points themselves
+    if (n_inst && !counts) return COUPE_ERR_CRASH;
+    uint64_t total = 0;
+    for (size_t q = 0; q < n_inst; q++) total += counts[q];
+    // a null array with points to read or ids to write (a resident mesh
+    // brings its own coordinates)
+    if (total && (!d_out || !d_weights || (!src && !d_points))) return COUPE_ERR_CRASH;
     TraceRange range(n_inst == 1 ? "coupe_hip rcb" : "coupe_hip rcb batch");
     memset(&cx.stats, 0, sizeof cx.stats);
     if (!rot) cx.prof.reset();

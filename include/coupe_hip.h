@@ -35,7 +35,9 @@ void coupe_hip_ctx_destroy(coupe_hip_ctx *ctx);
  *   stream      hipStream_t to run on (NULL: the null stream)
  *
  * Same semantics, errors and ids as coupe_rcb.  Returns when the ids are
- * written (the stream is synchronised).
+ * written (the stream is synchronised).  A null array with n > 0 returns
+ * COUPE_ERR_CRASH before any device work (so do the batched entries below
+ * when their instances hold points).
  * Replaces: Rcb::partition, coupe/src/algorithms/recursive_bisection.rs:805-811
  * (through rcb(), :644-705), for data already in device memory.
  */

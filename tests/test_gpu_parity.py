@@ -557,3 +557,33 @@ def test_batch_of_both_dimensions(oracle_mod, seed):
         assert np.array_equal(got[off:off + n], exp), f"instance {q}: n={n} D={d} L={iters} tol={tol}"
         off += n
         poff += n * d
+
+
+def test_null_arrays_refused():
+    """A null array with points to read or ids to write is refused before
+    any device work, by every entry that takes device arrays."""
+    import torch
+
+    lib = _native.lib()
+    ctx = coupe_amd._context_for(0)
+    n = 1000
+    pts = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    w = torch.ones(n, dtype=torch.float64, device="cuda")
+    out = torch.zeros(n, dtype=torch.int64, device="cuda")
+    P, W, O = pts.data_ptr(), w.data_ptr(), out.data_ptr()
+    for p, ww, o in [(None, W, O), (P, None, O), (P, W, None)]:
+        print(f"[gpu] rcb_device with a null array: {p is None, ww is None, o is None}", flush=True)
+        assert lib.coupe_hip_rcb_device(ctx.handle, None, o, n, 3, p, _native.COUPE_DOUBLE, ww, 4,
+                                        0.05) == _native.CRASH
+        inst = (_native.Instance * 1)()
+        inst[0].n, inst[0].iter_count, inst[0].tolerance = n, 4, 0.05
+        assert lib.coupe_hip_rcb_batch(ctx.handle, None, o, 3, p, _native.COUPE_DOUBLE, ww, inst,
+                                       1) == _native.CRASH
+    loads = (C.c_double * 4)()
+    assert lib.coupe_hip_parts_load(ctx.handle, None, None, n, _native.COUPE_DOUBLE, W, 4, loads) == _native.CRASH
+    imb = C.c_double()
+    assert lib.coupe_hip_imbalance(ctx.handle, None, O, n, _native.COUPE_DOUBLE, None, 4,
+                                   C.byref(imb)) == _native.CRASH
+    # no points: nothing to read or write, so null arrays are fine
+    assert lib.coupe_hip_rcb_device(ctx.handle, None, None, 0, 3, None, _native.COUPE_DOUBLE, None, 4,
+                                    0.05) == _native.OK
