@@ -33,7 +33,7 @@ namespace coupe_hip {
 // select exact integer accumulation.  Points are read through LDS so every
 // global load is contiguous.
 // ---------------------------------------------------------------------------
-constexpr int kProPoints = kBlock * 4;   // points per LDS transpose round
+constexpr int kProPoints = kBlock * 4;   // points per prologue round (4 per thread)
 
 // Per-instance bounding box and weight sums, reduced over the workgroup and
 // added to the instance with one atomic per field.
@@ -172,9 +172,9 @@ struct WeightScan {
     }
 };
 
-// Workgroups stride over the level-0 tiles (each inside one instance) and
-// read the AoS points of a tile through LDS so every global load is
-// contiguous.
+// Workgroups stride over the level-0 tiles (each inside one instance); each
+// lane reads its own points' AoS doubles (a wavefront's three loads per point
+// cover whole lines between them, so no LDS staging is needed).
 // With ROT every point p is replaced by rot.m p (Rib's obb_to_aabb map,
 // recursive_bisection.rs:850) before its coordinates are used.
 template <int D, typename WIN, bool ROT>
@@ -186,7 +186,6 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
     // w32, w16 (each optional): the weights as int32 and as 16-bit unsigned
     // integers; the latter are used when every weight is an integer in
     // [0, 65535] (not_u16 stays 0)
-    __shared__ double sp[kProPoints * D];
     __shared__ double s_d[kBlock / 64 * 8];
     __shared__ uint64_t s_u[kBlock / 64];
     const int tid = threadIdx.x;
@@ -213,23 +212,19 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
             const uint32_t cnt = T.count - r0 < (uint32_t)kProPoints ? T.count - r0 : (uint32_t)kProPoints;
             const double *src = ipts + (base - ibeg) * D;
             const WIN *wsrc = iwin + (base - ibeg);
-            // every load of the round issued back to back: the AoS doubles
-            // (clamped to the round's last one) and the points' weights.
-            // The caller's arrays are read once and the working arrays
-            // written once, each far larger than the caches: non-temporal
-            // loads and stores (2^27 points 3D: 1.23 -> 1.20 ms)
-            constexpr int kDv = kProPoints * D / kBlock;
-            const uint32_t nd = cnt * D;
-            double dv[kDv];
+            // every load of the round issued back to back (clamped to the
+            // round's last point).  The caller's arrays are read once and the
+            // working arrays written once, each far larger than the caches:
+            // non-temporal loads and stores (2^27 points 3D: 1.23 -> 1.20 ms)
+            double qv[4][D];
             WIN wv[4];
 #pragma unroll
-            for (int j = 0; j < kDv; j++) dv[j] = __builtin_nontemporal_load(&src[min((uint32_t)(j * kBlock + tid), nd - 1u)]);
+            for (int k = 0; k < 4; k++) {
+                const uint32_t pc = min((uint32_t)(k * kBlock + tid), cnt - 1u);
 #pragma unroll
-            for (int k = 0; k < 4; k++) wv[k] = __builtin_nontemporal_load(&wsrc[min((uint32_t)(k * kBlock + tid), cnt - 1u)]);
-#pragma unroll
-            for (int j = 0; j < kDv; j++)
-                if ((uint32_t)(j * kBlock + tid) < nd) sp[j * kBlock + tid] = dv[j];
-            __syncthreads();
+                for (int a = 0; a < D; a++) qv[k][a] = __builtin_nontemporal_load(&src[(uint64_t)pc * D + a]);
+                wv[k] = __builtin_nontemporal_load(&wsrc[pc]);
+            }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const uint32_t p = k * kBlock + tid;
@@ -237,7 +232,7 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                     const uint64_t i = base + p;
                     double q[D];
 #pragma unroll
-                    for (int a = 0; a < D; a++) q[a] = sp[p * D + a];
+                    for (int a = 0; a < D; a++) q[a] = qv[k][a];
                     if constexpr (ROT) {
                         double r[D];
 #pragma unroll
@@ -260,7 +255,6 @@ k_prologue(const Tile *__restrict__ tiles, uint32_t n_tiles, const double *__res
                     ws.add(wv[k], i, w32, w16);
                 }
             }
-            __syncthreads();
         }
     }
     if (cur != 0xffffffffu) flush_instance<D>(&red[(blockIdx.x % kCopies) * n_inst + cur], mn, mx, ws.su, ws.sf, s_d, s_u);
