@@ -145,7 +145,7 @@ struct Context {
     bool graphs = false;             // replay a call of a known shape as one captured graph
     uint32_t copy_parts = 256;       // levels of at most this many parts flush into per-XCD copies
     uint32_t search_chunk = 2;       // consecutive tiles a workgroup takes at a time in the search passes
-    double move_nt_bytes = 2.5e9;    // moves of more bytes than this store non-temporally
+    double move_nt_bytes = 2.5e9;    // moves and id descents of more bytes than this go non-temporal
     int max_bins = kMaxBins;         // bins per part of the later search passes (16..256, a power of two)
     int wg_cap = 0;                  // workgroups per CU of the persistent kernels (0: as many as fit)
     // tuned switches per batch class (tune_key), chosen by coupe_hip_autotune

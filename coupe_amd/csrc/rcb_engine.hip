@@ -879,12 +879,20 @@ static bool run_levels(Context &cx, hipStream_t st, const BatchPlan &B, int wrap
 
     // Part ids in input order (:589-601, :698-702).
     if (dense) {
-        const unsigned g = persistent_grid(cx, k_descend_dense<D>, B.tiles0);
-        // reads the input-order f32 coordinates, writes the 8-byte id
-        timed(cx, st, COUPE_HIP_K_IDS, (4.0 * D + 8.0) * (double)n, [&] {
-            k_descend_dense<D><<<g, kBlock, 0, st>>>(tiles0, (uint32_t)B.tiles0, insts,
-                                                     heap_split, cs[0][0], cs[0][1], cs[0][2], d_out);
-        });
+        // reads the input-order f32 coordinates, writes the 8-byte id;
+        // non-temporal above the moves' size threshold (2^27 points 3D:
+        // 0.521 -> 0.495 ms, and the next call's first pass 0.604 -> 0.578;
+        // 2^24 points: 0.060 -> 0.063 ms, so cached there)
+        const double bytes = (4.0 * D + 8.0) * (double)n;
+        auto descend = [&](auto kernel) {
+            const unsigned g = persistent_grid(cx, kernel, B.tiles0);
+            timed(cx, st, COUPE_HIP_K_IDS, bytes, [&] {
+                kernel<<<g, kBlock, 0, st>>>(tiles0, (uint32_t)B.tiles0, insts, heap_split, cs[0][0], cs[0][1],
+                                             cs[0][2], d_out);
+            });
+        };
+        if (bytes > cx.move_nt_bytes) descend(k_descend_dense<D, true>);
+        else descend(k_descend_dense<D, false>);
     } else {
         // single instance, deep tree: node records and leaf ids
         Part *parts = (Part *)cx.parts[cur].ptr;

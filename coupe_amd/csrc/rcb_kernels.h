@@ -1831,7 +1831,9 @@ constexpr int kDenseLevels = 24;
 constexpr int kDenseCached = 12;   // 4095 splits, 16 KB of LDS
 
 // Six wavefronts per SIMD (72 VGPRs): more loads in flight than at five.
-template <int D>
+// NT: non-temporal loads of the coordinates and stores of the ids (chosen
+// by the host for large calls, where the caches cannot hold either).
+template <int D, bool NT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
 k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance *__restrict__ insts,
                 const float *__restrict__ heap_split, const float *__restrict__ c0, const float *__restrict__ c1,
@@ -1864,9 +1866,15 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
         for (int k = 0; k < kItems; k++) {
             const uint32_t p = k * kBlock + threadIdx.x;
             const uint64_t c = T.begin + (p < T.count ? p : T.count - 1);   // clamped, unconditional loads
-            x0[k] = c0[c];
-            x1[k] = c1[c];
-            x2[k] = D == 3 ? c2[c] : 0.f;
+            if constexpr (NT) {
+                x0[k] = __builtin_nontemporal_load(&c0[c]);
+                x1[k] = __builtin_nontemporal_load(&c1[c]);
+                x2[k] = D == 3 ? __builtin_nontemporal_load(&c2[c]) : 0.f;
+            } else {
+                x0[k] = c0[c];
+                x1[k] = c1[c];
+                x2[k] = D == 3 ? c2[c] : 0.f;
+            }
             h[k] = 0;
         }
         // D levels at a time from the root: their axes are 0..D-1, so each
@@ -1900,7 +1908,11 @@ k_descend_dense(const Tile *__restrict__ tiles, uint32_t n_tiles, const Instance
 #pragma unroll
         for (int k = 0; k < kItems; k++) {
             const uint32_t p = k * kBlock + threadIdx.x;
-            if (p < T.count) out[src + (T.begin - ibeg) + p] = (uint64_t)h[k] - off;
+            if (p < T.count) {
+                uint64_t *o = &out[src + (T.begin - ibeg) + p];
+                if constexpr (NT) __builtin_nontemporal_store((uint64_t)h[k] - off, o);
+                else *o = (uint64_t)h[k] - off;
+            }
         }
     }
 }
