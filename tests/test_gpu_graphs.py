@@ -240,8 +240,10 @@ def test_level_with_no_queued_pass_reruns_when_new_data_needs_one(oracle_mod):
 def test_move_store_modes(oracle_mod, monkeypatch, nt_bytes):
     """The move stores non-temporally when its size passes the switch and the
     shape's last call moved tiles to few destinations (grid order), else
-    through the caches: both forms give the oracle's ids, on data in grid
-    order and in random order, on the first and the later calls of a shape."""
+    through the caches, and the id descent reads and writes non-temporally
+    when its size passes the switch: both forms give the oracle's ids, on
+    data in grid order and in random order, on the first and the later calls
+    of a shape."""
     torch = pytest.importorskip("torch")
     monkeypatch.setenv("COUPE_HIP_MOVE_NT_BYTES", nt_bytes)
     from coupe_amd import workloads
